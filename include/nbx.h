@@ -1,0 +1,204 @@
+/*
+ * nbx.h -- C ABI of the MI355X (gfx950) nekBench solver hot path.
+ *
+ * Three drop-in boundaries, each replacing one reference interface:
+ *   1. axhelm kernels      <- loadAxKernel + axKernel(Nelements, offset, ggeo, D, lambda, q, Aq)
+ *   2. ogs gather-scatter  <- ogsSetup / ogsGatherScatterMany / oogs::start+finish
+ *   3. BP driver           <- setup / BPPCG / BPAxOperator of nekBone
+ *
+ * Conventions (shared by every entry point):
+ *   - plain C types only; all array arguments are device pointers unless the
+ *     comment says "host";
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream);
+ *     work is enqueued asynchronously on it, as the reference enqueues on the
+ *     OCCA device's current stream;
+ *   - the caller owns every buffer it passes; handles own what they allocate;
+ *   - return value 0 = success, < 0 = error (see NBX_ERR_*); nbx_last_error()
+ *     gives the message.  No entry point calls exit();
+ *   - dlong is int32 (reference makefile:40, dlong=int), hlong is int64
+ *     (makefile:45); ggeo and element offsets are indexed in 64 bits inside.
+ *   - layout: element-major, node index i + j*Nq + k*Nq^2
+ *     (axhelm/kernel/CPU/axhelm.c:52); ggeo[e][7][Np] with factor order
+ *     GWJ, G00, G01, G02, G11, G12, G22 (core/params.h:55-62); field f of a
+ *     multi-field vector at f*offset.
+ */
+#ifndef NBX_H
+#define NBX_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t nbx_dlong;
+typedef int64_t nbx_hlong;
+
+#define NBX_SUCCESS 0
+#define NBX_ERR_INVALID (-1)     /* bad argument */
+#define NBX_ERR_UNSUPPORTED (-2) /* combination not built (e.g. N > 15) */
+#define NBX_ERR_HIP (-3)         /* HIP runtime error */
+#define NBX_ERR_ALLOC (-4)       /* allocation failed */
+#define NBX_ERR_DIVERGED (-5)    /* PCG breakdown (non-finite or non-positive curvature) */
+
+typedef enum { NBX_F64 = 0, NBX_F32 = 1, NBX_BF16 = 2, NBX_F16 = 3 } nbx_prec;
+/* ogs types / ops (3rdParty/ogs/ogs.hpp:128-140: "double", "float", "int",
+ * "long long int"; "add", "mul", "min", "max") */
+typedef enum { NBX_DOUBLE = 0, NBX_FLOAT = 1, NBX_INT = 2, NBX_LONG = 3 } nbx_type;
+typedef enum { NBX_ADD = 0, NBX_MUL = 1, NBX_MIN = 2, NBX_MAX = 3 } nbx_op;
+
+const char* nbx_last_error(void);
+int nbx_version(void);
+int nbx_device_synchronize(void);
+
+/* ======================= 1. axhelm ======================================
+ * Replaces the kernels loaded by loadAxKernel (axhelm/kernelHelper.cpp:3-58),
+ * called as kernel(Nelements, offset, ggeo, D, lambda, q, Aq)
+ * (axhelm/main.cpp:132, nekBone/solve.cpp:218) and, for the Partial forms,
+ * kernel(Nelements, offset, elementList, ggeo, D, lambda, q, Aq)
+ * (axhelm/kernel/VOLTA/axhelm.okl:158-165, nekBone/solve.cpp:192-212).
+ *
+ *   Aq = D^T (lambda0 . G D q) + lambda1 . GWJ . q      (bk = 0, axhelm_v0)
+ *   Aq = D^T (G D q)                                    (bk = 1, axhelm_bk_v0)
+ * for Nfields = 1 or 3 (_n3: the same ggeo and lambda for every field).
+ * lambda[0..offset) is lambda0, lambda[offset..2 offset) lambda1 (per node),
+ * ignored when bk = 1 (may be NULL then).  q and Aq must not overlap; Aq is
+ * overwritten.  elementList == NULL processes elements 0..Nelements-1.
+ * prec selects the arithmetic and the element type of ggeo, D, lambda, q, Aq:
+ * NBX_F64 (double), NBX_F32 (float).  N in [1, 15]. */
+int nbx_axhelm(int N, int Nfields, int bk, nbx_prec prec,
+               nbx_dlong Nelements, nbx_dlong offset, const nbx_dlong* elementList,
+               const void* ggeo, const void* D, const void* lambda,
+               const void* q, void* Aq, void* stream);
+
+/* Same, with the kernel chosen by the reference's name grammar
+ * "axhelm[Partial][_bk][_n3]_v<k>" (nekBone/setup.cpp:313-320,
+ * axhelm/main.cpp:105-109), FP64.  Partial names require elementList. */
+int nbx_axhelm_by_name(const char* kernelName, int N,
+                       nbx_dlong Nelements, nbx_dlong offset, const nbx_dlong* elementList,
+                       const double* ggeo, const double* D, const double* lambda,
+                       const double* q, double* Aq, void* stream);
+
+/* Local Jacobi diagonal of lambda0*A + lambda1*B (nekBone/kernel/updateJacobi.okl:27-108);
+ * mapB (may be NULL) marks nodes whose diagonal is 1. */
+int nbx_update_jacobi_local(int N, nbx_dlong Nelements, nbx_dlong offset, const int32_t* mapB,
+                            const double* ggeo, const double* D, const double* lambda,
+                            double* diagA, void* stream);
+
+/* ======================= 2. ogs gather-scatter ==========================
+ * Replaces ogsSetup (3rdParty/ogs/src/ogsSetup.cpp:68-355) for one rank and
+ * ogsGatherScatterMany / oogs::start+finish
+ * (3rdParty/ogs/src/ogsGatherScatterMany.cpp:54-183, oogs.cpp:248-342).
+ * ids (host, length N): nodes sorted by (|id|, local index) with id 0
+ * excluded; a group is a run of equal signed id in that order (ogsSetup.cpp:163;
+ * all-positive ids simply group by value).  Groups and member order are exactly
+ * those of ogsSetup: groups in order of first appearance by local index,
+ * members ascending.  The
+ * gather-scatter combines the members of every group of two or more nodes
+ * in that order, starting from the op's identity, and writes the result to
+ * each member; singletons and excluded nodes are left untouched
+ * (3rdParty/ogs/okl/gatherScatterMany.okl:58-86). */
+typedef struct nbx_ogs nbx_ogs;
+nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose);
+int nbx_ogs_gather_scatter_many(nbx_ogs* ogs, void* v, int k, nbx_dlong stride,
+                                nbx_type type, nbx_op op, void* stream);
+int nbx_ogs_gather_scatter(nbx_ogs* ogs, void* v, nbx_type type, nbx_op op, void* stream);
+/* 1/multiplicity per local node, 1 for excluded nodes (ogsSetup.cpp:325-352); device */
+const double* nbx_ogs_inv_degree(const nbx_ogs* ogs);
+/* sizes: N, Nlocal (ids != 0), NlocalGather (groups), groups with >= 2 members,
+ * nodes in groups with >= 2 members (host outputs, any may be NULL) */
+int nbx_ogs_sizes(const nbx_ogs* ogs, nbx_dlong* N, nbx_dlong* Nlocal, nbx_dlong* NlocalGather,
+                  nbx_dlong* NsharedGather, nbx_dlong* Nshared);
+/* host copies of localGatherOffsets[NlocalGather+1] and localGatherIds[Nlocal] */
+int nbx_ogs_copy_csr(const nbx_ogs* ogs, nbx_dlong* offsets, nbx_dlong* ids);
+void nbx_ogs_free(nbx_ogs* ogs);
+
+/* Dirichlet mask: q[maskIds[n]] = 0 (nekBone/kernel/utils.okl:127-134) */
+int nbx_mask(nbx_dlong Nmasked, const nbx_dlong* maskIds, double* q, void* stream);
+
+/* ======================= mesh inputs (host) ==============================
+ * Produce the hot path's inputs as the reference's mesh setup does
+ * (core/meshSetup.cpp:2265-2451).  All arrays here are HOST memory. */
+typedef struct {
+  int N;                       /* polynomial degree */
+  int NX, NY, NZ;              /* elements per direction */
+  int periodic;                /* 1: periodic box as the reference's nekBone */
+  double perturb;              /* interior vertex noise, fraction of element width */
+  double zaspect;              /* 0: uniform z; > 0: geometric z grading, first layer hx/zaspect */
+  uint64_t seed;
+  double x0, x1, y0, y1, z0, z1;
+} nbx_box;
+
+/* GLL nodes and weights (core/meshBasis.cpp:140-176) and D[i*Nq+j] = l_j'(z_i)
+ * (core/meshBasis.cpp:775-790) */
+int nbx_gll(int N, double* z, double* w, double* D);
+/* uniform [-1,1) stream, counter-based splitmix64 */
+int nbx_urand_fill(uint64_t seed, uint64_t stream, int64_t n, double* out);
+/* Element vertices, nodes (x, y, z may be NULL), ggeo and global ids
+ * (ids may be NULL; mask_boundary zeroes boundary ids for Dirichlet).
+ * Returns the minimum Jacobian through minJ (may be NULL). */
+int nbx_box_mesh(const nbx_box* box, int mask_boundary, double* x, double* y, double* z,
+                 double* ggeo, nbx_hlong* globalIds, double* minJ);
+
+/* ======================= BLAS-1 of the PCG loop ==========================
+ * nekBone/kernel/utils.okl.  Reductions return their value through a device
+ * (result_dev) or host (result_host) pointer; results are deterministic: a
+ * fixed tree independent of timing (tiles of 256 lanes x 8 values, lane
+ * fma chain, per-wave halving tree, (S0+S2)+(S1+S3), partials combined the
+ * same way in a last-block pass). */
+int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
+                               const double* x, const double* y, double* result_host, void* stream);
+int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream);
+
+/* ======================= 3. BP driver ====================================
+ * Replaces nekBone's setup()/solveSetup() (nekBone/setup.cpp:36-367),
+ * BPPCG (nekBone/solve.cpp:44-113) and BPAxOperator (solve.cpp:181-235) on
+ * one GPU.  The handle owns the mesh (D, ggeo, gather-scatter), the Jacobi
+ * diagonal and the PCG workspace; r and x are the caller's. */
+typedef struct nbx_bp nbx_bp;
+
+typedef enum { NBX_PRECON_COPY = 0, NBX_PRECON_JACOBI = 1 } nbx_precon;
+
+typedef struct {
+  nbx_box box;
+  int Nfields;          /* 1 or 3 */
+  int bpmode;           /* 1: BK kernels (BP5/BP6), lambda1 forced to 0 (setup.cpp:43-53) */
+  double lambda1;       /* Helmholtz shift (non-BK) */
+  int dirichlet;        /* 1: zero boundary nodes (ids 0 + mask) */
+  nbx_precon precon;
+  int flexible;         /* "PCG+FLEXIBLE": beta = -alpha (z.Ap)/rdotz_old (solve.cpp:92-95) */
+  int verbose;
+  int rhs_kind;         /* 0: U[-1,1) per local node (seed stream 3+field), masked, assembled;
+                           1: (3 pi^2 + lambda1) JW cos(pi x) cos(pi y) cos(pi z), assembled
+                              (nekBone/setup.cpp:77-99) */
+} nbx_bp_options;
+
+typedef struct {
+  int iterations;       /* BPPCG's return value (iter - 1) */
+  int converged;
+  double rdotr0;        /* weighted |r|^2 at iteration 1 */
+  double rdotr;         /* weighted |r|^2 at exit */
+  double elapsed_s;     /* wall time of the solve on the device */
+} nbx_pcg_stats;
+
+nbx_bp* nbx_bp_setup(const nbx_bp_options* opt, void* stream);
+void nbx_bp_free(nbx_bp* bp);
+/* sizes and device arrays owned by the handle:
+ * name in {"ggeo","D","lambda","invDegree","invDiagA","rhs","maskIds"} */
+int nbx_bp_sizes(const nbx_bp* bp, nbx_dlong* Nelements, nbx_dlong* Np, nbx_dlong* fieldOffset,
+                 nbx_dlong* Nmasked);
+const void* nbx_bp_device_array(const nbx_bp* bp, const char* name);
+nbx_ogs* nbx_bp_ogs(nbx_bp* bp);
+/* BPAxOperator: Aq = gs(axhelm(q)) (+ mask); returns pAp = <q, Aq>_w through pAp_host */
+int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double* q, double* Aq, double* pAp_host);
+/* BPPCG: solves A x = r in place (r becomes the residual), tol relative,
+ * stopping test rdotr <= max(tol^2 rdotr_1, tol^2) (solve.cpp:85-90). */
+int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
+               nbx_pcg_stats* stats);
+/* per-iteration history of the last solve (host outputs, length >= iterations+1):
+ * rdotr and alpha per iteration; returns the count written */
+int nbx_bp_history(const nbx_bp* bp, double* rdotr, double* alpha, int capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,341 @@
+"""nekbench_amd -- MI355X (gfx950) nekBench solver hot path.
+
+Python view of the C ABI declared in include/nbx.h (libnbx.so, built in-tree
+by nekbench_amd/build.py).  Device buffers are passed as torch tensors on a
+ROCm device (only their data pointers cross the boundary); host buffers as
+numpy arrays.  There is no fallback: if libnbx.so is missing or a call fails,
+an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libnbx.so"
+
+NBX_F64, NBX_F32, NBX_BF16, NBX_F16 = 0, 1, 2, 3
+NBX_DOUBLE, NBX_FLOAT, NBX_INT, NBX_LONG = 0, 1, 2, 3
+NBX_ADD, NBX_MUL, NBX_MIN, NBX_MAX = 0, 1, 2, 3
+PRECON_COPY, PRECON_JACOBI = 0, 1
+# reference ggeo factor order (core/params.h:55-62)
+GWJID, G00ID, G01ID, G02ID, G11ID, G12ID, G22ID, NGGEO = 0, 1, 2, 3, 4, 5, 6, 7
+
+_lib = None
+
+
+class NbxError(RuntimeError):
+    pass
+
+
+class Box(C.Structure):
+    """nbx_box: box mesh description (include/nbx.h)."""
+    _fields_ = [("N", C.c_int), ("NX", C.c_int), ("NY", C.c_int), ("NZ", C.c_int),
+                ("periodic", C.c_int), ("perturb", C.c_double), ("zaspect", C.c_double),
+                ("seed", C.c_uint64),
+                ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double), ("y1", C.c_double),
+                ("z0", C.c_double), ("z1", C.c_double)]
+
+    @classmethod
+    def make(cls, N, NX, NY, NZ, periodic=0, perturb=0.0, zaspect=0.0, seed=1,
+             extent=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+        return cls(N, NX, NY, NZ, periodic, perturb, zaspect, seed, *extent)
+
+    @property
+    def Nelements(self):
+        return self.NX * self.NY * self.NZ
+
+    @property
+    def Np(self):
+        return (self.N + 1) ** 3
+
+
+class BPOptions(C.Structure):
+    """nbx_bp_options (include/nbx.h)."""
+    _fields_ = [("box", Box), ("Nfields", C.c_int), ("bpmode", C.c_int), ("lambda1", C.c_double),
+                ("dirichlet", C.c_int), ("precon", C.c_int), ("flexible", C.c_int), ("verbose", C.c_int),
+                ("rhs_kind", C.c_int)]
+
+
+class PCGStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("rdotr0", C.c_double),
+                ("rdotr", C.c_double), ("elapsed_s", C.c_double)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_SIGS = {
+    "nbx_last_error": (C.c_char_p, []),
+    "nbx_version": (_I, []),
+    "nbx_device_synchronize": (_I, []),
+    "nbx_axhelm": (_I, [_I, _I, _I, _I, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "nbx_axhelm_by_name": (_I, [C.c_char_p, _I, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
+    "nbx_update_jacobi_local": (_I, [_I, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P]),
+    "nbx_ogs_setup": (_P, [C.c_int32, _P, _I]),
+    "nbx_ogs_gather_scatter_many": (_I, [_P, _P, _I, C.c_int32, _I, _I, _P]),
+    "nbx_ogs_gather_scatter": (_I, [_P, _P, _I, _I, _P]),
+    "nbx_ogs_inv_degree": (_P, [_P]),
+    "nbx_ogs_sizes": (_I, [_P, _P, _P, _P, _P, _P]),
+    "nbx_ogs_copy_csr": (_I, [_P, _P, _P]),
+    "nbx_ogs_free": (None, [_P]),
+    "nbx_mask": (_I, [C.c_int32, _P, _P, _P]),
+    "nbx_gll": (_I, [_I, _P, _P, _P]),
+    "nbx_urand_fill": (_I, [C.c_uint64, C.c_uint64, C.c_int64, _P]),
+    "nbx_box_mesh": (_I, [C.POINTER(Box), _I, _P, _P, _P, _P, _P, _P]),
+    "nbx_weighted_inner_product": (_I, [C.c_int32, C.c_int32, _I, _P, _P, _P, _P, _P]),
+    "nbx_scaled_add": (_I, [C.c_int32, _D, _P, _D, _P, _P]),
+    "nbx_bp_setup": (_P, [C.POINTER(BPOptions), _P]),
+    "nbx_bp_free": (None, [_P]),
+    "nbx_bp_sizes": (_I, [_P, _P, _P, _P, _P]),
+    "nbx_bp_device_array": (_P, [_P, C.c_char_p]),
+    "nbx_bp_ogs": (_P, [_P]),
+    "nbx_bp_ax_operator": (_I, [_P, _P, _P, _P, _P]),
+    "nbx_bp_pcg": (_I, [_P, _P, _P, _P, _D, _I, C.POINTER(PCGStats)]),
+    "nbx_bp_history": (_I, [_P, _P, _P, _I]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libnbx.so (built by nekbench_amd.build); raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NbxError(f"{LIB_PATH} not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        l = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def last_error() -> str:
+    return lib().nbx_last_error().decode()
+
+
+def check(rc: int, what: str = "nbx call") -> None:
+    if rc != 0:
+        raise NbxError(f"{what} failed ({rc}): {last_error()}")
+
+
+def ptr(t) -> int | None:
+    """Device (torch) or host (numpy) data pointer; None stays NULL."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        assert t.flags["C_CONTIGUOUS"]
+        return t.ctypes.data
+    assert t.is_contiguous()
+    return t.data_ptr()
+
+
+def stream_ptr(stream=None) -> int | None:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+# ---------------------------------------------------------------- host setup
+def gll(N: int):
+    z = np.zeros(N + 1)
+    w = np.zeros(N + 1)
+    D = np.zeros((N + 1) * (N + 1))
+    check(lib().nbx_gll(N, ptr(z), ptr(w), ptr(D)), "nbx_gll")
+    return z, w, D
+
+
+def urand(seed: int, stream: int, n: int) -> np.ndarray:
+    out = np.empty(n)
+    check(lib().nbx_urand_fill(seed, stream, n, ptr(out)), "nbx_urand_fill")
+    return out
+
+
+def box_mesh(box: Box, mask_boundary: bool = False, nodes: bool = False, ids: bool = True):
+    """Host mesh inputs: dict with ggeo [E*7*Np], ids [E*Np] (int64), x,y,z."""
+    E, Np = box.Nelements, box.Np
+    out = {"ggeo": np.empty(E * NGGEO * Np)}
+    if nodes:
+        for c in "xyz":
+            out[c] = np.empty(E * Np)
+    if ids:
+        out["ids"] = np.empty(E * Np, dtype=np.int64)
+    minJ = C.c_double(0.0)
+    check(lib().nbx_box_mesh(C.byref(box), int(mask_boundary), ptr(out.get("x")), ptr(out.get("y")),
+                             ptr(out.get("z")), ptr(out["ggeo"]), ptr(out.get("ids")), C.byref(minJ)),
+          "nbx_box_mesh")
+    out["minJ"] = minJ.value
+    return out
+
+
+# ---------------------------------------------------------------- device ops
+def axhelm(N, Nfields, bk, Nelements, offset, ggeo, D, lam, q, Aq, elementList=None, prec=NBX_F64, stream=None):
+    check(lib().nbx_axhelm(N, Nfields, int(bk), prec, Nelements, offset, ptr(elementList), ptr(ggeo), ptr(D),
+                           ptr(lam), ptr(q), ptr(Aq), stream_ptr(stream)), "nbx_axhelm")
+
+
+def axhelm_by_name(name, N, Nelements, offset, ggeo, D, lam, q, Aq, elementList=None, stream=None):
+    check(lib().nbx_axhelm_by_name(name.encode(), N, Nelements, offset, ptr(elementList), ptr(ggeo), ptr(D),
+                                   ptr(lam), ptr(q), ptr(Aq), stream_ptr(stream)), f"nbx_axhelm_by_name({name})")
+
+
+def update_jacobi_local(N, Nelements, offset, ggeo, D, lam, diag, mapB=None, stream=None):
+    check(lib().nbx_update_jacobi_local(N, Nelements, offset, ptr(mapB), ptr(ggeo), ptr(D), ptr(lam), ptr(diag),
+                                        stream_ptr(stream)), "nbx_update_jacobi_local")
+
+
+def mask(maskIds, q, stream=None):
+    check(lib().nbx_mask(maskIds.numel(), ptr(maskIds), ptr(q), stream_ptr(stream)), "nbx_mask")
+
+
+def weighted_inner_product(N, offset, Nfields, w, x, y, stream=None) -> float:
+    out = C.c_double(0.0)
+    check(lib().nbx_weighted_inner_product(N, offset, Nfields, ptr(w), ptr(x), ptr(y), C.byref(out),
+                                           stream_ptr(stream)), "nbx_weighted_inner_product")
+    return out.value
+
+
+def scaled_add(N, alpha, x, beta, y, stream=None):
+    check(lib().nbx_scaled_add(N, alpha, ptr(x), beta, ptr(y), stream_ptr(stream)), "nbx_scaled_add")
+
+
+class Ogs:
+    """Gather-scatter handle (nbx_ogs_setup / nbx_ogs_gather_scatter_many)."""
+
+    _types = {"double": NBX_DOUBLE, "float": NBX_FLOAT, "int": NBX_INT, "long long int": NBX_LONG}
+    _ops = {"add": NBX_ADD, "mul": NBX_MUL, "min": NBX_MIN, "max": NBX_MAX}
+
+    def __init__(self, ids: np.ndarray, verbose: bool = False, handle=None):
+        self._owned = handle is None
+        if handle is None:
+            ids = np.ascontiguousarray(ids, dtype=np.int64)
+            handle = lib().nbx_ogs_setup(ids.size, ptr(ids), int(verbose))
+            if not handle:
+                raise NbxError(f"nbx_ogs_setup failed: {last_error()}")
+        self.h = handle
+
+    def sizes(self):
+        v = [C.c_int32(0) for _ in range(5)]
+        check(lib().nbx_ogs_sizes(self.h, *[C.byref(x) for x in v]), "nbx_ogs_sizes")
+        return dict(zip(["N", "Nlocal", "NlocalGather", "NsharedGather", "Nshared"], [x.value for x in v]))
+
+    def csr(self):
+        s = self.sizes()
+        off = np.zeros(s["NlocalGather"] + 1, dtype=np.int32)
+        ids = np.zeros(max(s["Nlocal"], 1), dtype=np.int32)
+        check(lib().nbx_ogs_copy_csr(self.h, ptr(off), ptr(ids)), "nbx_ogs_copy_csr")
+        return off, ids[: s["Nlocal"]]
+
+    def gather_scatter_many(self, v, k, stride, type_="double", op="add", stream=None):
+        check(lib().nbx_ogs_gather_scatter_many(self.h, ptr(v), k, stride, self._types[type_], self._ops[op],
+                                                stream_ptr(stream)), "nbx_ogs_gather_scatter_many")
+
+    def gather_scatter(self, v, type_="double", op="add", stream=None):
+        check(lib().nbx_ogs_gather_scatter(self.h, ptr(v), self._types[type_], self._ops[op], stream_ptr(stream)),
+              "nbx_ogs_gather_scatter")
+
+    def inv_degree_ptr(self) -> int:
+        return lib().nbx_ogs_inv_degree(self.h)
+
+    def free(self):
+        if self.h and self._owned:
+            lib().nbx_ogs_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BP:
+    """BP driver handle: nekBone's setup + BPPCG + BPAxOperator on one GPU
+    (nbx_bp_setup / nbx_bp_pcg / nbx_bp_ax_operator)."""
+
+    def __init__(self, box: Box, Nfields=1, bpmode=False, lambda1=0.0, dirichlet=True, precon=PRECON_JACOBI,
+                 flexible=False, verbose=False, rhs_kind=0, stream=None):
+        self.opt = BPOptions(box, Nfields, int(bpmode), lambda1, int(dirichlet), precon, int(flexible),
+                             int(verbose), rhs_kind)
+        self.h = lib().nbx_bp_setup(C.byref(self.opt), stream_ptr(stream))
+        if not self.h:
+            raise NbxError(f"nbx_bp_setup failed: {last_error()}")
+        v = [C.c_int32(0) for _ in range(4)]
+        check(lib().nbx_bp_sizes(self.h, *[C.byref(x) for x in v]), "nbx_bp_sizes")
+        self.Nelements, self.Np, self.fieldOffset, self.Nmasked = [x.value for x in v]
+        self.Nfields = Nfields
+        self.Nlocal = self.Nelements * self.Np
+
+    def device_array(self, name, n, dtype=None):
+        """torch view (no copy) of a device array owned by the handle."""
+        import torch
+        addr = lib().nbx_bp_device_array(self.h, name.encode())
+        if not addr:
+            raise NbxError(last_error())
+        dtype = dtype or (torch.int32 if name == "maskIds" else torch.float64)
+        return _wrap_device(addr, n, dtype)
+
+    def ogs(self) -> Ogs:
+        return Ogs(None, handle=lib().nbx_bp_ogs(self.h))
+
+    def ax_operator(self, q, Aq, lam=None) -> float:
+        pAp = C.c_double(0.0)
+        check(lib().nbx_bp_ax_operator(self.h, ptr(lam), ptr(q), ptr(Aq), C.byref(pAp)), "nbx_bp_ax_operator")
+        return pAp.value
+
+    def pcg(self, r, x, tol, maxit, lam=None) -> PCGStats:
+        st = PCGStats()
+        rc = lib().nbx_bp_pcg(self.h, ptr(lam), ptr(r), ptr(x), tol, maxit, C.byref(st))
+        if rc < 0:
+            raise NbxError(f"nbx_bp_pcg failed ({rc}): {last_error()}")
+        return st
+
+    def history(self, n):
+        rd = np.zeros(n)
+        al = np.zeros(n)
+        k = lib().nbx_bp_history(self.h, ptr(rd), ptr(al), n)
+        if k < 0:
+            raise NbxError(last_error())
+        return rd[:k], al[:k]
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().nbx_bp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _wrap_device(addr: int, n: int, dtype):
+    """Torch tensor over device memory owned by a library handle: a
+    non-owning view through __cuda_array_interface__, or, if this torch build
+    cannot import one, a device-to-device copy (hipMemcpy)."""
+    import torch
+    typestr = {torch.float64: "<f8", torch.int32: "<i4", torch.float32: "<f4"}[dtype]
+
+    class _Holder:
+        __cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (addr, False), "version": 3,
+                                    "strides": None}
+    try:
+        t = torch.as_tensor(_Holder(), device="cuda")
+        if t.data_ptr() == addr and t.dtype == dtype and t.numel() == n:
+            return t
+    except Exception:
+        pass
+    out = torch.empty(n, dtype=dtype, device="cuda")
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    torch.cuda.synchronize()
+    rc = hip.hipMemcpy(out.data_ptr(), addr, n * out.element_size(), 3)  # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise NbxError(f"hipMemcpy failed ({rc})")
+    return out

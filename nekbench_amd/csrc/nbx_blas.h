@@ -1,0 +1,22 @@
+// Internal interface of the reduction kernels (nbx_blas.hip).
+#pragma once
+#include "nbx_reduce.h"
+
+namespace nbx {
+
+// Device scratch of a deterministic reduction: one partial per tile and value,
+// an arrival ticket (zero between launches) and a small result slot.
+struct ReduceWork {
+  double* partials = nullptr;
+  unsigned int* ticket = nullptr;
+  double* result = nullptr;
+  unsigned int capacity = 0;  // tiles
+  int init(int64_t maxN, int nvals);
+  void release();
+};
+
+// out_dev[0] = <x,y>_w, and out_dev[1] = <x,x>_w when xx (device pointers)
+int launch_wdot(int32_t N, int32_t offset, int Nf, const double* w, const double* x, const double* y, bool xx,
+                ReduceWork& ws, double* out_dev, hipStream_t s);
+
+}  // namespace nbx

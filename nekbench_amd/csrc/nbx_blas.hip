@@ -1,0 +1,127 @@
+// BLAS-1 kernels of the nekBone PCG loop (nekBone/kernel/utils.okl) with the
+// deterministic reduction order of nbx_reduce.h.
+#include "nbx_blas.h"
+#include <mutex>
+
+namespace nbx {
+
+// <x,y>_w = sum_id w[id] sum_fld x[id+fld*offset] y[id+fld*offset]
+// (weightedMultipleInnerProduct2, nekBone/kernel/utils.okl:413-458); with
+// XX also <x,x>_w (weightedInnerProductUpdate, utils.okl:687-793).
+template <bool XX>
+__global__ __launch_bounds__(256) void k_wdot(const int32_t N, const int32_t offset, const int Nf,
+                                              const double* __restrict__ w, const double* __restrict__ x,
+                                              const double* __restrict__ y, double* __restrict__ partials,
+                                              unsigned int* __restrict__ ticket, double* __restrict__ out)
+{
+  constexpr int NV = XX ? 2 : 1;
+  __shared__ double s4[NV * 4];
+  double v[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) v[c] = 0.0;
+  const int64_t b0 = (int64_t)blockIdx.x * kRedTile + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kRedPerLane; ++u) {
+    const int64_t id = b0 + u * kRedLanes;
+    if (id < N) {
+      const double wid = w[id];
+      for (int f = 0; f < Nf; ++f) {
+        const double xf = x[id + (int64_t)f * offset];
+        const double wx = wid * xf;
+        v[0] = __builtin_fma(wx, y[id + (int64_t)f * offset], v[0]);
+        if (XX) v[NV - 1] = __builtin_fma(wx, xf, v[NV - 1]);
+      }
+    }
+  }
+  block_sum256<NV>(v, s4);
+  if (last_block_combine<NV>(v, partials, ticket, s4) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) out[c] = v[c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scaled_add(const int32_t N, const double alpha,
+                                                    const double* __restrict__ x, const double beta,
+                                                    double* __restrict__ y)
+{
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) y[n] = alpha * x[n] + beta * y[n];
+}
+
+int launch_wdot(int32_t N, int32_t offset, int Nf, const double* w, const double* x, const double* y, bool xx,
+                ReduceWork& ws, double* out_dev, hipStream_t s)
+{
+  if (N <= 0) return fail(NBX_ERR_INVALID, "wdot: N=%d", N);
+  const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
+  if (nb > ws.capacity) return fail(NBX_ERR_INVALID, "wdot: workspace too small (%u < %u)", ws.capacity, nb);
+  if (xx)
+    hipLaunchKernelGGL(k_wdot<true>, dim3(nb), dim3(256), 0, s, N, offset, Nf, w, x, y, ws.partials, ws.ticket,
+                       out_dev);
+  else
+    hipLaunchKernelGGL(k_wdot<false>, dim3(nb), dim3(256), 0, s, N, offset, Nf, w, x, y, ws.partials, ws.ticket,
+                       out_dev);
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}
+
+int ReduceWork::init(int64_t maxN, int nvals)
+{
+  capacity = (unsigned int)((maxN + kRedTile - 1) / kRedTile);
+  if (capacity < 1) capacity = 1;
+  NBX_HIP(hipMalloc(&partials, sizeof(double) * capacity * (size_t)nvals));
+  NBX_HIP(hipMalloc(&ticket, sizeof(unsigned int) * 4));
+  NBX_HIP(hipMemset(ticket, 0, sizeof(unsigned int) * 4));
+  NBX_HIP(hipMalloc(&result, sizeof(double) * 8));
+  return NBX_SUCCESS;
+}
+
+void ReduceWork::release()
+{
+  if (partials) (void)hipFree(partials);
+  if (ticket) (void)hipFree(ticket);
+  if (result) (void)hipFree(result);
+  partials = nullptr;
+  ticket = nullptr;
+  result = nullptr;
+  capacity = 0;
+}
+
+}  // namespace nbx
+
+using namespace nbx;
+
+// Workspace for the stand-alone reduction entry point; grown on demand, one
+// per process.  Calls that overlap from several host threads are serialised.
+static std::mutex g_wdot_mutex;
+static ReduceWork g_wdot_ws;
+
+extern "C" int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
+                                          const double* x, const double* y, double* result_host, void* stream)
+{
+  if (N <= 0 || Nfields < 1 || !w || !x || !y || !result_host)
+    return fail(NBX_ERR_INVALID, "weighted_inner_product: bad arguments");
+  if (Nfields > 1 && offset < N) return fail(NBX_ERR_INVALID, "weighted_inner_product: offset < N");
+  std::lock_guard<std::mutex> lock(g_wdot_mutex);
+  const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
+  if (nb > g_wdot_ws.capacity) {
+    NBX_HIP(hipDeviceSynchronize());
+    g_wdot_ws.release();
+    int rc = g_wdot_ws.init(N, 2);
+    if (rc) return rc;
+  }
+  hipStream_t s = as_stream(stream);
+  int rc = launch_wdot(N, offset, Nfields, w, x, y, false, g_wdot_ws, g_wdot_ws.result, s);
+  if (rc) return rc;
+  NBX_HIP(hipMemcpyAsync(result_host, g_wdot_ws.result, sizeof(double), hipMemcpyDeviceToHost, s));
+  NBX_HIP(hipStreamSynchronize(s));
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream)
+{
+  if (N <= 0) return N == 0 ? NBX_SUCCESS : fail(NBX_ERR_INVALID, "scaled_add: N < 0");
+  if (!x || !y) return fail(NBX_ERR_INVALID, "scaled_add: null array");
+  hipLaunchKernelGGL(k_scaled_add, dim3((N + 255) / 256), dim3(256), 0, as_stream(stream), N, alpha, x, beta, y);
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}

@@ -1,0 +1,528 @@
+// BP driver: nekBone's setup and PCG (nekBone/setup.cpp:36-367,
+// nekBone/solve.cpp:44-293) on one GPU.
+//
+// The PCG loop follows BPPCG (solve.cpp:44-113) step for step, with every
+// scalar kept on the device: the reduction kernels finish with a last-block
+// pass that also performs the scalar recurrences (TOL, beta, alpha and the
+// stopping test), and every kernel of an iteration starts by reading the
+// device `done` flag, so the host enqueues iterations in chunks and reads the
+// scalars back once per chunk instead of twice per iteration.
+//
+// Per-iteration kernels (reference call in brackets):
+//   1. z = M r, <r,z>_w, <r,r>_w [, <z,Ap>_w]  (BPPreconditioner + BPWeightedInnerProduct2 [+ flexible dot])
+//   2. p = z + beta p                          (BPScaledAdd)
+//   3. Ap = axhelm(p)                          (BPAxOperator: kernel)
+//   4. Ap = gs(Ap), Ap[mask] = 0               (BPAxOperator: oogs::start/finish; mask)
+//   5. pAp = <p,Ap>_w, alpha = rdotz/pAp       (BPAxOperator: BPWeightedInnerProduct)
+//   6. x += alpha p, r -= alpha Ap             (BPUpdatePCG)
+// Element-wise arithmetic uses one fma per update; the dots use the fixed
+// reduction tree of nbx_reduce.h.  The CPU oracle (nbo_pcg) restates exactly
+// this sequence.
+#include "nbx_blas.h"
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace nbx {
+
+struct PcgScalars {
+  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, beta, pAp, zdotAp;
+  int iter, done, iterations, converged, breakdown, maxit, fixed, flexible;
+};
+
+__device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, double rdotr, double zdotAp,
+                                               double* hist_rdotr)
+{
+  const int it = S->iter + 1;
+  S->iter = it;
+  if (it > S->maxit) {  // the reference's loop ends before this iteration's work (solve.cpp:76)
+    S->done = 1;
+    S->iterations = S->maxit;
+    return;
+  }
+  S->rdotz2 = S->rdotz1;
+  S->rdotz1 = rdotz;
+  S->rdotr = rdotr;
+  S->zdotAp = zdotAp;
+  hist_rdotr[it] = rdotr;
+  if (it == 1) {
+    S->rdotr0 = rdotr;
+    S->TOL = fmax(S->tol2 * rdotr, S->tol2);  // solve.cpp:85
+  }
+  if (!isfinite(rdotr)) {
+    S->done = 1;
+    S->breakdown = 1;
+    S->iterations = it - 1;
+    return;
+  }
+  if (rdotr <= S->TOL && !S->fixed) {  // solve.cpp:90
+    S->done = 1;
+    S->converged = 1;
+    S->iterations = it - 1;
+    return;
+  }
+  if (S->flexible)
+    S->beta = -S->alpha * S->zdotAp / S->rdotz2;  // solve.cpp:92-95
+  else
+    S->beta = (it == 1) ? 0.0 : S->rdotz1 / S->rdotz2;  // solve.cpp:97
+}
+
+// 1. z = M r (Jacobi: invDiag r; copy: z is r itself) and the dots
+template <bool JAC, bool FLEX>
+__global__ __launch_bounds__(256) void k_pcg_precon_dots(const int32_t N, const int32_t offset, const int Nf,
+                                                         const double* __restrict__ w,
+                                                         const double* __restrict__ invDiag,
+                                                         const double* __restrict__ r, double* __restrict__ z,
+                                                         const double* __restrict__ Ap,
+                                                         double* __restrict__ partials,
+                                                         unsigned int* __restrict__ ticket, PcgScalars* S,
+                                                         double* __restrict__ hist_rdotr)
+{
+  if (S->done) return;
+  constexpr int NV = FLEX ? 3 : 2;
+  __shared__ double s4[NV * 4];
+  double v[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) v[c] = 0.0;
+  const int64_t b0 = (int64_t)blockIdx.x * kRedTile + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kRedPerLane; ++u) {
+    const int64_t id = b0 + u * kRedLanes;
+    if (id < N) {
+      const double wid = w[id];
+      for (int f = 0; f < Nf; ++f) {
+        const int64_t iid = id + (int64_t)f * offset;
+        const double rf = r[iid];
+        double zf = rf;
+        if (JAC) {
+          zf = invDiag[id] * rf;
+          z[iid] = zf;
+        }
+        v[0] = __builtin_fma(wid * rf, zf, v[0]);
+        v[1] = __builtin_fma(wid * rf, rf, v[1]);
+        if (FLEX) v[NV - 1] = __builtin_fma(wid * zf, Ap[iid], v[NV - 1]);
+      }
+    }
+  }
+  block_sum256<NV>(v, s4);
+  if (last_block_combine<NV>(v, partials, ticket, s4) && threadIdx.x == 0)
+    pcg_after_dots(S, v[0], v[1], FLEX ? v[NV - 1] : 0.0, hist_rdotr);
+}
+
+// 2. p = z + beta p
+__global__ __launch_bounds__(256) void k_pcg_pupdate(const int64_t Nall, const double* __restrict__ z,
+                                                     double* __restrict__ p, const PcgScalars* S)
+{
+  if (S->done) return;
+  const double beta = S->beta;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < Nall) p[n] = __builtin_fma(beta, p[n], z[n]);
+}
+
+// 5. pAp = <p,Ap>_w, alpha
+__global__ __launch_bounds__(256) void k_pcg_pap(const int32_t N, const int32_t offset, const int Nf,
+                                                 const double* __restrict__ w, const double* __restrict__ p,
+                                                 const double* __restrict__ Ap, double* __restrict__ partials,
+                                                 unsigned int* __restrict__ ticket, PcgScalars* S,
+                                                 double* __restrict__ hist_alpha)
+{
+  if (S->done) return;
+  __shared__ double s4[4];
+  double v[1] = {0.0};
+  const int64_t b0 = (int64_t)blockIdx.x * kRedTile + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kRedPerLane; ++u) {
+    const int64_t id = b0 + u * kRedLanes;
+    if (id < N) {
+      const double wid = w[id];
+      for (int f = 0; f < Nf; ++f) {
+        const int64_t iid = id + (int64_t)f * offset;
+        v[0] = __builtin_fma(wid * p[iid], Ap[iid], v[0]);
+      }
+    }
+  }
+  block_sum256<1>(v, s4);
+  if (last_block_combine<1>(v, partials, ticket, s4) && threadIdx.x == 0) {
+    S->pAp = v[0];
+    const double alpha = S->rdotz1 / v[0];  // solve.cpp:104
+    S->alpha = alpha;
+    hist_alpha[S->iter] = alpha;
+    if (!isfinite(alpha)) {
+      S->done = 1;
+      S->breakdown = 1;
+      S->iterations = S->iter - 1;
+    }
+  }
+}
+
+// 6. x += alpha p; r -= alpha Ap  (BPUpdatePCG, nekBone/kernel/utils.okl:795-807)
+__global__ __launch_bounds__(256) void k_pcg_update(const int64_t Nall, const double* __restrict__ p,
+                                                    const double* __restrict__ Ap, double* __restrict__ x,
+                                                    double* __restrict__ r, const PcgScalars* S)
+{
+  if (S->done) return;
+  const double alpha = S->alpha;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < Nall) {
+    x[n] = __builtin_fma(alpha, p[n], x[n]);
+    r[n] = __builtin_fma(-alpha, Ap[n], r[n]);
+  }
+}
+
+// vecInv (nekBone/kernel/utils.okl:605-613)
+__global__ __launch_bounds__(256) void k_vec_inv(const int64_t N, double* __restrict__ v)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) v[n] = 1.0 / v[n];
+}
+
+// rhs (host-generated) masked and scaled per field is uploaded, then assembled
+__global__ __launch_bounds__(256) void k_scale_by(const int64_t N, double* __restrict__ v, const double* __restrict__ s)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) v[n] *= s[n];
+}
+
+inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace nbx
+
+using namespace nbx;
+
+struct nbx_bp {
+  nbx_bp_options opt{};
+  int N = 0, Nq = 0, Np = 0, Nf = 1;
+  int32_t E = 0, Nlocal = 0, fieldOffset = 0, Nmasked = 0;
+  int64_t Nall = 0;
+  hipStream_t stream = nullptr;
+  nbx_ogs* ogs = nullptr;
+  double *D = nullptr, *ggeo = nullptr, *lambda = nullptr, *invDiagA = nullptr, *rhs = nullptr;
+  double *p = nullptr, *z = nullptr, *Ap = nullptr, *Ax = nullptr;
+  int32_t* maskIds = nullptr;
+  ReduceWork red;
+  PcgScalars* S = nullptr;
+  PcgScalars* hS = nullptr;  // pinned host copy
+  double *hist_rdotr = nullptr, *hist_alpha = nullptr;
+  int hist_cap = 0;
+  int last_iters = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<void*> owned;
+
+  int alloc(double** ptr, int64_t n)
+  {
+    NBX_HIP(hipMalloc((void**)ptr, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+    NBX_HIP(hipMemset(*ptr, 0, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+    owned.push_back(*ptr);
+    return NBX_SUCCESS;
+  }
+};
+
+static void bp_release(nbx_bp* bp)
+{
+  if (!bp) return;
+  (void)hipDeviceSynchronize();
+  for (void* p : bp->owned) (void)hipFree(p);
+  if (bp->maskIds) (void)hipFree(bp->maskIds);
+  if (bp->S) (void)hipFree(bp->S);
+  if (bp->hS) (void)hipHostFree(bp->hS);
+  if (bp->ev0) (void)hipEventDestroy(bp->ev0);
+  if (bp->ev1) (void)hipEventDestroy(bp->ev1);
+  bp->red.release();
+  if (bp->ogs) nbx_ogs_free(bp->ogs);
+  delete bp;
+}
+
+static int bp_setup_impl(nbx_bp* bp)
+{
+  const nbx_bp_options& o = bp->opt;
+  const nbx_box& b = o.box;
+  bp->N = b.N;
+  bp->Nq = b.N + 1;
+  bp->Np = bp->Nq * bp->Nq * bp->Nq;
+  bp->Nf = o.Nfields;
+  if (bp->Nf != 1 && bp->Nf != 3) return fail(NBX_ERR_INVALID, "bp: Nfields must be 1 or 3");
+  if (o.precon == NBX_PRECON_JACOBI && bp->Nf > 1)  // nekBone/setup.cpp:158-162
+    return fail(NBX_ERR_UNSUPPORTED, "bp: JACOBI preconditioner not supported for Nfields>1");
+  const int64_t E = (int64_t)b.NX * b.NY * b.NZ;
+  if (E * bp->Np * bp->Nf > INT32_MAX) return fail(NBX_ERR_INVALID, "bp: problem exceeds dlong range");
+  bp->E = (int32_t)E;
+  bp->Nlocal = (int32_t)(E * bp->Np);
+  bp->fieldOffset = bp->Nlocal;  // Np*(Nelements + totalHaloPairs) on one rank (setup.cpp:73-74)
+  bp->Nall = (int64_t)bp->Nf * bp->fieldOffset;
+  const double lambda1 = o.bpmode ? 0.0 : o.lambda1;  // setup.cpp:51-53
+  const bool manufactured = o.rhs_kind == 1;
+
+  // host mesh inputs
+  std::vector<double> ggeo((size_t)E * 7 * bp->Np), x, y, z;
+  std::vector<nbx_hlong> ids(bp->Nlocal);
+  if (manufactured) {
+    x.resize(bp->Nlocal);
+    y.resize(bp->Nlocal);
+    z.resize(bp->Nlocal);
+  }
+  int rc = nbx_box_mesh(&b, o.dirichlet ? 1 : 0, manufactured ? x.data() : nullptr,
+                        manufactured ? y.data() : nullptr, manufactured ? z.data() : nullptr, ggeo.data(),
+                        ids.data(), nullptr);
+  if (rc) return rc;
+  std::vector<double> zg(bp->Nq), wg(bp->Nq), D(bp->Nq * bp->Nq);
+  nbx_gll(bp->N, zg.data(), wg.data(), D.data());
+
+  if ((rc = bp->alloc(&bp->D, D.size())) || (rc = bp->alloc(&bp->ggeo, (int64_t)ggeo.size())) ||
+      (rc = bp->alloc(&bp->lambda, 2 * (int64_t)bp->fieldOffset)) || (rc = bp->alloc(&bp->invDiagA, bp->Nlocal)) ||
+      (rc = bp->alloc(&bp->rhs, bp->Nall)) || (rc = bp->alloc(&bp->p, bp->Nall)) ||
+      (rc = bp->alloc(&bp->z, bp->Nall)) || (rc = bp->alloc(&bp->Ap, bp->Nall)) || (rc = bp->alloc(&bp->Ax, bp->Nall)))
+    return rc;
+  NBX_HIP(hipMemcpy(bp->D, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(bp->ggeo, ggeo.data(), sizeof(double) * ggeo.size(), hipMemcpyHostToDevice));
+  {
+    std::vector<double> lam(2 * (size_t)bp->fieldOffset);
+    for (int32_t i = 0; i < bp->fieldOffset; ++i) {  // setup.cpp:104-108
+      lam[i] = 1.0;
+      lam[i + (size_t)bp->fieldOffset] = lambda1;
+    }
+    NBX_HIP(hipMemcpy(bp->lambda, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice));
+  }
+  // gather-scatter on the (masked) ids
+  bp->ogs = nbx_ogs_setup(bp->Nlocal, ids.data(), o.verbose);
+  if (!bp->ogs) return NBX_ERR_HIP;
+  std::vector<int32_t> mask;
+  for (int32_t i = 0; i < bp->Nlocal; ++i)
+    if (ids[i] == 0) mask.push_back(i);
+  bp->Nmasked = (int32_t)mask.size();
+  NBX_HIP(hipMalloc(&bp->maskIds, sizeof(int32_t) * std::max<size_t>(mask.size(), 1)));
+  if (!mask.empty())
+    NBX_HIP(hipMemcpy(bp->maskIds, mask.data(), sizeof(int32_t) * mask.size(), hipMemcpyHostToDevice));
+
+  // right-hand side (setup.cpp:77-99): manufactured cosine or seeded uniform,
+  // masked, then assembled
+  {
+    std::vector<double> r(bp->Nall);
+    if (manufactured) {
+      for (int32_t e = 0; e < bp->E; ++e)
+        for (int n = 0; n < bp->Np; ++n) {
+          const int64_t id = (int64_t)e * bp->Np + n;
+          const double JW = ggeo[(size_t)bp->Np * (e * 7 + 0) + n];
+          const double v = (3. * M_PI * M_PI + lambda1) * JW * cos(M_PI * x[id]) * cos(M_PI * y[id]) *
+                           cos(M_PI * z[id]);
+          for (int f = 0; f < bp->Nf; ++f) r[id + (int64_t)f * bp->fieldOffset] = v;
+        }
+    } else {
+      for (int f = 0; f < bp->Nf; ++f) {
+        nbx_urand_fill(b.seed, 3 + f, bp->Nlocal, r.data() + (int64_t)f * bp->fieldOffset);
+      }
+    }
+    for (int f = 0; f < bp->Nf; ++f)
+      for (int32_t m : mask) r[m + (int64_t)f * bp->fieldOffset] = 0.0;
+    NBX_HIP(hipMemcpy(bp->rhs, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice));
+    rc = gs_launch(bp->ogs, bp->rhs, bp->Nf, bp->fieldOffset, NBX_DOUBLE, NBX_ADD, nullptr, bp->stream);
+    if (rc) return rc;
+  }
+  if ((rc = bp->red.init(bp->Nlocal, 3))) return rc;
+  NBX_HIP(hipMalloc(&bp->S, sizeof(PcgScalars)));
+  NBX_HIP(hipHostMalloc((void**)&bp->hS, sizeof(PcgScalars), hipHostMallocDefault));
+  NBX_HIP(hipEventCreate(&bp->ev0));
+  NBX_HIP(hipEventCreate(&bp->ev1));
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  return NBX_SUCCESS;
+}
+
+extern "C" nbx_bp* nbx_bp_setup(const nbx_bp_options* opt, void* stream)
+{
+  if (!opt) {
+    fail(NBX_ERR_INVALID, "bp_setup: null options");
+    return nullptr;
+  }
+  nbx_bp* bp = new nbx_bp();
+  bp->opt = *opt;
+  bp->stream = as_stream(stream);
+  if (bp_setup_impl(bp) != NBX_SUCCESS) {
+    std::string msg = nbx_last_error();
+    bp_release(bp);
+    fail(NBX_ERR_INVALID, "%s", msg.c_str());
+    return nullptr;
+  }
+  return bp;
+}
+
+extern "C" void nbx_bp_free(nbx_bp* bp) { bp_release(bp); }
+
+extern "C" int nbx_bp_sizes(const nbx_bp* bp, nbx_dlong* Nelements, nbx_dlong* Np, nbx_dlong* fieldOffset,
+                            nbx_dlong* Nmasked)
+{
+  if (!bp) return fail(NBX_ERR_INVALID, "bp_sizes: null handle");
+  if (Nelements) *Nelements = bp->E;
+  if (Np) *Np = bp->Np;
+  if (fieldOffset) *fieldOffset = bp->fieldOffset;
+  if (Nmasked) *Nmasked = bp->Nmasked;
+  return NBX_SUCCESS;
+}
+
+extern "C" const void* nbx_bp_device_array(const nbx_bp* bp, const char* name)
+{
+  if (!bp || !name) return nullptr;
+  const std::string n(name);
+  if (n == "ggeo") return bp->ggeo;
+  if (n == "D") return bp->D;
+  if (n == "lambda") return bp->lambda;
+  if (n == "invDegree") return nbx_ogs_inv_degree(bp->ogs);
+  if (n == "invDiagA") return bp->invDiagA;
+  if (n == "rhs") return bp->rhs;
+  if (n == "maskIds") return bp->maskIds;
+  fail(NBX_ERR_INVALID, "bp_device_array: unknown array '%s'", name);
+  return nullptr;
+}
+
+extern "C" nbx_ogs* nbx_bp_ogs(nbx_bp* bp) { return bp ? bp->ogs : nullptr; }
+
+// Aq = mask(gs(axhelm(q)))  (BPAxOperator without the dot)
+static int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* Aq, const int* skip)
+{
+  const nbx_bp_options& o = bp->opt;
+  int rc = axhelm_launch(bp->N, bp->Nf, o.bpmode ? 1 : 0, NBX_F64, bp->E, bp->fieldOffset, nullptr, bp->ggeo,
+                         bp->D, lambda, q, Aq, skip, bp->stream);
+  if (rc) return rc;
+  rc = gs_launch(bp->ogs, Aq, bp->Nf, bp->fieldOffset, NBX_DOUBLE, NBX_ADD, skip, bp->stream);
+  if (rc) return rc;
+  for (int f = 0; f < bp->Nf && bp->Nmasked; ++f) {
+    rc = mask_launch(bp->Nmasked, bp->maskIds, Aq + (int64_t)f * bp->fieldOffset, skip, bp->stream);
+    if (rc) return rc;
+  }
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double* q, double* Aq, double* pAp_host)
+{
+  if (!bp || !q || !Aq) return fail(NBX_ERR_INVALID, "bp_ax_operator: bad arguments");
+  if (!lambda) lambda = bp->lambda;
+  int rc = bp_apply(bp, lambda, q, Aq, nullptr);
+  if (rc) return rc;
+  if (pAp_host) {
+    rc = launch_wdot(bp->Nlocal, bp->fieldOffset, bp->Nf, nbx_ogs_inv_degree(bp->ogs), q, Aq, false, bp->red,
+                     bp->red.result, bp->stream);
+    if (rc) return rc;
+    NBX_HIP(hipMemcpyAsync(pAp_host, bp->red.result, sizeof(double), hipMemcpyDeviceToHost, bp->stream));
+  }
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  return NBX_SUCCESS;
+}
+
+// updateJacobi (solve.cpp:146-165): local diagonal, assembled, inverted
+static int bp_update_jacobi(nbx_bp* bp, const double* lambda)
+{
+  int rc = nbx_update_jacobi_local(bp->N, bp->E, bp->fieldOffset, nullptr, bp->ggeo, bp->D, lambda, bp->invDiagA,
+                                   bp->stream);
+  if (rc) return rc;
+  rc = gs_launch(bp->ogs, bp->invDiagA, 1, bp->fieldOffset, NBX_DOUBLE, NBX_ADD, nullptr, bp->stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_vec_inv, grid_for(bp->Nlocal), dim3(256), 0, bp->stream, (int64_t)bp->Nlocal, bp->invDiagA);
+  NBX_HIP_LAUNCH();
+  // masked nodes are not in any group: their diagonal stays local; they carry
+  // r = 0, so z = 0 there whatever the value
+  return NBX_SUCCESS;
+}
+
+static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, double* x)
+{
+  const nbx_bp_options& o = bp->opt;
+  const bool jac = o.precon == NBX_PRECON_JACOBI;
+  const bool flex = o.flexible != 0;
+  const unsigned int nb = (unsigned int)((bp->Nlocal + kRedTile - 1) / kRedTile);
+  const double* w = nbx_ogs_inv_degree(bp->ogs);
+  const double* zsrc = jac ? bp->z : r;
+#define NBX_PD(J, F)                                                                                          \
+  hipLaunchKernelGGL((k_pcg_precon_dots<J, F>), dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, \
+                     bp->Nf, w, bp->invDiagA, r, bp->z, bp->Ap, bp->red.partials, bp->red.ticket, bp->S,          \
+                     bp->hist_rdotr)
+  if (jac && flex) NBX_PD(true, true);
+  else if (jac) NBX_PD(true, false);
+  else if (flex) NBX_PD(false, true);
+  else NBX_PD(false, false);
+#undef NBX_PD
+  NBX_HIP_LAUNCH();
+  hipLaunchKernelGGL(k_pcg_pupdate, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, zsrc, bp->p, bp->S);
+  NBX_HIP_LAUNCH();
+  int rc = bp_apply(bp, lambda, bp->p, bp->Ap, &bp->S->done);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pcg_pap, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, bp->Nf, w, bp->p,
+                     bp->Ap, bp->red.partials, bp->red.ticket, bp->S, bp->hist_alpha);
+  NBX_HIP_LAUNCH();
+  hipLaunchKernelGGL(k_pcg_update, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, bp->p, bp->Ap, x, r,
+                     bp->S);
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
+                          nbx_pcg_stats* stats)
+{
+  if (!bp || !r || !x) return fail(NBX_ERR_INVALID, "bp_pcg: bad arguments");
+  if (maxit < 0) return fail(NBX_ERR_INVALID, "bp_pcg: maxit < 0");
+  if (!lambda) lambda = bp->lambda;
+  if (maxit + 2 > bp->hist_cap) {
+    NBX_HIP(hipStreamSynchronize(bp->stream));
+    if (bp->hist_rdotr) (void)hipFree(bp->hist_rdotr);
+    if (bp->hist_alpha) (void)hipFree(bp->hist_alpha);
+    bp->hist_cap = maxit + 2;
+    NBX_HIP(hipMalloc(&bp->hist_rdotr, sizeof(double) * bp->hist_cap));
+    NBX_HIP(hipMalloc(&bp->hist_alpha, sizeof(double) * bp->hist_cap));
+  }
+  NBX_HIP(hipMemsetAsync(bp->hist_rdotr, 0, sizeof(double) * bp->hist_cap, bp->stream));
+  NBX_HIP(hipMemsetAsync(bp->hist_alpha, 0, sizeof(double) * bp->hist_cap, bp->stream));
+  PcgScalars s0{};
+  s0.rdotz1 = 1.0;  // solve.cpp:58-60
+  s0.rdotz2 = 0.0;
+  s0.tol2 = tol * tol;
+  s0.maxit = maxit;
+  s0.fixed = tol > 0 ? 0 : 1;  // solve() sets FIXED ITERATION COUNT when tol <= 0 (main.cpp:17-22)
+  s0.flexible = bp->opt.flexible ? 1 : 0;
+  *bp->hS = s0;
+  NBX_HIP(hipMemcpyAsync(bp->S, bp->hS, sizeof(PcgScalars), hipMemcpyHostToDevice, bp->stream));
+  NBX_HIP(hipEventRecord(bp->ev0, bp->stream));
+  // Ax = A x; r = r - Ax  (solve.cpp:68-69)
+  int rc = bp_apply(bp, lambda, x, bp->Ax, nullptr);
+  if (rc) return rc;
+  rc = nbx_scaled_add((nbx_dlong)bp->Nall, -1.0, bp->Ax, 1.0, r, bp->stream);
+  if (rc) return rc;
+  if (bp->opt.precon == NBX_PRECON_JACOBI && (rc = bp_update_jacobi(bp, lambda))) return rc;  // solve.cpp:72
+  // iterations in chunks; the device flag ends the work, the host only polls
+  const int chunk = 16;
+  int enqueued = 0;
+  for (;;) {
+    const int n = std::min(chunk, maxit + 1 - enqueued);
+    for (int c = 0; c < n; ++c)
+      if ((rc = bp_enqueue_iteration(bp, lambda, r, x))) return rc;
+    enqueued += n;
+    NBX_HIP(hipMemcpyAsync(bp->hS, bp->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, bp->stream));
+    NBX_HIP(hipStreamSynchronize(bp->stream));
+    if (bp->hS->done || enqueued > maxit) break;
+  }
+  NBX_HIP(hipEventRecord(bp->ev1, bp->stream));
+  NBX_HIP(hipEventSynchronize(bp->ev1));
+  float ms = 0.f;
+  NBX_HIP(hipEventElapsedTime(&ms, bp->ev0, bp->ev1));
+  const PcgScalars& S = *bp->hS;
+  bp->last_iters = S.iterations;
+  if (stats) {
+    stats->iterations = S.iterations;
+    stats->converged = S.converged;
+    stats->rdotr0 = S.rdotr0;
+    stats->rdotr = S.rdotr;
+    stats->elapsed_s = ms * 1e-3;
+  }
+  if (S.breakdown) return fail(NBX_ERR_DIVERGED, "bp_pcg: breakdown at iteration %d", S.iter);
+  return S.iterations;
+}
+
+extern "C" int nbx_bp_history(const nbx_bp* bp, double* rdotr, double* alpha, int capacity)
+{
+  if (!bp || capacity < 0) return fail(NBX_ERR_INVALID, "bp_history: bad arguments");
+  const int n = std::min(capacity, std::min(bp->last_iters + 2, bp->hist_cap));
+  if (n <= 0) return 0;
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  if (rdotr) NBX_HIP(hipMemcpy(rdotr, bp->hist_rdotr, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (alpha) NBX_HIP(hipMemcpy(alpha, bp->hist_alpha, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return n;
+}

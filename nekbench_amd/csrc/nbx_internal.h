@@ -1,0 +1,46 @@
+// Internal helpers shared by the nbx translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "../../include/nbx.h"
+
+namespace nbx {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kGWJ = 0, kG00 = 1, kG01 = 2, kG02 = 3, kG11 = 4, kG12 = 5, kG22 = 6, kNggeo = 7;
+
+// reduction tile: 256 lanes x 8 values (see nbx_blas.hip)
+constexpr int kRedLanes = 256;
+constexpr int kRedPerLane = 8;
+constexpr int kRedTile = kRedLanes * kRedPerLane;
+
+// launchers shared between translation units; `skip` (device int, may be
+// null) turns the launch into a no-op when *skip != 0
+int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32_t offset,
+                  const int32_t* elementList, const void* ggeo, const void* D, const void* lambda,
+                  const void* q, void* Aq, const int* skip, hipStream_t s);
+int gs_launch(const nbx_ogs* o, void* v, int k, int32_t stride, nbx_type type, nbx_op op, const int* skip,
+              hipStream_t s);
+int mask_launch(int32_t Nmasked, const int32_t* maskIds, double* q, const int* skip, hipStream_t s);
+
+}  // namespace nbx
+
+#define NBX_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t nbx_e_ = (call);                                                              \
+    if (nbx_e_ != hipSuccess)                                                                \
+      return nbx::fail(NBX_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,               \
+                       hipGetErrorString(nbx_e_));                                           \
+  } while (0)
+
+#define NBX_HIP_LAUNCH()                                                                     \
+  do {                                                                                       \
+    hipError_t nbx_e_ = hipGetLastError();                                                   \
+    if (nbx_e_ != hipSuccess)                                                                \
+      return nbx::fail(NBX_ERR_HIP, "%s:%d launch: %s", __FILE__, __LINE__,                 \
+                       hipGetErrorString(nbx_e_));                                           \
+  } while (0)

@@ -1,0 +1,266 @@
+"""ctypes view of the CPU oracle (libnbo.so) and of the reference CPU kernels
+compiled into oracle/_ref.  TEST INFRASTRUCTURE ONLY: imported by tests/,
+the smoke check and bench.py's CPU-baseline leg, never by the product."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REF_DIR = HERE / "_ref"
+_lib = None
+
+
+class Box(C.Structure):
+    _fields_ = [("N", C.c_int), ("NX", C.c_int), ("NY", C.c_int), ("NZ", C.c_int),
+                ("periodic", C.c_int), ("perturb", C.c_double), ("zaspect", C.c_double),
+                ("seed", C.c_uint64),
+                ("x0", C.c_double), ("x1", C.c_double), ("y0", C.c_double), ("y1", C.c_double),
+                ("z0", C.c_double), ("z1", C.c_double)]
+
+    @classmethod
+    def make(cls, N, NX, NY, NZ, periodic=0, perturb=0.0, zaspect=0.0, seed=1,
+             extent=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+        return cls(N, NX, NY, NZ, periodic, perturb, zaspect, seed, *extent)
+
+    @property
+    def E(self):
+        return self.NX * self.NY * self.NZ
+
+    @property
+    def Np(self):
+        return (self.N + 1) ** 3
+
+
+class Ogs(C.Structure):
+    _fields_ = [("N", C.c_int32), ("Nlocal", C.c_int32), ("NlocalGather", C.c_int32),
+                ("offsets", C.POINTER(C.c_int32)), ("gatherIds", C.POINTER(C.c_int32)),
+                ("invDegree", C.POINTER(C.c_double))]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = HERE / "libnbo.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        _lib = C.CDLL(str(path))
+        _lib.nbo_urand.restype = C.c_double
+        _lib.nbo_urand.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        _lib.nbo_splitmix64.restype = C.c_uint64
+        _lib.nbo_splitmix64.argtypes = [C.c_uint64]
+        _lib.nbo_geometric_factors.restype = C.c_double
+        for f in ("nbo_tree_wdot", "nbo_tree_dot", "nbo_tree_sum_partials"):
+            getattr(_lib, f).restype = C.c_double
+        _lib.nbo_box_nelements.restype = C.c_int64
+    return _lib
+
+
+def p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+def urand(seed, stream, n):
+    out = np.empty(n)
+    lib().nbo_urand_fill(C.c_uint64(seed), C.c_uint64(stream), C.c_int64(n), p(out))
+    return out
+
+
+def gll(N):
+    z = np.empty(N + 1)
+    w = np.empty(N + 1)
+    lib().nbo_gll(N, p(z), p(w))
+    D = np.empty((N + 1) ** 2)
+    lib().nbo_dmatrix(N, p(z), p(D))
+    return z, w, D
+
+
+def box_mesh(box: Box, mask_boundary=False):
+    """Oracle mesh: dict with EX/EY/EZ, EToV, x, y, z, ggeo, ids, D, w, z_gll."""
+    E, N, Np = box.E, box.N, box.Np
+    l = lib()
+    EX, EY, EZ = (np.empty(E * 8) for _ in range(3))
+    EToV = np.empty(E * 8, dtype=np.int64)
+    l.nbo_box_vertices(C.byref(box), p(EX), p(EY), p(EZ), p(EToV))
+    zg, wg, D = gll(N)
+    x, y, z = (np.empty(E * Np) for _ in range(3))
+    l.nbo_physical_nodes(N, C.c_int64(E), p(EX), p(EY), p(EZ), p(zg), p(x), p(y), p(z))
+    ggeo = np.empty(E * 7 * Np)
+    minJ = l.nbo_geometric_factors(N, C.c_int64(E), p(x), p(y), p(z), p(D), p(wg), p(ggeo))
+    ids = np.empty(E * Np, dtype=np.int64)
+    l.nbo_global_ids(C.byref(box), int(mask_boundary), p(ids))
+    return dict(EX=EX, EY=EY, EZ=EZ, EToV=EToV, x=x, y=y, z=z, ggeo=ggeo, ids=ids, D=D, w=wg,
+                z_gll=zg, minJ=minJ)
+
+
+def ogs_setup(ids):
+    ids = np.ascontiguousarray(ids, dtype=np.int64)
+    o = Ogs()
+    lib().nbo_ogs_setup(C.c_int32(ids.size), p(ids), C.byref(o))
+    off = np.ctypeslib.as_array(o.offsets, shape=(o.NlocalGather + 1,)).copy()
+    gid = np.ctypeslib.as_array(o.gatherIds, shape=(max(o.Nlocal, 1),)).copy()[: o.Nlocal]
+    invd = np.ctypeslib.as_array(o.invDegree, shape=(max(o.N, 1),)).copy()[: o.N]
+    return o, off, gid, invd
+
+
+def ogs_free(o):
+    lib().nbo_ogs_free(C.byref(o))
+
+
+TYPES = {"double": 0, "float": 1, "int": 2, "long long int": 3}
+OPS = {"add": 0, "mul": 1, "min": 2, "max": 3}
+
+
+def gs_many(o, q, k, stride, type_="double", op="add"):
+    rc = lib().nbo_gs_many(C.byref(o), p(q), k, C.c_int32(stride), TYPES[type_], OPS[op])
+    assert rc == 0
+
+
+def axhelm(N, Nf, bk, E, offset, ggeo, D, lam, q, elementList=None, out=None):
+    Aq = np.zeros_like(q) if out is None else out
+    lib().nbo_axhelm(N, Nf, int(bk), C.c_int32(E), C.c_int32(offset), p(elementList), p(ggeo), p(D),
+                     p(lam), p(q), p(Aq))
+    return Aq
+
+
+def axhelm_f32(N, Nf, bk, E, offset, ggeo, D, lam, q):
+    Aq = np.zeros_like(q)
+    lib().nbo_axhelm_f32(N, Nf, int(bk), C.c_int32(E), C.c_int32(offset), p(ggeo), p(D), p(lam), p(q), p(Aq))
+    return Aq
+
+
+def axhelm_reference(Nq, E, lam, ggeo, D, q):
+    out = np.zeros_like(q)
+    lib().nbo_axhelm_reference(Nq, E, C.c_double(lam), p(ggeo), p(D), p(q), p(out))
+    return out
+
+
+def jacobi_local(N, E, offset, ggeo, D, lam, mapB=None):
+    out = np.zeros(E * (N + 1) ** 3)
+    lib().nbo_jacobi_local(N, C.c_int32(E), C.c_int32(offset), p(mapB), p(ggeo), p(D), p(lam), p(out))
+    return out
+
+
+def tree_wdot(w, x, y):
+    return lib().nbo_tree_wdot(C.c_int64(x.size), p(w), p(x), p(y))
+
+
+def tree_dot(x, y):
+    return lib().nbo_tree_dot(C.c_int64(x.size), p(x), p(y))
+
+
+# ---------------------------------------------------------------- oracle/_ref
+def ref_available(N=7) -> bool:
+    return (REF_DIR / f"libaxhelm_N{N}.so").exists() and (REF_DIR / "libaxref.so").exists()
+
+
+_ref_cache = {}
+
+
+def _ref(name):
+    if name not in _ref_cache:
+        _ref_cache[name] = C.CDLL(str(REF_DIR / name))
+    return _ref_cache[name]
+
+
+def ref_axhelm(N, kernel, E, offset, ggeo, D, lam, q):
+    """Call the reference's compiled CPU kernel axhelm[_bk][_n3]_v0 (axhelm/kernel/CPU/axhelm.c)."""
+    l = _ref(f"libaxhelm_N{N}.so")
+    f = getattr(l, kernel)
+    Aq = np.zeros_like(q)
+    if lam is None:
+        lam = np.zeros(2 * offset)
+    f(C.byref(C.c_int(E)), C.byref(C.c_int(offset)), p(ggeo), p(D), p(lam), p(q), p(Aq))
+    return Aq
+
+
+def ref_axhelm_reference(Nq, E, lam, ggeo, D, q):
+    l = _ref("libaxref.so")
+    out = np.zeros_like(q)
+    l.ref_axhelmReference(Nq, E, C.c_double(lam), p(ggeo), p(D), p(q), p(out))
+    return out
+
+
+def ref_weighted_inner_product_update(Nf, N, offset, w, a, b):
+    """weightedInnerProductUpdate (nekBone/kernel/CPU/weightedInnerProduct.c:63-84)."""
+    l = _ref(f"libwip_nf{Nf}.so")
+    out = np.zeros(2)
+    l.weightedInnerProductUpdate(C.byref(C.c_int(N)), C.byref(C.c_int(offset)), C.byref(C.c_int(1)),
+                                 p(w), p(a), p(b), p(out))
+    return out
+
+
+class BPStruct(C.Structure):
+    _fields_ = [("N", C.c_int), ("Nf", C.c_int), ("bk", C.c_int), ("E", C.c_int32), ("offset", C.c_int32),
+                ("ggeo", C.c_void_p), ("D", C.c_void_p), ("lam", C.c_void_p), ("ogs", C.c_void_p),
+                ("w", C.c_void_p), ("Nmasked", C.c_int32), ("maskIds", C.c_void_p)]
+
+
+class PCGStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("breakdown", C.c_int), ("iter", C.c_int),
+                ("rdotr0", C.c_double), ("rdotr", C.c_double), ("TOL", C.c_double)]
+
+
+class OracleBP:
+    """The oracle's view of nekBone's BP problem: mesh, ogs, lambda, rhs --
+    built exactly as the library's nbx_bp_setup builds them."""
+
+    def __init__(self, box: Box, Nf=1, bpmode=False, lambda1=0.0, dirichlet=True, rhs_kind=0):
+        self.box = box
+        self.N, self.Nf = box.N, Nf
+        m = box_mesh(box, mask_boundary=dirichlet)
+        self.mesh = m
+        self.E, self.Np = box.E, box.Np
+        self.offset = self.E * self.Np
+        self.lambda1 = 0.0 if bpmode else lambda1
+        self.bk = int(bool(bpmode))
+        self.lam = np.concatenate([np.ones(self.offset), np.full(self.offset, self.lambda1)])
+        self.ogs, self.off, self.gid, self.invd = ogs_setup(m["ids"])
+        self.maskIds = np.nonzero(m["ids"] == 0)[0].astype(np.int32) if dirichlet else np.zeros(0, np.int32)
+        self.struct = BPStruct(self.N, Nf, self.bk, self.E, self.offset, m["ggeo"].ctypes.data,
+                               m["D"].ctypes.data, self.lam.ctypes.data, C.addressof(self.ogs),
+                               self.invd.ctypes.data, self.maskIds.size,
+                               self.maskIds.ctypes.data if self.maskIds.size else None)
+        r = np.zeros(Nf * self.offset)
+        if rhs_kind == 1:
+            JW = m["ggeo"].reshape(self.E, 7, self.Np)[:, 0].ravel()
+            v = (3.0 * np.pi * np.pi + self.lambda1) * JW * np.cos(np.pi * m["x"]) * np.cos(np.pi * m["y"]) * \
+                np.cos(np.pi * m["z"])
+            for f in range(Nf):
+                r[f * self.offset:(f + 1) * self.offset] = v
+        else:
+            for f in range(Nf):
+                r[f * self.offset:(f + 1) * self.offset] = urand(box.seed, 3 + f, self.offset)
+        for f in range(Nf):
+            r[self.maskIds + f * self.offset] = 0.0
+        gs_many(self.ogs, r, Nf, self.offset)
+        self.rhs = r
+
+    def apply(self, q):
+        Aq = np.zeros_like(q)
+        lib().nbo_bp_apply(C.byref(self.struct), p(q), p(Aq))
+        return Aq
+
+    def wdot(self, x, y):
+        f = lib().nbo_bp_wdot
+        f.restype = C.c_double
+        return f(C.byref(self.struct), p(x), p(y))
+
+    def pcg(self, r, x, tol, maxit, jacobi=True, flexible=False):
+        hr = np.zeros(maxit + 2)
+        ha = np.zeros(maxit + 2)
+        st = PCGStats()
+        lib().nbo_bp_pcg(C.byref(self.struct), int(jacobi), int(flexible), p(r), p(x), C.c_double(tol), maxit,
+                         p(hr), p(ha), C.byref(st))
+        return st, hr, ha
+
+    def __del__(self):
+        try:
+            ogs_free(self.ogs)
+        except Exception:
+            pass

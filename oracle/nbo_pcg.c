@@ -1,0 +1,140 @@
+/*
+ * nbo_pcg.c -- CPU oracle of nekBone's BP operator and PCG loop.
+ *
+ * TEST INFRASTRUCTURE ONLY (see nbo.h).
+ *
+ * Restates BPPCG (nekBone/solve.cpp:44-113), BPAxOperator (:181-235),
+ * updateJacobi (:146-165), BPWeightedInnerProduct[2] (:237-287),
+ * BPScaledAdd (:289-293) and BPUpdatePCG (:136-142, utils.okl:795-807) in the
+ * library's arithmetic order: one fma per element-wise update, dots in the
+ * fixed tile tree, scalars evaluated as C expressions in the reference's form.
+ *
+ * Pinning: the operator is pinned through nbo_axhelm / nbo_gs_many; the loop
+ * by CG invariants on small problems (orthogonality of residuals, A-conjugacy
+ * of directions, monotone A-norm error), by the dense direct solve of the
+ * assembled system, and by the manufactured cosine solution of
+ * nekBone/main.cpp:105-132 (tests/test_oracle_pcg.py).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "nbo.h"
+
+#define TILE 2048
+
+/* partial sums of NV values per tile in the library's tree, then combined */
+#define TILE_TREE(NVV, N_, OUT, BODY)                                              \
+  do {                                                                             \
+    const int64_t nb_ = ((N_) + TILE - 1) / TILE;                                  \
+    double* parts_ = (double*)malloc(sizeof(double) * (NVV) * (nb_ > 0 ? nb_ : 1)); \
+    _Pragma("omp parallel for schedule(static)")                                   \
+    for (int64_t b = 0; b < nb_; ++b) {                                            \
+      double lane_[NVV][256];                                                      \
+      for (int t = 0; t < 256; ++t) {                                              \
+        double acc[NVV];                                                           \
+        for (int c = 0; c < (NVV); ++c) acc[c] = 0.0;                              \
+        for (int u = 0; u < 8; ++u) {                                              \
+          const int64_t id = b * TILE + u * 256 + t;                               \
+          if (id < (N_)) { BODY }                                                  \
+        }                                                                          \
+        for (int c = 0; c < (NVV); ++c) lane_[c][t] = acc[c];                      \
+      }                                                                            \
+      for (int c = 0; c < (NVV); ++c) parts_[c * nb_ + b] = nbo_tree256(lane_[c]); \
+    }                                                                              \
+    for (int c = 0; c < (NVV); ++c) (OUT)[c] = nbo_tree_sum_partials(nb_, parts_ + c * nb_); \
+    free(parts_);                                                                  \
+  } while (0)
+
+void nbo_bp_apply(const nbo_bp* bp, const double* q, double* Aq)
+{
+  nbo_axhelm(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, q, Aq);
+  nbo_gs_many(bp->ogs, Aq, bp->Nf, bp->offset, NBO_DOUBLE, NBO_ADD);
+  for (int f = 0; f < bp->Nf; ++f)
+    for (int32_t m = 0; m < bp->Nmasked; ++m) Aq[bp->maskIds[m] + (int64_t)f * bp->offset] = 0.0;
+}
+
+double nbo_bp_wdot(const nbo_bp* bp, const double* x, const double* y)
+{
+  double out[1];
+  const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  TILE_TREE(1, N, out, {
+    const double wid = bp->w[id];
+    for (int f = 0; f < bp->Nf; ++f) {
+      const int64_t iid = id + (int64_t)f * bp->offset;
+      acc[0] = fma(wid * x[iid], y[iid], acc[0]);
+    }
+  });
+  return out[0];
+}
+
+int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, double* r, double* x, double tol, int maxit,
+               double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
+{
+  const int Np = (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  const int64_t N = (int64_t)bp->E * Np;
+  const int64_t Nall = (int64_t)bp->Nf * bp->offset;
+  const int fixed = tol > 0 ? 0 : 1;
+  double* p = (double*)calloc(Nall, sizeof(double));
+  double* z = (double*)calloc(Nall, sizeof(double));
+  double* Ap = (double*)calloc(Nall, sizeof(double));
+  double* Ax = (double*)calloc(Nall, sizeof(double));
+  double* invDiag = NULL;
+  double rdotz1 = 1.0, rdotz2 = 0.0, alpha = 0.0, beta = 0.0, TOL = 0.0, rdotr = 0.0, rdotr0 = 0.0;
+  int converged = 0, breakdown = 0, iterations = maxit, it;
+
+  /* Ax = A x; r = r - Ax (solve.cpp:68-69) */
+  nbo_bp_apply(bp, x, Ax);
+  for (int64_t n = 0; n < Nall; ++n) r[n] = -1.0 * Ax[n] + 1.0 * r[n];
+  if (jacobi) {  /* solve.cpp:72, 146-165 */
+    invDiag = (double*)malloc(sizeof(double) * N);
+    nbo_jacobi_local(bp->N, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, invDiag);
+    nbo_gs_many(bp->ogs, invDiag, 1, bp->offset, NBO_DOUBLE, NBO_ADD);
+    for (int64_t n = 0; n < N; ++n) invDiag[n] = 1.0 / invDiag[n];
+  }
+  for (it = 1;; ++it) {
+    if (it > maxit) { iterations = maxit; break; }
+    /* z = M r; (r.z)_w, (r.r)_w [, (z.Ap)_w]  (solve.cpp:77-83, 94) */
+    double d[3];
+    const double* zsrc = jacobi ? z : r;
+    TILE_TREE(3, N, d, {
+      const double wid = bp->w[id];
+      for (int f = 0; f < bp->Nf; ++f) {
+        const int64_t iid = id + (int64_t)f * bp->offset;
+        const double rf = r[iid];
+        double zf = rf;
+        if (jacobi) { zf = invDiag[id] * rf; z[iid] = zf; }
+        acc[0] = fma(wid * rf, zf, acc[0]);
+        acc[1] = fma(wid * rf, rf, acc[1]);
+        if (flexible) acc[2] = fma(wid * zf, Ap[iid], acc[2]);
+      }
+    });
+    rdotz2 = rdotz1;
+    rdotz1 = d[0];
+    rdotr = d[1];
+    if (hist_rdotr) hist_rdotr[it] = rdotr;
+    if (it == 1) { rdotr0 = rdotr; TOL = fmax(tol * tol * rdotr, tol * tol); }  /* solve.cpp:85 */
+    if (!isfinite(rdotr)) { breakdown = 1; iterations = it - 1; break; }
+    if (rdotr <= TOL && !fixed) { converged = 1; iterations = it - 1; break; }  /* :90 */
+    if (flexible) beta = -alpha * d[2] / rdotz2;                                    /* :92-95 */
+    else beta = (it == 1) ? 0.0 : rdotz1 / rdotz2;                                 /* :97 */
+    /* p = z + beta p (:100) */
+    for (int64_t n = 0; n < Nall; ++n) p[n] = fma(beta, p[n], zsrc[n]);
+    /* Ap = A p; pAp = (p.Ap)_w; alpha = rdotz/pAp (:102-104) */
+    nbo_bp_apply(bp, p, Ap);
+    const double pAp = nbo_bp_wdot(bp, p, Ap);
+    alpha = rdotz1 / pAp;
+    if (hist_alpha) hist_alpha[it] = alpha;
+    if (!isfinite(alpha)) { breakdown = 1; iterations = it - 1; break; }
+    /* x += alpha p; r -= alpha Ap (:108) */
+    for (int64_t n = 0; n < Nall; ++n) {
+      x[n] = fma(alpha, p[n], x[n]);
+      r[n] = fma(-alpha, Ap[n], r[n]);
+    }
+  }
+  if (st) {
+    st->iterations = iterations; st->converged = converged; st->breakdown = breakdown; st->iter = it;
+    st->rdotr0 = rdotr0; st->rdotr = rdotr; st->TOL = TOL;
+  }
+  free(p); free(z); free(Ap); free(Ax); free(invDiag);
+  return iterations;
+}

@@ -1,0 +1,206 @@
+"""GPU parity: axhelm, gather-scatter and reductions through the C ABI against
+the CPU oracle on the same seeded inputs.  FP64 results are expected to be
+bit-identical (same operation order, explicit fma on both sides); the tests
+assert the documented tolerance (1e-12 relative) and, separately, record
+bit-exactness where the design promises it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def T(dev, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def H(t):
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def meshes(nbo):
+    out = {}
+    for N, dims in ((1, (3, 2, 2)), (2, (3, 2, 2)), (3, (4, 3, 2)), (5, (2, 2, 2)), (7, (4, 3, 3)),
+                    (9, (3, 2, 2)), (15, (2, 1, 1))):
+        box = nbo.Box.make(N, *dims, perturb=0.1, zaspect=2.0, seed=20 + N)
+        out[N] = (box, nbo.box_mesh(box))
+    return out
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 9, 15])
+@pytest.mark.parametrize("Nf,bk", [(1, 1), (1, 0), (3, 1), (3, 0)])
+def test_axhelm_f64_matches_oracle(nbx, nbo, dev, meshes, N, Nf, bk):
+    import torch
+    box, m = meshes[N]
+    E, off = box.E, box.E * box.Np
+    q = nbo.urand(N, 2, Nf * off)
+    lam = np.concatenate([1 + 0.5 * nbo.urand(N, 4, off), 0.5 + 0.4 * nbo.urand(N, 5, off)])
+    ref = nbo.axhelm(N, Nf, bk, E, off, m["ggeo"], m["D"], lam, q)
+    dq, dAq = T(dev, q), torch.full((Nf * off,), float("nan"), dtype=torch.float64, device=dev)
+    nbx.axhelm(N, Nf, bk, E, off, T(dev, m["ggeo"]), T(dev, m["D"]), T(dev, lam), dq, dAq)
+    torch.cuda.synchronize()
+    got = H(dAq)
+    assert relerr(got, ref) <= 1e-12
+    assert np.array_equal(got, ref), f"not bit-exact: rel {relerr(got, ref):.3e}"
+
+
+@pytest.mark.parametrize("N", [3, 7])
+def test_axhelm_partial_and_by_name(nbx, nbo, dev, meshes, N):
+    import torch
+    box, m = meshes[N]
+    E, off = box.E, box.E * box.Np
+    q = nbo.urand(3, 2, 3 * off)
+    lam = np.concatenate([1 + 0.5 * nbo.urand(3, 4, off), 0.5 + 0.4 * nbo.urand(3, 5, off)])
+    lst = np.array(sorted(set([E - 1, 0, E // 2, 1])), dtype=np.int32)[::-1].copy()
+    dg, dD, dl = T(dev, m["ggeo"]), T(dev, m["D"]), T(dev, lam)
+    for name, nf, bk in (("axhelmPartial_v0", 1, 0), ("axhelmPartial_bk_n3_v0", 3, 1), ("axhelm_bk_v0", 1, 1),
+                         ("axhelm_n3_v0", 3, 0)):
+        qq = np.ascontiguousarray(q[: nf * off])
+        partial = name.startswith("axhelmPartial")
+        init = np.full(nf * off, 7.0)
+        ref = nbo.axhelm(N, nf, bk, len(lst) if partial else E, off, m["ggeo"], m["D"], lam, qq,
+                         elementList=lst if partial else None, out=init.copy())
+        dAq = T(dev, init)
+        nbx.axhelm_by_name(name, N, len(lst) if partial else E, off, dg, dD, dl, T(dev, qq), dAq,
+                           elementList=T(dev, lst) if partial else None)
+        torch.cuda.synchronize()
+        assert np.array_equal(H(dAq), ref), name
+    with pytest.raises(nbx.NbxError):
+        nbx.axhelm_by_name("axhelm_bk_v", N, E, off, dg, dD, dl, T(dev, q[:off]), dAq)
+    with pytest.raises(nbx.NbxError):
+        nbx.axhelm_by_name("axhelmPartial_bk_v0", N, E, off, dg, dD, dl, T(dev, q[:off]), dAq)
+
+
+@pytest.mark.parametrize("N", [3, 7, 9])
+@pytest.mark.parametrize("bk", [1, 0])
+def test_axhelm_f32_matches_oracle(nbx, nbo, dev, meshes, N, bk):
+    import torch
+    box, m = meshes[N]
+    E, off = box.E, box.E * box.Np
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    q = f(nbo.urand(4, 2, off))
+    lam = f(np.concatenate([1 + 0.5 * nbo.urand(4, 4, off), 0.5 + 0.4 * nbo.urand(4, 5, off)]))
+    g32, D32 = f(m["ggeo"]), f(m["D"])
+    ref32 = nbo.axhelm_f32(N, 1, bk, E, off, g32, D32, lam, q)
+    ref64 = nbo.axhelm(N, 1, bk, E, off, m["ggeo"], m["D"], lam.astype(np.float64), q.astype(np.float64))
+    dAq = torch.empty(off, dtype=torch.float32, device=dev)
+    nbx.axhelm(N, 1, bk, E, off, T(dev, g32), T(dev, D32), T(dev, lam), T(dev, q), dAq, prec=nbx.NBX_F32)
+    torch.cuda.synchronize()
+    got = H(dAq)
+    assert np.array_equal(got, ref32)
+    assert relerr(got.astype(np.float64), ref64) < 5e-5
+
+
+@pytest.mark.parametrize("N", [3, 7])
+def test_jacobi_local_matches_oracle(nbx, nbo, dev, meshes, N):
+    import torch
+    box, m = meshes[N]
+    E, off = box.E, box.E * box.Np
+    lam = np.concatenate([1 + 0.5 * nbo.urand(6, 4, off), 0.5 + 0.4 * nbo.urand(6, 5, off)])
+    ref = nbo.jacobi_local(N, E, off, m["ggeo"], m["D"], lam)
+    d = torch.empty(off, dtype=torch.float64, device=dev)
+    nbx.update_jacobi_local(N, E, off, T(dev, m["ggeo"]), T(dev, m["D"]), T(dev, lam), d)
+    torch.cuda.synchronize()
+    assert np.array_equal(H(d), ref)
+
+
+def test_ogs_csr_matches_oracle(nbx, nbo):
+    rng = np.random.default_rng(3)
+    for ids in (rng.integers(-500, 501, size=5000).astype(np.int64),
+                nbo.box_mesh(nbo.Box.make(3, 3, 2, 2), mask_boundary=True)["ids"],
+                nbo.box_mesh(nbo.Box.make(2, 3, 3, 3, periodic=1))["ids"],
+                np.zeros(10, np.int64), np.arange(1, 11, dtype=np.int64)):
+        o = nbx.Ogs(ids)
+        oo, off, gid, invd = nbo.ogs_setup(ids)
+        try:
+            goff, gids = o.csr()
+            assert np.array_equal(goff, off) and np.array_equal(gids, gid)
+            s = o.sizes()
+            deg = np.diff(off)
+            assert s["NsharedGather"] == int((deg > 1).sum()) and s["Nshared"] == int(deg[deg > 1].sum())
+            import torch
+            if ids.size:
+                w = nbx._wrap_device(o.inv_degree_ptr(), ids.size, torch.float64)
+                assert np.array_equal(H(w), invd)
+        finally:
+            nbo.ogs_free(oo)
+            o.free()
+
+
+@pytest.mark.parametrize("type_,dtype", [("double", np.float64), ("float", np.float32), ("int", np.int32),
+                                         ("long long int", np.int64)])
+@pytest.mark.parametrize("op", ["add", "mul", "min", "max"])
+def test_gs_many_matches_oracle(nbx, nbo, dev, type_, dtype, op):
+    import torch
+    rng = np.random.default_rng(11)
+    box = nbo.Box.make(3, 4, 3, 3, perturb=0.1, seed=2)
+    ids = nbo.box_mesh(box, mask_boundary=True)["ids"]
+    n, k = ids.size, 3
+    stride = n + 13
+    if np.issubdtype(dtype, np.integer):
+        q = rng.integers(-5, 6, size=k * stride).astype(dtype)
+    else:
+        q = (rng.standard_normal(k * stride) * (0.9 if op == "mul" else 1.0)).astype(dtype)
+    oo, *_ = nbo.ogs_setup(ids)
+    o = nbx.Ogs(ids)
+    try:
+        ref = q.copy()
+        nbo.gs_many(oo, ref, k, stride, type_, op)
+        dq = T(dev, q)
+        o.gather_scatter_many(dq, k, stride, type_, op)
+        torch.cuda.synchronize()
+        assert np.array_equal(H(dq), ref)
+        # single-field form
+        ref1 = q[:n].copy()
+        nbo.gs_many(oo, ref1, 1, n, type_, op)
+        dq1 = T(dev, q[:n])
+        o.gather_scatter(dq1, type_, op)
+        torch.cuda.synchronize()
+        assert np.array_equal(H(dq1), ref1)
+    finally:
+        nbo.ogs_free(oo)
+        o.free()
+
+
+def test_gs_degree_round_trip_gpu(nbx, dev):
+    import torch
+    box = nbx.Box.make(7, 6, 5, 4, perturb=0.1, seed=9)
+    ids = nbx.box_mesh(box, mask_boundary=True)["ids"]
+    o = nbx.Ogs(ids)
+    q = torch.ones(ids.size, dtype=torch.float64, device=dev)
+    o.gather_scatter(q)
+    mult = H(q)
+    assert set(np.unique(mult)) <= {1.0, 2.0, 4.0, 8.0}
+    q = 1.0 / q
+    o.gather_scatter(q)
+    assert np.all(H(q) == 1.0)
+
+
+@pytest.mark.parametrize("n", [1, 255, 2048, 2049, 100000, 1 << 20])
+@pytest.mark.parametrize("Nf", [1, 3])
+def test_weighted_inner_product_matches_oracle_tree(nbx, nbo, dev, n, Nf):
+    w = 1.0 / (1 + np.floor(4 * np.abs(nbo.urand(1, 6, n))))
+    off = n + 3
+    x = nbo.urand(2, 7, Nf * off)
+    y = nbo.urand(3, 8, Nf * off)
+    got = nbx.weighted_inner_product(n, off, Nf, T(dev, w), T(dev, x), T(dev, y))
+    if Nf == 1:
+        ref = nbo.tree_wdot(w, x[:n].copy(), y[:n].copy())
+        assert got == ref
+    exact = sum(float(np.dot(w, x[f * off:f * off + n] * y[f * off:f * off + n])) for f in range(Nf))
+    assert abs(got - exact) <= 1e-12 * max(1.0, abs(exact)) * np.sqrt(n)
+
+
+def test_scaled_add(nbx, dev):
+    import torch
+    x = torch.linspace(-1, 1, 1001, dtype=torch.float64, device=dev)
+    y = torch.cos(x)
+    ref = -2.5 * x + 0.5 * y
+    nbx.scaled_add(1001, -2.5, x, 0.5, y)
+    torch.cuda.synchronize()
+    assert torch.equal(y, ref)

@@ -1,0 +1,140 @@
+"""Oracle axhelm / Jacobi pinned by the reference's compiled CPU kernels
+(golden vectors in tests/golden, and oracle/_ref live when it is built) and by
+the operator's mathematical invariants."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("N", [2, 7, 9])
+def test_oracle_matches_reference_golden(nbo, N):
+    d = np.load(GOLD / f"axhelm_N{N}.npz")
+    E, off = int(d["E"]), int(d["offset"])
+    ggeo, D, q, lam = d["ggeo"], d["D"], d["q"], d["lam"]
+    q1 = np.ascontiguousarray(q[:off])
+    cases = [("axhelm_v0", 1, 0, q1), ("axhelm_bk_v0", 1, 1, q1), ("axhelm_n3_v0", 3, 0, q),
+             ("axhelm_bk_n3_v0", 3, 1, q)]
+    for name, nf, bk, qq in cases:
+        got = nbo.axhelm(N, nf, bk, E, off, ggeo, D, lam, qq.copy())
+        assert relerr(got, d[name]) < 1e-14, name
+    # axhelmReference: same loop order, plain additions -> bit-exact
+    for l1 in (0.0, 1.1):
+        got = nbo.axhelm_reference(N + 1, E, l1, ggeo, D, q1.copy())
+        assert np.array_equal(got, d[f"axhelmReference_{l1}"])
+    # the non-BK kernel with lambda0 = 1, lambda1 = const is axhelmReference
+    lam_c = np.concatenate([np.ones(off), np.full(off, 1.1)])
+    got = nbo.axhelm(N, 1, 0, E, off, ggeo, D, lam_c, q1.copy())
+    assert relerr(got, d["axhelmReference_1.1"]) < 1e-14
+    # weighted dot: restated tree vs the reference's sequential OpenMP sum
+    ref = d["weightedInnerProductUpdate"]
+    w, a, b = d["wip_w"], d["wip_a"], d["wip_b"]
+    assert abs(nbo.tree_wdot(w, a, b) - ref[0]) < 1e-13 * abs(ref[0]) + 1e-15
+    assert abs(nbo.tree_wdot(w, a, a) - ref[1]) < 1e-13 * abs(ref[1])
+
+
+@pytest.mark.skipif(not Path(GOLD.parent.parent / "oracle" / "_ref" / "libaxhelm_N7.so").exists(),
+                    reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("N,dims", [(1, (2, 2, 2)), (3, (3, 2, 2)), (7, (4, 3, 2)), (9, (3, 2, 2))])
+def test_oracle_matches_reference_live(nbo, N, dims):
+    box = nbo.Box.make(N, *dims, perturb=0.1, zaspect=3.0, seed=99)
+    m = nbo.box_mesh(box)
+    E, off = box.E, box.E * box.Np
+    q = nbo.urand(5, 2, 3 * off)
+    lam = np.concatenate([1 + 0.3 * nbo.urand(5, 4, off), 2 + nbo.urand(5, 5, off)])
+    for name, nf, bk in (("axhelm_v0", 1, 0), ("axhelm_bk_v0", 1, 1), ("axhelm_n3_v0", 3, 0),
+                         ("axhelm_bk_n3_v0", 3, 1)):
+        qq = np.ascontiguousarray(q[: nf * off])
+        ref = nbo.ref_axhelm(N, name, E, off, m["ggeo"], m["D"], lam, qq.copy())
+        got = nbo.axhelm(N, nf, bk, E, off, m["ggeo"], m["D"], lam, qq.copy())
+        assert relerr(got, ref) < 1e-14, name
+
+
+def element_matrix(nbo, N, ggeo, D, lam=None, bk=True):
+    Np = (N + 1) ** 3
+    A = np.zeros((Np, Np))
+    lam = np.ones(2 * Np) if lam is None else lam
+    for c in range(Np):
+        e = np.zeros(Np)
+        e[c] = 1.0
+        A[:, c] = nbo.axhelm(N, 1, int(bk), 1, Np, ggeo, D, lam, e)
+    return A
+
+
+@pytest.mark.parametrize("N", [2, 3, 5])
+def test_operator_invariants(nbo, N):
+    box = nbo.Box.make(N, 1, 1, 1, perturb=0.0)
+    m = nbo.box_mesh(box)
+    Np = box.Np
+    A = element_matrix(nbo, N, m["ggeo"], m["D"])
+    # symmetric, constants in the kernel, positive semidefinite
+    assert np.abs(A - A.T).max() < 1e-12 * np.abs(A).max()
+    assert np.abs(A @ np.ones(Np)).max() < 1e-12 * np.abs(A).max()
+    assert np.linalg.eigvalsh(A).min() > -1e-12 * np.abs(A).max()
+    # energy: u^T A u = int |grad u|^2 exactly for u = x^a y^b on the unit cube
+    x, y, z = m["x"], m["y"], m["z"]
+    for (a, b) in ((1, 0), (1, 1), (2, 1)):
+        if a + b > N:
+            continue
+        u = x ** a * y ** b
+        gx2 = (a * a / (2 * a - 1) if a else 0.0) / (2 * b + 1)
+        gy2 = (b * b / (2 * b - 1) if b else 0.0) / (2 * a + 1)
+        assert abs(u @ A @ u - (gx2 + gy2)) < 1e-12
+    # mass term: u^T B u = int u^2 for degree <= N-1 (lambda0 = 0, lambda1 = 1)
+    lam = np.concatenate([np.zeros(Np), np.ones(Np)])
+    B = element_matrix(nbo, N, m["ggeo"], m["D"], lam, bk=False)
+    assert np.abs(B - np.diag(np.diag(B))).max() == 0.0
+    u = x + 2 * y
+    assert abs(u @ B @ u - (1 / 3 + 4 / 3 + 2 * 2 * 0.25)) < 1e-12
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_jacobi_is_element_diagonal(nbo, N):
+    box = nbo.Box.make(N, 1, 1, 1, perturb=0.0)
+    m = nbo.box_mesh(box)
+    Np = box.Np
+    # perturbed single element: G is full, lambda0 varies per node
+    box2 = nbo.Box.make(N, 3, 3, 3, perturb=0.15, seed=4)
+    m2 = nbo.box_mesh(box2)
+    g = np.ascontiguousarray(m2["ggeo"].reshape(27, 7 * Np)[13])
+    lam = np.concatenate([1 + 0.5 * nbo.urand(2, 4, Np), 0.7 + 0.2 * nbo.urand(2, 5, Np)])
+    A = element_matrix(nbo, N, g, m["D"], lam, bk=False)
+    diag = nbo.jacobi_local(N, 1, Np, g, m["D"], lam)
+    assert np.abs(diag - np.diag(A)).max() < 1e-13 * np.abs(A).max()
+    mapB = np.zeros(Np, dtype=np.int32)
+    mapB[::3] = 1
+    d2 = nbo.jacobi_local(N, 1, Np, g, m["D"], lam, mapB)
+    assert np.all(d2[::3] == 1.0) and np.array_equal(d2[1::3], diag[1::3])
+
+
+def test_partial_element_list(nbo):
+    N = 3
+    box = nbo.Box.make(N, 3, 2, 1, perturb=0.1, seed=8)
+    m = nbo.box_mesh(box)
+    E, off = box.E, box.E * box.Np
+    q = nbo.urand(3, 2, off)
+    full = nbo.axhelm(N, 1, 1, E, off, m["ggeo"], m["D"], None, q)
+    lst = np.array([4, 1, 5], dtype=np.int32)
+    part = nbo.axhelm(N, 1, 1, 3, off, m["ggeo"], m["D"], None, q, elementList=lst, out=np.full(off, 7.0))
+    Np = box.Np
+    for e in range(E):
+        blk = slice(e * Np, (e + 1) * Np)
+        if e in lst:
+            assert np.array_equal(part[blk], full[blk])
+        else:
+            assert np.all(part[blk] == 7.0)
+
+
+def test_f32_restatement_close(nbo):
+    N = 7
+    d = np.load(GOLD / f"axhelm_N{N}.npz")
+    E, off = int(d["E"]), int(d["offset"])
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    got = nbo.axhelm_f32(N, 1, 0, E, off, f(d["ggeo"]), f(d["D"]), f(d["lam"]), f(d["q"][:off]))
+    assert relerr(got.astype(np.float64), d["axhelm_v0"]) < 2e-5

@@ -1,0 +1,97 @@
+"""Oracle BP operator / PCG pinned by a dense direct solve of the assembled
+system, CG theory, and nekBone's manufactured-solution check."""
+import numpy as np
+import pytest
+
+
+def assembled_matrix(ob):
+    """Dense global matrix on the unique unmasked nodes, by applying the
+    oracle operator (axhelm + gs + mask) to the continuous unit vectors."""
+    off, gid = ob.off, ob.gid
+    G = len(off) - 1
+    A = np.zeros((G, G))
+    first = gid[off[:-1]]
+    for g in range(G):
+        e = np.zeros(ob.offset)
+        e[gid[off[g]:off[g + 1]]] = 1.0
+        A[:, g] = ob.apply(e)[first]
+    return A, first
+
+
+@pytest.mark.parametrize("N,dims,bpmode,l1", [(3, (2, 2, 2), True, 0.0), (2, (3, 2, 2), False, 0.9)])
+def test_pcg_matches_dense_solve(nbo, N, dims, bpmode, l1):
+    box = nbo.Box.make(N, *dims, perturb=0.1, seed=4)
+    ob = nbo.OracleBP(box, bpmode=bpmode, lambda1=l1)
+    A, first = assembled_matrix(ob)
+    assert np.abs(A - A.T).max() < 1e-12 * np.abs(A).max()
+    assert np.linalg.eigvalsh(A).min() > 0
+    xs = np.linalg.solve(A, ob.rhs[first])
+    for jac in (True, False):
+        r, x = ob.rhs.copy(), np.zeros(ob.offset)
+        st, hr, ha = ob.pcg(r, x, 1e-13, 500, jacobi=jac)
+        assert st.converged and st.iterations < A.shape[0] + 5
+        assert np.abs(x[first] - xs).max() < 1e-10 * np.abs(xs).max()
+        # x is continuous (equal at every member of a group) and zero on the mask
+        for g in range(len(ob.off) - 1):
+            mem = ob.gid[ob.off[g]:ob.off[g + 1]]
+            assert np.all(x[mem] == x[mem[0]])
+        assert np.all(x[ob.maskIds] == 0.0)
+        # the recorded residual norms are the weighted |r|^2 of CG (decrease overall)
+        assert hr[1] == st.rdotr0 and hr[st.iterations + 1] <= 1e-26 * hr[1] * 1e3
+
+
+def test_pcg_invariants_first_iterations(nbo):
+    """Textbook PCG in numpy on the dense system, same recurrences: the first
+    iterates' alpha and |r|_w^2 agree to rounding."""
+    box = nbo.Box.make(3, 2, 2, 2, perturb=0.1, seed=6)
+    ob = nbo.OracleBP(box, bpmode=False, lambda1=0.5)
+    A, first = assembled_matrix(ob)
+    # Jacobi on the assembled diagonal
+    Minv = 1.0 / np.diag(A)
+    b = ob.rhs[first]
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = Minv * r
+    p = z.copy()
+    rz = r @ z
+    alphas, rr = [], [r @ r]
+    for _ in range(8):
+        Ap = A @ p
+        a = rz / (p @ Ap)
+        alphas.append(a)
+        x += a * p
+        r -= a * Ap
+        rr.append(r @ r)
+        z = Minv * r
+        rz, rz_old = r @ z, rz
+        p = z + (rz / rz_old) * p
+    rl, xl = ob.rhs.copy(), np.zeros(ob.offset)
+    st, hr, ha = ob.pcg(rl, xl, 0.0, 8, jacobi=True)
+    assert np.allclose(ha[1:9], alphas, rtol=1e-10, atol=0)
+    assert np.allclose(hr[1:9], rr[:8], rtol=1e-9, atol=0)
+
+
+def test_flexible_equals_standard_for_fixed_preconditioner(nbo):
+    box = nbo.Box.make(3, 2, 3, 2, perturb=0.1, seed=8)
+    ob = nbo.OracleBP(box, bpmode=False, lambda1=0.2)
+    r1, x1 = ob.rhs.copy(), np.zeros(ob.offset)
+    st1, h1, _ = ob.pcg(r1, x1, 1e-10, 300, jacobi=True, flexible=False)
+    r2, x2 = ob.rhs.copy(), np.zeros(ob.offset)
+    st2, h2, _ = ob.pcg(r2, x2, 1e-10, 300, jacobi=True, flexible=True)
+    assert abs(st1.iterations - st2.iterations) <= 1
+    assert np.abs(x1 - x2).max() < 1e-8 * np.abs(x1).max()
+
+
+def test_manufactured_solution(nbo):
+    # nekBone's check (main.cpp:105-132): periodic [-1,1]^3, lambda = 1.1
+    errs = []
+    for N in (3, 5):
+        box = nbo.Box.make(N, 4, 4, 4, periodic=1, extent=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0))
+        ob = nbo.OracleBP(box, bpmode=False, lambda1=1.1, dirichlet=False, rhs_kind=1)
+        r, x = ob.rhs.copy(), np.zeros(ob.offset)
+        st, _, _ = ob.pcg(r, x, 1e-12, 2000, jacobi=True)
+        assert st.converged
+        m = ob.mesh
+        exact = np.cos(np.pi * m["x"]) * np.cos(np.pi * m["y"]) * np.cos(np.pi * m["z"])
+        errs.append(np.abs(x - exact).max())
+    assert errs[1] < errs[0] / 10 and errs[1] < 1e-3
