@@ -102,6 +102,13 @@ def lib() -> C.CDLL:
     """Load libnbx.so (built by nekbench_amd.build); raises if it is absent."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64.so.7; loading it first makes libnbx's
+        # DT_NEEDED bind to that same runtime, so device pointers and streams
+        # from torch are valid inside the library (one HIP runtime per process)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not LIB_PATH.exists():
             raise NbxError(f"{LIB_PATH} not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
         l = C.CDLL(str(LIB_PATH))

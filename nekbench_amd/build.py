@@ -24,7 +24,7 @@ HIPCC = shutil.which("hipcc") or f"{ROCM}/bin/hipcc"
 ARCH = os.environ.get("NBX_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-mcode-object-version=5", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", f"-I{INCLUDE}",
 ]
