@@ -12,7 +12,9 @@ value = nekBone's throughput convention Nfields * iterations * E * N^3 / t
 (GDOF/s, nekBone/main.cpp:156-158) with t the wall time of one solve.
 roofline: the FP64 axhelm kernel of the solve (BK, N=7, 32768 elements),
 algorithmic bytes E*Np*(2+7)*8 per launch (axhelm/main.cpp:162-163) over the
-average launch time measured with HIP events on the library's stream.
+average launch time measured with HIP events on the library's stream;
+traffic = HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE runs (profiles/traffic.json, calibrated on a 1 GiB copy).
 cpu_baseline: the CPU oracle's restatement of the same solve on a bounded
 sample (setup + 5 PCG iterations of the same problem), OpenMP on the host.
 """
@@ -66,6 +68,9 @@ def main():
     ap.add_argument("--ax-launches", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="short fixed sequence for rocprofv3 counter runs: a 1 GiB device copy (calibration), "
+                         "5 axhelm launches, one 10-iteration PCG solve; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -91,6 +96,24 @@ def main():
         r.copy_(b)
         x.zero_()
         return bp.pcg(r, x, args.tol, args.maxit)
+
+    if args.profile_pass:
+        src = torch.zeros(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        ggeo = bp.device_array("ggeo", bp.Nelements * 7 * bp.Np)
+        D = bp.device_array("D", (N + 1) ** 2)
+        Ap = torch.empty_like(x)
+        x.copy_(b)
+        for _ in range(5):
+            nbx.axhelm(N, 1, 1, bp.Nelements, n, ggeo, D, None, x, Ap)
+        r.copy_(b)
+        x.zero_()
+        bp.pcg(r, x, 0.0, 10)
+        torch.cuda.synchronize()
+        log("[bench] profile pass done")
+        return
 
     for _ in range(args.warmup):
         st = step()
@@ -125,6 +148,10 @@ def main():
     log(f"[bench] axhelm BK N=7 E={E}: {ax_s * 1e6:.1f} us/launch, {ax_gbs:.0f} GB/s, "
         f"{E * N ** 3 / ax_s / 1e9:.1f} GDOF/s")
 
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists() and NX == 32:
+        traffic = json.loads(tfile.read_text()).get("axhelm_bk_f64_n7_32cube", {}).get("traffic_bytes_per_launch")
     out = {
         "metric": METRIC, "value": round(gdofs, 4), "unit": "GDOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
@@ -135,7 +162,7 @@ def main():
                    "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
                    "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_axhelm<double,8,1,BK>", "bytes_per_launch": ax_bytes,
                      "us_per_launch": round(ax_s * 1e6, 3)},
         "cpu_baseline": None,

@@ -170,6 +170,10 @@ typedef struct {
   int rhs_kind;         /* 0: U[-1,1) per local node (seed stream 3+field), masked, assembled;
                            1: (3 pi^2 + lambda1) JW cos(pi x) cos(pi y) cos(pi z), assembled
                               (nekBone/setup.cpp:77-99) */
+  int fused;            /* 0: the reference's kernel sequence (pAp = <p, gs(A p)>_w);
+                           1: p-update and pAp = sum_local p (A_L p) fused into the operator
+                              pass, gather-scatter and mask in one launch (same algorithm,
+                              pAp rounded differently) */
 } nbx_bp_options;
 
 typedef struct {

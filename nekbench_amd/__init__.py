@@ -56,7 +56,7 @@ class BPOptions(C.Structure):
     """nbx_bp_options (include/nbx.h)."""
     _fields_ = [("box", Box), ("Nfields", C.c_int), ("bpmode", C.c_int), ("lambda1", C.c_double),
                 ("dirichlet", C.c_int), ("precon", C.c_int), ("flexible", C.c_int), ("verbose", C.c_int),
-                ("rhs_kind", C.c_int)]
+                ("rhs_kind", C.c_int), ("fused", C.c_int)]
 
 
 class PCGStats(C.Structure):
@@ -266,9 +266,9 @@ class BP:
     (nbx_bp_setup / nbx_bp_pcg / nbx_bp_ax_operator)."""
 
     def __init__(self, box: Box, Nfields=1, bpmode=False, lambda1=0.0, dirichlet=True, precon=PRECON_JACOBI,
-                 flexible=False, verbose=False, rhs_kind=0, stream=None):
+                 flexible=False, verbose=False, rhs_kind=0, fused=True, stream=None):
         self.opt = BPOptions(box, Nfields, int(bpmode), lambda1, int(dirichlet), precon, int(flexible),
-                             int(verbose), rhs_kind)
+                             int(verbose), rhs_kind, int(fused))
         self.h = lib().nbx_bp_setup(C.byref(self.opt), stream_ptr(stream))
         if not self.h:
             raise NbxError(f"nbx_bp_setup failed: {last_error()}")

@@ -17,6 +17,7 @@
 //   Gqr = fma(G02,qt, fma(G01,qs, G00*qr)), Gqs, Gqt alike; non-BK * lambda0
 //   Aq  = ((Aqr + Aqs) + Aqt), non-BK: fma(GWJ*lambda1, q, .)
 #include "nbx_internal.h"
+#include "nbx_reduce.h"
 #include <cstring>
 #include <cstdio>
 #include <cstdlib>
@@ -30,6 +31,87 @@ __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { r
 template <>
 __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Per-workgroup state of the operator: LDS slabs and the lane's D vectors.
+template <typename T, int NQ>
+struct AxLane {
+  T Di[NQ], Dj[NQ], DTi[NQ], DTj[NQ];
+  int t, i, j;
+  __device__ __forceinline__ void init(const T* s_D)
+  {
+    t = threadIdx.x;
+    i = t % NQ;
+    j = t / NQ;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      Di[m] = s_D[i * NQ + m];
+      Dj[m] = s_D[j * NQ + m];
+      DTi[m] = s_D[m * NQ + i];
+      DTj[m] = s_D[m * NQ + j];
+    }
+  }
+};
+
+// One field of one element: out[k] = (A q)(i,j,k) for the lane's column rq.
+// g, l0, l1 point at the lane's node (k = 0) of ggeo, lambda0, lambda1.
+template <typename T, int NQ, bool BK>
+__device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&rq)[NQ], T (&out)[NQ],
+                                              const T* __restrict__ s_D, T* s_q, T* s_Gr, T* s_Gs,
+                                              const T* __restrict__ g, const T* __restrict__ l0,
+                                              const T* __restrict__ l1)
+{
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  T rAt[NQ], rArs[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) rAt[k] = T(0);
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    __syncthreads();
+    s_q[L.t] = rq[k];
+    __syncthreads();
+    T qr = T(0), qs = T(0), qt = T(0);
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      qr = fmaT(L.Di[m], s_q[L.j * NQ + m], qr);
+      qs = fmaT(L.Dj[m], s_q[m * NQ + L.i], qs);
+      qt = fmaT(s_D[k * NQ + m], rq[m], qt);
+    }
+    const int n = k * NQ2;
+    const T G00 = ldnt(&g[kG00 * NP + n]), G01 = ldnt(&g[kG01 * NP + n]), G02 = ldnt(&g[kG02 * NP + n]);
+    const T G11 = ldnt(&g[kG11 * NP + n]), G12 = ldnt(&g[kG12 * NP + n]), G22 = ldnt(&g[kG22 * NP + n]);
+    T Gr = fmaT(G02, qt, fmaT(G01, qs, G00 * qr));
+    T Gs = fmaT(G12, qt, fmaT(G11, qs, G01 * qr));
+    T Gt = fmaT(G22, qt, fmaT(G12, qs, G02 * qr));
+    if (!BK) {
+      const T lam0 = ldnt(&l0[n]);
+      Gr = lam0 * Gr;
+      Gs = lam0 * Gs;
+      Gt = lam0 * Gt;
+    }
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) rAt[m] = fmaT(s_D[k * NQ + m], Gt, rAt[m]);
+    s_Gr[L.t] = Gr;
+    s_Gs[L.t] = Gs;
+    __syncthreads();
+    T ar = T(0), as = T(0);
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      ar = fmaT(L.DTi[m], s_Gr[L.j * NQ + m], ar);
+      as = fmaT(L.DTj[m], s_Gs[m * NQ + L.i], as);
+    }
+    rArs[k] = ar + as;
+  }
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    T res = rArs[k] + rAt[k];
+    if (!BK) {
+      const int n = k * NQ2;
+      res = fmaT(ldnt(&g[kGWJ * NP + n]) * ldnt(&l1[n]), rq[k], res);
+    }
+    out[k] = res;
+  }
+}
+
 template <typename T, int NQ, int NF, bool BK, bool PARTIAL>
 __global__ __launch_bounds__(NQ* NQ) void k_axhelm(const int32_t Nelements, const int32_t offset,
                                                    const int32_t* __restrict__ elementList,
@@ -40,87 +122,87 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm(const int32_t Nelements, cons
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   if (skip && *skip) return;
-  __shared__ T s_D[NQ2];
-  __shared__ T s_q[NQ2];
-  __shared__ T s_Gr[NQ2];
-  __shared__ T s_Gs[NQ2];
-
-  const int t = threadIdx.x;
-  const int i = t % NQ;
-  const int j = t / NQ;
+  __shared__ T s_D[NQ2], s_q[NQ2], s_Gr[NQ2], s_Gs[NQ2];
   const int64_t e = PARTIAL ? (int64_t)elementList[blockIdx.x] : (int64_t)blockIdx.x;
-
-  s_D[t] = D[t];
+  s_D[threadIdx.x] = D[threadIdx.x];
   __syncthreads();
-  T Di[NQ], Dj[NQ], DTi[NQ], DTj[NQ];
-#pragma unroll
-  for (int m = 0; m < NQ; ++m) {
-    Di[m] = s_D[i * NQ + m];
-    Dj[m] = s_D[j * NQ + m];
-    DTi[m] = s_D[m * NQ + i];
-    DTj[m] = s_D[m * NQ + j];
-  }
-
-  const T* __restrict__ g = ggeo + e * (int64_t)(kNggeo * NP) + t;
-  const T* __restrict__ l0 = lambda + e * NP + t;
-  const T* __restrict__ l1 = lambda + e * NP + offset + t;
-
+  AxLane<T, NQ> L;
+  L.init(s_D);
+  const T* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
+  const T* l0 = lambda + e * NP + L.t;
+  const T* l1 = lambda + e * NP + offset + L.t;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    const int64_t base = e * NP + (int64_t)f * offset + t;
-    T rq[NQ], rAt[NQ], rArs[NQ];
+    const int64_t base = e * NP + (int64_t)f * offset + L.t;
+    T rq[NQ], out[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) rq[k] = ldnt(&q[base + k * NQ2]);
+    axhelm_column<T, NQ, BK>(L, rq, out, s_D, s_q, s_Gr, s_Gs, g, l0, l1);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) stnt(out[k], &Aq[base + k * NQ2]);
+  }
+}
+
+// PCG form (FP64): p = z + beta p (BPScaledAdd, nekBone/solve.cpp:100), then
+// Ap_local = A p, and the element's share of pAp = sum_nodes p * Ap_local.
+// Because p is continuous and zero on the mask, sum_local p (A_L p) equals
+// <p, gs(A_L p)>_w exactly in real arithmetic (the weights 1/multiplicity
+// undo the gather-scatter); this value is what BPAxOperator's dot returns.
+// Lane (i,j) accumulates fma(p, Ap) over fields then k; the NQ*NQ lanes are
+// summed by a halving tree over the next power of two (zero padded); the
+// element partials go to partials[e].
+template <int NQ, int NF, bool BK>
+__global__ __launch_bounds__(NQ* NQ) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
+                                                       const double* __restrict__ ggeo, const double* __restrict__ D,
+                                                       const double* __restrict__ lambda,
+                                                       const double* __restrict__ z, double* __restrict__ p,
+                                                       double* __restrict__ Ap, double* __restrict__ partials,
+                                                       const PcgFlags* __restrict__ S)
+{
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int P2 = NQ2 <= 1 ? 1 : (NQ2 <= 2 ? 2 : (NQ2 <= 4 ? 4 : (NQ2 <= 8 ? 8 : (NQ2 <= 16 ? 16 : (NQ2 <= 32 ? 32 : (NQ2 <= 64 ? 64 : (NQ2 <= 128 ? 128 : 256)))))));
+  if (S->done) return;
+  const double beta = S->beta;
+  __shared__ double s_D[NQ2], s_q[NQ2], s_Gr[NQ2], s_Gs[NQ2], s_red[P2];
+  const int64_t e = blockIdx.x;
+  s_D[threadIdx.x] = D[threadIdx.x];
+  __syncthreads();
+  AxLane<double, NQ> L;
+  L.init(s_D);
+  const double* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
+  const double* l0 = lambda + e * NP + L.t;
+  const double* l1 = lambda + e * NP + offset + L.t;
+  double acc = 0.0;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int64_t base = e * NP + (int64_t)f * offset + L.t;
+    double rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      rq[k] = q[base + k * NQ2];
-      rAt[k] = T(0);
+      rq[k] = __builtin_fma(beta, p[base + k * NQ2], ldnt(&z[base + k * NQ2]));
+      p[base + k * NQ2] = rq[k];
     }
+    axhelm_column<double, NQ, BK>(L, rq, out, s_D, s_q, s_Gr, s_Gs, g, l0, l1);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      __syncthreads();
-      s_q[t] = rq[k];
-      __syncthreads();
-      T qr = T(0), qs = T(0), qt = T(0);
-#pragma unroll
-      for (int m = 0; m < NQ; ++m) {
-        qr = fmaT(Di[m], s_q[j * NQ + m], qr);
-        qs = fmaT(Dj[m], s_q[m * NQ + i], qs);
-        qt = fmaT(s_D[k * NQ + m], rq[m], qt);
-      }
-      const int n = k * NQ2;
-      const T G00 = g[kG00 * NP + n], G01 = g[kG01 * NP + n], G02 = g[kG02 * NP + n];
-      const T G11 = g[kG11 * NP + n], G12 = g[kG12 * NP + n], G22 = g[kG22 * NP + n];
-      T Gr = fmaT(G02, qt, fmaT(G01, qs, G00 * qr));
-      T Gs = fmaT(G12, qt, fmaT(G11, qs, G01 * qr));
-      T Gt = fmaT(G22, qt, fmaT(G12, qs, G02 * qr));
-      if (!BK) {
-        const T lam0 = l0[n];
-        Gr = lam0 * Gr;
-        Gs = lam0 * Gs;
-        Gt = lam0 * Gt;
-      }
-#pragma unroll
-      for (int m = 0; m < NQ; ++m) rAt[m] = fmaT(s_D[k * NQ + m], Gt, rAt[m]);
-      s_Gr[t] = Gr;
-      s_Gs[t] = Gs;
-      __syncthreads();
-      T ar = T(0), as = T(0);
-#pragma unroll
-      for (int m = 0; m < NQ; ++m) {
-        ar = fmaT(DTi[m], s_Gr[j * NQ + m], ar);
-        as = fmaT(DTj[m], s_Gs[m * NQ + i], as);
-      }
-      rArs[k] = ar + as;
-    }
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      T res = rArs[k] + rAt[k];
-      if (!BK) {
-        const int n = k * NQ2;
-        res = fmaT(g[kGWJ * NP + n] * l1[n], rq[k], res);
-      }
-      Aq[base + k * NQ2] = res;
+      stnt(out[k], &Ap[base + k * NQ2]);
+      acc = __builtin_fma(rq[k], out[k], acc);
     }
   }
+  double sum;
+  if (NQ2 == 64) {
+    sum = wave_sum(acc);
+  } else {
+    s_red[L.t] = acc;
+    if (L.t + NQ2 < P2) s_red[L.t + NQ2] = 0.0;
+    for (int h = P2 / 2; h >= 1; h >>= 1) {
+      __syncthreads();
+      if (L.t < h) s_red[L.t] = s_red[L.t] + s_red[L.t + h];
+    }
+    sum = s_red[0];
+  }
+  if (L.t == 0) partials[e] = sum;
 }
 
 // Local Jacobi diagonal, nekBone/kernel/updateJacobi.okl:27-108, same
@@ -242,6 +324,50 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
     default:
       return fail(NBX_ERR_UNSUPPORTED, "axhelm: precision %d not supported by nbx_axhelm", (int)prec);
   }
+}
+
+int axhelm_pcg_launch(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const double* ggeo,
+                      const double* D, const double* lambda, const double* z, double* p, double* Ap,
+                      double* partials, const PcgFlags* S, hipStream_t s)
+{
+#define NBX_AXP(NV)                                                                                           \
+  case NV:                                                                                                    \
+    if (Nf == 1 && bk)                                                                                        \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, true>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,     \
+                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
+    else if (Nf == 1)                                                                                         \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, false>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,    \
+                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
+    else if (bk)                                                                                              \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,     \
+                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,    \
+                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
+    break;
+  if (Nf != 1 && Nf != 3) return fail(NBX_ERR_UNSUPPORTED, "axhelm_pcg: Nfields=%d", Nf);
+  switch (N) {
+    NBX_AXP(1)
+    NBX_AXP(2)
+    NBX_AXP(3)
+    NBX_AXP(4)
+    NBX_AXP(5)
+    NBX_AXP(6)
+    NBX_AXP(7)
+    NBX_AXP(8)
+    NBX_AXP(9)
+    NBX_AXP(10)
+    NBX_AXP(11)
+    NBX_AXP(12)
+    NBX_AXP(13)
+    NBX_AXP(14)
+    NBX_AXP(15)
+    default:
+      return fail(NBX_ERR_UNSUPPORTED, "axhelm_pcg: N=%d", N);
+  }
+#undef NBX_AXP
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
 }
 
 }  // namespace nbx

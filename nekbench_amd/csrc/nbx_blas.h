@@ -4,11 +4,10 @@
 
 namespace nbx {
 
-// Device scratch of a deterministic reduction: one partial per tile and value,
-// an arrival ticket (zero between launches) and a small result slot.
+// Device scratch of a deterministic reduction: one partial per tile and value
+// and a small result slot.
 struct ReduceWork {
   double* partials = nullptr;
-  unsigned int* ticket = nullptr;
   double* result = nullptr;
   unsigned int capacity = 0;  // tiles
   int init(int64_t maxN, int nvals);

@@ -11,8 +11,7 @@ namespace nbx {
 template <bool XX>
 __global__ __launch_bounds__(256) void k_wdot(const int32_t N, const int32_t offset, const int Nf,
                                               const double* __restrict__ w, const double* __restrict__ x,
-                                              const double* __restrict__ y, double* __restrict__ partials,
-                                              unsigned int* __restrict__ ticket, double* __restrict__ out)
+                                              const double* __restrict__ y, double* __restrict__ partials)
 {
   constexpr int NV = XX ? 2 : 1;
   __shared__ double s4[NV * 4];
@@ -34,7 +33,17 @@ __global__ __launch_bounds__(256) void k_wdot(const int32_t N, const int32_t off
     }
   }
   block_sum256<NV>(v, s4);
-  if (last_block_combine<NV>(v, partials, ticket, s4) && threadIdx.x == 0) {
+  store_partials<NV>(v, partials);
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_wdot_finish(const unsigned int nb, const double* __restrict__ partials,
+                                                     double* __restrict__ out)
+{
+  __shared__ double s4[NV * 4];
+  double v[NV];
+  combine_partials<NV>(v, partials, nb, s4);
+  if (threadIdx.x == 0) {
 #pragma unroll
     for (int c = 0; c < NV; ++c) out[c] = v[c];
   }
@@ -54,12 +63,13 @@ int launch_wdot(int32_t N, int32_t offset, int Nf, const double* w, const double
   if (N <= 0) return fail(NBX_ERR_INVALID, "wdot: N=%d", N);
   const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
   if (nb > ws.capacity) return fail(NBX_ERR_INVALID, "wdot: workspace too small (%u < %u)", ws.capacity, nb);
-  if (xx)
-    hipLaunchKernelGGL(k_wdot<true>, dim3(nb), dim3(256), 0, s, N, offset, Nf, w, x, y, ws.partials, ws.ticket,
-                       out_dev);
-  else
-    hipLaunchKernelGGL(k_wdot<false>, dim3(nb), dim3(256), 0, s, N, offset, Nf, w, x, y, ws.partials, ws.ticket,
-                       out_dev);
+  if (xx) {
+    hipLaunchKernelGGL(k_wdot<true>, dim3(nb), dim3(256), 0, s, N, offset, Nf, w, x, y, ws.partials);
+    hipLaunchKernelGGL(k_wdot_finish<2>, dim3(1), dim3(256), 0, s, nb, ws.partials, out_dev);
+  } else {
+    hipLaunchKernelGGL(k_wdot<false>, dim3(nb), dim3(256), 0, s, N, offset, Nf, w, x, y, ws.partials);
+    hipLaunchKernelGGL(k_wdot_finish<1>, dim3(1), dim3(256), 0, s, nb, ws.partials, out_dev);
+  }
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
 }
@@ -69,8 +79,6 @@ int ReduceWork::init(int64_t maxN, int nvals)
   capacity = (unsigned int)((maxN + kRedTile - 1) / kRedTile);
   if (capacity < 1) capacity = 1;
   NBX_HIP(hipMalloc(&partials, sizeof(double) * capacity * (size_t)nvals));
-  NBX_HIP(hipMalloc(&ticket, sizeof(unsigned int) * 4));
-  NBX_HIP(hipMemset(ticket, 0, sizeof(unsigned int) * 4));
   NBX_HIP(hipMalloc(&result, sizeof(double) * 8));
   return NBX_SUCCESS;
 }
@@ -78,10 +86,8 @@ int ReduceWork::init(int64_t maxN, int nvals)
 void ReduceWork::release()
 {
   if (partials) (void)hipFree(partials);
-  if (ticket) (void)hipFree(ticket);
   if (result) (void)hipFree(result);
   partials = nullptr;
-  ticket = nullptr;
   result = nullptr;
   capacity = 0;
 }

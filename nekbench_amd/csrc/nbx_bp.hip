@@ -2,24 +2,28 @@
 // nekBone/solve.cpp:44-293) on one GPU.
 //
 // The PCG loop follows BPPCG (solve.cpp:44-113) step for step, with every
-// scalar kept on the device: the reduction kernels finish with a last-block
-// pass that also performs the scalar recurrences (TOL, beta, alpha and the
-// stopping test), and every kernel of an iteration starts by reading the
-// device `done` flag, so the host enqueues iterations in chunks and reads the
+// scalar kept on the device: each reduction ends in a one-block kernel that
+// also performs the scalar recurrences (TOL, beta, alpha and the stopping
+// test), and every kernel of an iteration starts by reading the device
+// `done` flag, so the host enqueues iterations in chunks and reads the
 // scalars back once per chunk instead of twice per iteration.
 //
 // Per-iteration kernels (reference call in brackets):
-//   1. z = M r, <r,z>_w, <r,r>_w [, <z,Ap>_w]  (BPPreconditioner + BPWeightedInnerProduct2 [+ flexible dot])
+//   1. x += alpha p, r -= alpha Ap of the previous iteration (BPUpdatePCG),
+//      z = M r, tile partials of <r,z>_w, <r,r>_w [, <z,Ap>_w]
+//      (BPPreconditioner + BPWeightedInnerProduct2 [+ flexible dot]);
+//      one-block finish: TOL, stopping test, beta
 //   2. p = z + beta p                          (BPScaledAdd)
 //   3. Ap = axhelm(p)                          (BPAxOperator: kernel)
 //   4. Ap = gs(Ap), Ap[mask] = 0               (BPAxOperator: oogs::start/finish; mask)
-//   5. pAp = <p,Ap>_w, alpha = rdotz/pAp       (BPAxOperator: BPWeightedInnerProduct)
-//   6. x += alpha p, r -= alpha Ap             (BPUpdatePCG)
-// Element-wise arithmetic uses one fma per update; the dots use the fixed
-// reduction tree of nbx_reduce.h.  The CPU oracle (nbo_pcg) restates exactly
-// this sequence.
+//   5. pAp = <p,Ap>_w, one-block finish: alpha = rdotz/pAp (BPWeightedInnerProduct)
+// Moving the update into the next iteration's first pass changes no value:
+// every element-wise operation is the same single fma in the same order.
+// The dots use the fixed reduction tree of nbx_reduce.h.  The CPU oracle
+// (nbo_bp_pcg) restates exactly this arithmetic.
 #include "nbx_blas.h"
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,17 +32,23 @@
 
 namespace nbx {
 
+// the leading members are PcgFlags (nbx_internal.h), read by operator kernels
 struct PcgScalars {
-  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, beta, pAp, zdotAp;
-  int iter, done, iterations, converged, breakdown, maxit, fixed, flexible;
+  int done, pad_;
+  double beta;
+  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, pAp, zdotAp;
+  int iter, iterations, converged, breakdown, maxit, fixed, flexible;
 };
+static_assert(offsetof(PcgScalars, done) == offsetof(PcgFlags, done) &&
+                  offsetof(PcgScalars, beta) == offsetof(PcgFlags, beta),
+              "PcgScalars must start with PcgFlags");
 
 __device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, double rdotr, double zdotAp,
                                                double* hist_rdotr)
 {
   const int it = S->iter + 1;
   S->iter = it;
-  if (it > S->maxit) {  // the reference's loop ends before this iteration's work (solve.cpp:76)
+  if (it > S->maxit) {  // loop end (solve.cpp:76); the last update has been applied
     S->done = 1;
     S->iterations = S->maxit;
     return;
@@ -70,18 +80,18 @@ __device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, doub
     S->beta = (it == 1) ? 0.0 : S->rdotz1 / S->rdotz2;  // solve.cpp:97
 }
 
-// 1. z = M r (Jacobi: invDiag r; copy: z is r itself) and the dots
-template <bool JAC, bool FLEX>
-__global__ __launch_bounds__(256) void k_pcg_precon_dots(const int32_t N, const int32_t offset, const int Nf,
-                                                         const double* __restrict__ w,
-                                                         const double* __restrict__ invDiag,
-                                                         const double* __restrict__ r, double* __restrict__ z,
-                                                         const double* __restrict__ Ap,
-                                                         double* __restrict__ partials,
-                                                         unsigned int* __restrict__ ticket, PcgScalars* S,
-                                                         double* __restrict__ hist_rdotr)
+// 1. [x += alpha p, r -= alpha Ap of the previous iteration]; z = M r
+//    (Jacobi: invDiag r; copy: z is r itself); tile partials of the dots
+template <bool JAC, bool FLEX, bool DEG>
+__global__ __launch_bounds__(256) void k_pcg_update_precon_dots(
+    const int32_t N, const int32_t offset, const int Nf, const double* __restrict__ w, const uint8_t* __restrict__ deg,
+    const double* __restrict__ invDiag, const double* __restrict__ p, const double* __restrict__ Ap,
+    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, double* __restrict__ partials,
+    const PcgScalars* S)
 {
   if (S->done) return;
+  const bool update = S->iter > 0;  // iteration 1 starts from r = b - A x
+  const double alpha = S->alpha;
   constexpr int NV = FLEX ? 3 : 2;
   __shared__ double s4[NV * 4];
   double v[NV];
@@ -92,24 +102,41 @@ __global__ __launch_bounds__(256) void k_pcg_precon_dots(const int32_t N, const 
   for (int u = 0; u < kRedPerLane; ++u) {
     const int64_t id = b0 + u * kRedLanes;
     if (id < N) {
-      const double wid = w[id];
+      const double wid = weight<DEG>(w, deg, id);
       for (int f = 0; f < Nf; ++f) {
         const int64_t iid = id + (int64_t)f * offset;
-        const double rf = r[iid];
+        double rf = ldnt(&r[iid]);
+        double apf = 0.0;
+        if (update || FLEX) apf = ldnt(&Ap[iid]);
+        if (update) {
+          stnt(__builtin_fma(alpha, ldnt(&p[iid]), ldnt(&x[iid])), &x[iid]);  // BPUpdatePCG, utils.okl:795-807
+          rf = __builtin_fma(-alpha, apf, rf);
+          stnt(rf, &r[iid]);
+        }
         double zf = rf;
         if (JAC) {
-          zf = invDiag[id] * rf;
-          z[iid] = zf;
+          zf = ldnt(&invDiag[id]) * rf;
+          stnt(zf, &z[iid]);
         }
         v[0] = __builtin_fma(wid * rf, zf, v[0]);
         v[1] = __builtin_fma(wid * rf, rf, v[1]);
-        if (FLEX) v[NV - 1] = __builtin_fma(wid * zf, Ap[iid], v[NV - 1]);
+        if (FLEX) v[NV - 1] = __builtin_fma(wid * zf, apf, v[NV - 1]);
       }
     }
   }
   block_sum256<NV>(v, s4);
-  if (last_block_combine<NV>(v, partials, ticket, s4) && threadIdx.x == 0)
-    pcg_after_dots(S, v[0], v[1], FLEX ? v[NV - 1] : 0.0, hist_rdotr);
+  store_partials<NV>(v, partials);
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_pcg_finish_dots(const unsigned int nb, const double* __restrict__ partials,
+                                                         PcgScalars* S, double* __restrict__ hist_rdotr)
+{
+  if (S->done) return;
+  __shared__ double s4[NV * 4];
+  double v[NV];
+  combine_partials<NV>(v, partials, nb, s4);
+  if (threadIdx.x == 0) pcg_after_dots(S, v[0], v[1], NV > 2 ? v[NV - 1] : 0.0, hist_rdotr);
 }
 
 // 2. p = z + beta p
@@ -122,12 +149,13 @@ __global__ __launch_bounds__(256) void k_pcg_pupdate(const int64_t Nall, const d
   if (n < Nall) p[n] = __builtin_fma(beta, p[n], z[n]);
 }
 
-// 5. pAp = <p,Ap>_w, alpha
+// 5. pAp = <p,Ap>_w (tile partials), then alpha = rdotz/pAp
+template <bool DEG>
 __global__ __launch_bounds__(256) void k_pcg_pap(const int32_t N, const int32_t offset, const int Nf,
-                                                 const double* __restrict__ w, const double* __restrict__ p,
+                                                 const double* __restrict__ w, const uint8_t* __restrict__ deg,
+                                                 const double* __restrict__ p,
                                                  const double* __restrict__ Ap, double* __restrict__ partials,
-                                                 unsigned int* __restrict__ ticket, PcgScalars* S,
-                                                 double* __restrict__ hist_alpha)
+                                                 const PcgScalars* S)
 {
   if (S->done) return;
   __shared__ double s4[4];
@@ -137,15 +165,25 @@ __global__ __launch_bounds__(256) void k_pcg_pap(const int32_t N, const int32_t 
   for (int u = 0; u < kRedPerLane; ++u) {
     const int64_t id = b0 + u * kRedLanes;
     if (id < N) {
-      const double wid = w[id];
+      const double wid = weight<DEG>(w, deg, id);
       for (int f = 0; f < Nf; ++f) {
         const int64_t iid = id + (int64_t)f * offset;
-        v[0] = __builtin_fma(wid * p[iid], Ap[iid], v[0]);
+        v[0] = __builtin_fma(wid * ldnt(&p[iid]), ldnt(&Ap[iid]), v[0]);
       }
     }
   }
   block_sum256<1>(v, s4);
-  if (last_block_combine<1>(v, partials, ticket, s4) && threadIdx.x == 0) {
+  store_partials<1>(v, partials);
+}
+
+__global__ __launch_bounds__(256) void k_pcg_finish_pap(const unsigned int nb, const double* __restrict__ partials,
+                                                        PcgScalars* S, double* __restrict__ hist_alpha)
+{
+  if (S->done) return;
+  __shared__ double s4[4];
+  double v[1];
+  combine_partials<1>(v, partials, nb, s4);
+  if (threadIdx.x == 0) {
     S->pAp = v[0];
     const double alpha = S->rdotz1 / v[0];  // solve.cpp:104
     S->alpha = alpha;
@@ -158,32 +196,11 @@ __global__ __launch_bounds__(256) void k_pcg_pap(const int32_t N, const int32_t 
   }
 }
 
-// 6. x += alpha p; r -= alpha Ap  (BPUpdatePCG, nekBone/kernel/utils.okl:795-807)
-__global__ __launch_bounds__(256) void k_pcg_update(const int64_t Nall, const double* __restrict__ p,
-                                                    const double* __restrict__ Ap, double* __restrict__ x,
-                                                    double* __restrict__ r, const PcgScalars* S)
-{
-  if (S->done) return;
-  const double alpha = S->alpha;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < Nall) {
-    x[n] = __builtin_fma(alpha, p[n], x[n]);
-    r[n] = __builtin_fma(-alpha, Ap[n], r[n]);
-  }
-}
-
 // vecInv (nekBone/kernel/utils.okl:605-613)
 __global__ __launch_bounds__(256) void k_vec_inv(const int64_t N, double* __restrict__ v)
 {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n < N) v[n] = 1.0 / v[n];
-}
-
-// rhs (host-generated) masked and scaled per field is uploaded, then assembled
-__global__ __launch_bounds__(256) void k_scale_by(const int64_t N, double* __restrict__ v, const double* __restrict__ s)
-{
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < N) v[n] *= s[n];
 }
 
 inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -201,6 +218,7 @@ struct nbx_bp {
   nbx_ogs* ogs = nullptr;
   double *D = nullptr, *ggeo = nullptr, *lambda = nullptr, *invDiagA = nullptr, *rhs = nullptr;
   double *p = nullptr, *z = nullptr, *Ap = nullptr, *Ax = nullptr;
+  double* elem_partials = nullptr;  // per-element shares of pAp (fused schedule)
   int32_t* maskIds = nullptr;
   ReduceWork red;
   PcgScalars* S = nullptr;
@@ -273,7 +291,8 @@ static int bp_setup_impl(nbx_bp* bp)
   if ((rc = bp->alloc(&bp->D, D.size())) || (rc = bp->alloc(&bp->ggeo, (int64_t)ggeo.size())) ||
       (rc = bp->alloc(&bp->lambda, 2 * (int64_t)bp->fieldOffset)) || (rc = bp->alloc(&bp->invDiagA, bp->Nlocal)) ||
       (rc = bp->alloc(&bp->rhs, bp->Nall)) || (rc = bp->alloc(&bp->p, bp->Nall)) ||
-      (rc = bp->alloc(&bp->z, bp->Nall)) || (rc = bp->alloc(&bp->Ap, bp->Nall)) || (rc = bp->alloc(&bp->Ax, bp->Nall)))
+      (rc = bp->alloc(&bp->z, bp->Nall)) || (rc = bp->alloc(&bp->Ap, bp->Nall)) || (rc = bp->alloc(&bp->Ax, bp->Nall)) ||
+      (rc = bp->alloc(&bp->elem_partials, E)))
     return rc;
   NBX_HIP(hipMemcpy(bp->D, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
   NBX_HIP(hipMemcpy(bp->ggeo, ggeo.data(), sizeof(double) * ggeo.size(), hipMemcpyHostToDevice));
@@ -384,13 +403,7 @@ static int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* A
   int rc = axhelm_launch(bp->N, bp->Nf, o.bpmode ? 1 : 0, NBX_F64, bp->E, bp->fieldOffset, nullptr, bp->ggeo,
                          bp->D, lambda, q, Aq, skip, bp->stream);
   if (rc) return rc;
-  rc = gs_launch(bp->ogs, Aq, bp->Nf, bp->fieldOffset, NBX_DOUBLE, NBX_ADD, skip, bp->stream);
-  if (rc) return rc;
-  for (int f = 0; f < bp->Nf && bp->Nmasked; ++f) {
-    rc = mask_launch(bp->Nmasked, bp->maskIds, Aq + (int64_t)f * bp->fieldOffset, skip, bp->stream);
-    if (rc) return rc;
-  }
-  return NBX_SUCCESS;
+  return gs_add_mask_launch(bp->ogs, Aq, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, skip, bp->stream);
 }
 
 extern "C" int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double* q, double* Aq, double* pAp_host)
@@ -431,26 +444,53 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
   const bool flex = o.flexible != 0;
   const unsigned int nb = (unsigned int)((bp->Nlocal + kRedTile - 1) / kRedTile);
   const double* w = nbx_ogs_inv_degree(bp->ogs);
+  const uint8_t* deg = ogs_degree(bp->ogs);
   const double* zsrc = jac ? bp->z : r;
-#define NBX_PD(J, F)                                                                                          \
-  hipLaunchKernelGGL((k_pcg_precon_dots<J, F>), dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, \
-                     bp->Nf, w, bp->invDiagA, r, bp->z, bp->Ap, bp->red.partials, bp->red.ticket, bp->S,          \
-                     bp->hist_rdotr)
-  if (jac && flex) NBX_PD(true, true);
-  else if (jac) NBX_PD(true, false);
-  else if (flex) NBX_PD(false, true);
-  else NBX_PD(false, false);
+#define NBX_PD(J, F, DG)                                                                                         \
+  hipLaunchKernelGGL((k_pcg_update_precon_dots<J, F, DG>), dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal,        \
+                     bp->fieldOffset, bp->Nf, w, deg, bp->invDiagA, bp->p, bp->Ap, x, r, bp->z, bp->red.partials,  \
+                     bp->S)
+#define NBX_PD2(J, F) \
+  if (deg) NBX_PD(J, F, true); else NBX_PD(J, F, false);
+  if (jac && flex) { NBX_PD2(true, true) }
+  else if (jac) { NBX_PD2(true, false) }
+  else if (flex) { NBX_PD2(false, true) }
+  else { NBX_PD2(false, false) }
+#undef NBX_PD2
 #undef NBX_PD
+  if (flex)
+    hipLaunchKernelGGL(k_pcg_finish_dots<3>, dim3(1), dim3(256), 0, bp->stream, nb, bp->red.partials, bp->S,
+                       bp->hist_rdotr);
+  else
+    hipLaunchKernelGGL(k_pcg_finish_dots<2>, dim3(1), dim3(256), 0, bp->stream, nb, bp->red.partials, bp->S,
+                       bp->hist_rdotr);
   NBX_HIP_LAUNCH();
+  if (o.fused) {
+    // p = z + beta p, Ap = A p and pAp element shares in one pass; then gs + mask
+    int rc = axhelm_pcg_launch(bp->N, bp->Nf, o.bpmode ? 1 : 0, bp->E, bp->fieldOffset, bp->ggeo, bp->D, lambda,
+                               zsrc, bp->p, bp->Ap, bp->elem_partials, reinterpret_cast<const PcgFlags*>(bp->S),
+                               bp->stream);
+    if (rc) return rc;
+    rc = gs_add_mask_launch(bp->ogs, bp->Ap, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, &bp->S->done,
+                            bp->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pcg_finish_pap, dim3(1), dim3(256), 0, bp->stream, (unsigned)bp->E, bp->elem_partials, bp->S,
+                       bp->hist_alpha);
+    NBX_HIP_LAUNCH();
+    return NBX_SUCCESS;
+  }
   hipLaunchKernelGGL(k_pcg_pupdate, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, zsrc, bp->p, bp->S);
   NBX_HIP_LAUNCH();
   int rc = bp_apply(bp, lambda, bp->p, bp->Ap, &bp->S->done);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_pcg_pap, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, bp->Nf, w, bp->p,
-                     bp->Ap, bp->red.partials, bp->red.ticket, bp->S, bp->hist_alpha);
-  NBX_HIP_LAUNCH();
-  hipLaunchKernelGGL(k_pcg_update, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, bp->p, bp->Ap, x, r,
-                     bp->S);
+  if (deg)
+    hipLaunchKernelGGL(k_pcg_pap<true>, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, bp->Nf, w,
+                       deg, bp->p, bp->Ap, bp->red.partials, bp->S);
+  else
+    hipLaunchKernelGGL(k_pcg_pap<false>, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, bp->Nf, w,
+                       deg, bp->p, bp->Ap, bp->red.partials, bp->S);
+  hipLaunchKernelGGL(k_pcg_finish_pap, dim3(1), dim3(256), 0, bp->stream, nb, bp->red.partials, bp->S,
+                     bp->hist_alpha);
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
 }

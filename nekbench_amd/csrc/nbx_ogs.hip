@@ -30,9 +30,13 @@ struct nbx_ogs {
   int32_t Nshared = 0;
   std::vector<int32_t> h_offsets;  // full CSR (all groups)
   std::vector<int32_t> h_ids;
-  int32_t* d_offsets = nullptr;    // shared groups only
+  int32_t* d_offsets = nullptr;    // shared groups of other sizes (CSR)
   int32_t* d_ids = nullptr;
+  int32_t Ngen = 0;                // groups in the CSR above
+  int32_t Nfix[3] = {0, 0, 0};     // groups of exactly 2, 4, 8 members
+  int32_t* d_fix[3] = {nullptr, nullptr, nullptr};  // their members, DEG per group
   double* d_invDegree = nullptr;
+  uint8_t* d_degree = nullptr;  // group size per node (1 if excluded); null if any group > 255
 };
 
 namespace nbx {
@@ -64,6 +68,16 @@ struct GsOp<T, NBX_MAX> {
   __device__ static T comb(T acc, T b) { return (b > acc) ? b : acc; }
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumbering so that XCD x runs the x-th contiguous eighth of the
+// groups keeps neighbouring groups (which share cache lines of the same
+// element faces) in one L2.  The grid is a multiple of 8 blocks.
+__device__ __forceinline__ int32_t xcd_block(int32_t nblocks8)
+{
+  const int32_t per = nblocks8 >> 3;
+  return (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+}
+
 template <typename T, int OP>
 __global__ __launch_bounds__(256) void k_gs_many(const int32_t Ngroups, const int k, const int32_t stride,
                                                  const int32_t* __restrict__ offsets,
@@ -71,7 +85,7 @@ __global__ __launch_bounds__(256) void k_gs_many(const int32_t Ngroups, const in
                                                  const int* __restrict__ skip)
 {
   using O = GsOp<T, OP>;
-  const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t g = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= Ngroups || (skip && *skip)) return;
   const int32_t s = offsets[g], e = offsets[g + 1];
   for (int f = 0; f < k; ++f) {
@@ -79,6 +93,102 @@ __global__ __launch_bounds__(256) void k_gs_many(const int32_t Ngroups, const in
     T acc = O::first ? qf[ids[s]] : O::init();
     for (int32_t n = s + (O::first ? 1 : 0); n < e; ++n) acc = O::comb(acc, qf[ids[n]]);
     for (int32_t n = s; n < e; ++n) qf[ids[n]] = acc;
+  }
+}
+
+// groups of exactly DEG members, members stored contiguously (vector loads)
+template <typename T, int OP, int DEG>
+__global__ __launch_bounds__(256) void k_gs_fixed(const int32_t Ngroups, const int k, const int32_t stride,
+                                                  const int32_t* __restrict__ ids, T* __restrict__ q,
+                                                  const int* __restrict__ skip)
+{
+  using O = GsOp<T, OP>;
+  const int32_t g = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (g >= Ngroups || (skip && *skip)) return;
+  int32_t m[DEG];
+  if (DEG == 2) {
+    const int2 v = *reinterpret_cast<const int2*>(ids + (int64_t)g * 2);
+    m[0] = v.x;
+    m[1] = v.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < DEG / 4; ++c) {
+      const int4 v = *reinterpret_cast<const int4*>(ids + (int64_t)g * DEG + 4 * c);
+      m[4 * c + 0] = v.x;
+      m[4 * c + 1] = v.y;
+      m[4 * c + 2] = v.z;
+      m[4 * c + 3] = v.w;
+    }
+  }
+  for (int f = 0; f < k; ++f) {
+    T* qf = q + (int64_t)f * stride;
+    T acc = O::first ? qf[m[0]] : O::init();
+#pragma unroll
+    for (int n = O::first ? 1 : 0; n < DEG; ++n) acc = O::comb(acc, qf[m[n]]);
+#pragma unroll
+    for (int n = 0; n < DEG; ++n) qf[m[n]] = acc;
+  }
+}
+
+// All classes of a double-add gather-scatter and the Dirichlet mask in one
+// launch.  Every class list (pairs, quads, octets, CSR groups, masked nodes)
+// is cut into 8 equal chunks, chunk x of every class runs on XCD x (blocks
+// b with b % 8 == x), so one L2 sees the same element range for every class.
+// Masked nodes belong to no group: mask writes never touch a gathered value.
+struct GsClasses {
+  int32_t n[5];          // pairs, quads, octets, CSR groups, masked nodes
+  int32_t per[5];        // items per XCD chunk
+  int32_t bstart[6];     // cumulative blocks per XCD
+  const int32_t* ids[5]; // members (pairs/quads/octets), CSR ids, mask ids
+  const int32_t* off;    // CSR offsets
+};
+
+__global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const int k, const int32_t stride,
+                                                     double* __restrict__ q, const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int32_t xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  int c = 0;
+  while (c < 5 && local >= C.bstart[c + 1]) ++c;
+  if (c == 5) return;
+  const int32_t item = xcd * C.per[c] + (local - C.bstart[c]) * 256 + (int32_t)threadIdx.x;
+  if (item >= min(C.n[c], (xcd + 1) * C.per[c])) return;
+  if (c == 4) {
+    const int32_t id = C.ids[4][item];
+    for (int f = 0; f < k; ++f) q[id + (int64_t)f * stride] = 0.0;
+    return;
+  }
+  if (c == 3) {
+    const int32_t s0 = C.off[item], e0 = C.off[item + 1];
+    const int32_t* idg = C.ids[3];
+    for (int f = 0; f < k; ++f) {
+      double* qf = q + (int64_t)f * stride;
+      double acc = 0.0;
+      for (int32_t n = s0; n < e0; ++n) acc = acc + qf[idg[n]];
+      for (int32_t n = s0; n < e0; ++n) qf[idg[n]] = acc;
+    }
+    return;
+  }
+  int32_t m[8];
+  const int deg = 2 << c;
+  if (c == 0) {
+    const int2 v = *reinterpret_cast<const int2*>(C.ids[0] + (int64_t)item * 2);
+    m[0] = v.x;
+    m[1] = v.y;
+  } else {
+    for (int h = 0; h < deg / 4; ++h) {
+      const int4 v = *reinterpret_cast<const int4*>(C.ids[c] + (int64_t)item * deg + 4 * h);
+      m[4 * h + 0] = v.x;
+      m[4 * h + 1] = v.y;
+      m[4 * h + 2] = v.z;
+      m[4 * h + 3] = v.w;
+    }
+  }
+  for (int f = 0; f < k; ++f) {
+    double* qf = q + (int64_t)f * stride;
+    double acc = 0.0;
+    for (int n = 0; n < deg; ++n) acc = acc + qf[m[n]];
+    for (int n = 0; n < deg; ++n) qf[m[n]] = acc;
   }
 }
 
@@ -90,27 +200,35 @@ __global__ __launch_bounds__(256) void k_mask(const int32_t Nmasked, const int32
   if (n < Nmasked) q[maskIds[n]] = 0.0;
 }
 
+inline dim3 grid8(int32_t n) { return dim3((unsigned)(((n + 255) / 256 + 7) / 8 * 8)); }
+
+template <typename T, int OP>
+static void gs_launch_op(const nbx_ogs* o, T* v, int k, int32_t stride, const int* skip, hipStream_t s)
+{
+  if (o->Nfix[0])
+    hipLaunchKernelGGL((k_gs_fixed<T, OP, 2>), grid8(o->Nfix[0]), dim3(256), 0, s, o->Nfix[0], k, stride, o->d_fix[0],
+                       v, skip);
+  if (o->Nfix[1])
+    hipLaunchKernelGGL((k_gs_fixed<T, OP, 4>), grid8(o->Nfix[1]), dim3(256), 0, s, o->Nfix[1], k, stride, o->d_fix[1],
+                       v, skip);
+  if (o->Nfix[2])
+    hipLaunchKernelGGL((k_gs_fixed<T, OP, 8>), grid8(o->Nfix[2]), dim3(256), 0, s, o->Nfix[2], k, stride, o->d_fix[2],
+                       v, skip);
+  if (o->Ngen)
+    hipLaunchKernelGGL((k_gs_many<T, OP>), grid8(o->Ngen), dim3(256), 0, s, o->Ngen, k, stride, o->d_offsets, o->d_ids,
+                       v, skip);
+}
+
 template <typename T>
 static int gs_launch_t(const nbx_ogs* o, void* v, int k, int32_t stride, nbx_op op, const int* skip,
                        hipStream_t s)
 {
-  const int32_t G = o->NsharedGather;
-  const dim3 grid((G + 255) / 256), block(256);
   switch (op) {
-    case NBX_ADD:
-      hipLaunchKernelGGL((k_gs_many<T, NBX_ADD>), grid, block, 0, s, G, k, stride, o->d_offsets, o->d_ids, (T*)v, skip);
-      break;
-    case NBX_MUL:
-      hipLaunchKernelGGL((k_gs_many<T, NBX_MUL>), grid, block, 0, s, G, k, stride, o->d_offsets, o->d_ids, (T*)v, skip);
-      break;
-    case NBX_MIN:
-      hipLaunchKernelGGL((k_gs_many<T, NBX_MIN>), grid, block, 0, s, G, k, stride, o->d_offsets, o->d_ids, (T*)v, skip);
-      break;
-    case NBX_MAX:
-      hipLaunchKernelGGL((k_gs_many<T, NBX_MAX>), grid, block, 0, s, G, k, stride, o->d_offsets, o->d_ids, (T*)v, skip);
-      break;
-    default:
-      return fail(NBX_ERR_INVALID, "gs: unknown op %d", (int)op);
+    case NBX_ADD: gs_launch_op<T, NBX_ADD>(o, (T*)v, k, stride, skip, s); break;
+    case NBX_MUL: gs_launch_op<T, NBX_MUL>(o, (T*)v, k, stride, skip, s); break;
+    case NBX_MIN: gs_launch_op<T, NBX_MIN>(o, (T*)v, k, stride, skip, s); break;
+    case NBX_MAX: gs_launch_op<T, NBX_MAX>(o, (T*)v, k, stride, skip, s); break;
+    default: return fail(NBX_ERR_INVALID, "gs: unknown op %d", (int)op);
   }
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
@@ -129,6 +247,28 @@ int gs_launch(const nbx_ogs* o, void* v, int k, int32_t stride, nbx_type type, n
   }
 }
 
+int gs_add_mask_launch(const nbx_ogs* o, double* v, int k, int32_t stride, int32_t Nmasked,
+                       const int32_t* maskIds, const int* skip, hipStream_t s)
+{
+  GsClasses C{};
+  const int32_t n[5] = {o->Nfix[0], o->Nfix[1], o->Nfix[2], o->Ngen, Nmasked};
+  const int32_t* ids[5] = {o->d_fix[0], o->d_fix[1], o->d_fix[2], o->d_ids, maskIds};
+  C.bstart[0] = 0;
+  for (int c = 0; c < 5; ++c) {
+    C.n[c] = n[c];
+    C.per[c] = (n[c] + 7) / 8;
+    C.ids[c] = ids[c];
+    C.bstart[c + 1] = C.bstart[c] + (C.per[c] + 255) / 256;
+  }
+  C.off = o->d_offsets;
+  if (C.bstart[5] == 0) return NBX_SUCCESS;
+  hipLaunchKernelGGL(k_gs_add_mask, dim3((unsigned)(8 * C.bstart[5])), dim3(256), 0, s, C, k, stride, v, skip);
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}
+
+const uint8_t* ogs_degree(const nbx_ogs* o) { return o ? o->d_degree : nullptr; }
+
 int mask_launch(int32_t Nmasked, const int32_t* maskIds, double* q, const int* skip, hipStream_t s)
 {
   if (Nmasked <= 0) return NBX_SUCCESS;
@@ -146,7 +286,10 @@ static void ogs_release(nbx_ogs* o)
   if (!o) return;
   if (o->d_offsets) (void)hipFree(o->d_offsets);
   if (o->d_ids) (void)hipFree(o->d_ids);
+  for (int c = 0; c < 3; ++c)
+    if (o->d_fix[c]) (void)hipFree(o->d_fix[c]);
   if (o->d_invDegree) (void)hipFree(o->d_invDegree);
+  if (o->d_degree) (void)hipFree(o->d_degree);
   delete o;
 }
 
@@ -200,22 +343,32 @@ extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose
   for (int32_t g = 0; g < G; ++g) o->h_offsets[g + 1] += o->h_offsets[g];
   o->h_ids.resize(Nlocal > 0 ? Nlocal : 1);
   std::vector<double> invDeg(N > 0 ? N : 1, 1.0);
-  std::vector<int32_t> sOff(1, 0), sIds;
+  std::vector<uint8_t> deg(N > 0 ? N : 1, 1);
+  bool degFits = true;
+  std::vector<int32_t> sOff(1, 0), sIds, fix[3];
   for (int32_t fin = 0; fin < G; ++fin) {
     const int32_t r = order[fin];
     const int32_t cnt = runStart[r + 1] - runStart[r];
     int32_t* dst = o->h_ids.data() + o->h_offsets[fin];
     for (int32_t m = 0; m < cnt; ++m) {
       dst[m] = nodes[runStart[r] + m].local;
-      invDeg[dst[m]] = 1.0 / (double)cnt;
+      invDeg[dst[m]] = 1.0 / (double)cnt;  // ogsSetup.cpp:342-343
+      deg[dst[m]] = (uint8_t)std::min(cnt, 255);
     }
+    if (cnt > 255) degFits = false;
     if (cnt > 1) {
-      sIds.insert(sIds.end(), dst, dst + cnt);
-      sOff.push_back((int32_t)sIds.size());
+      o->NsharedGather++;
+      o->Nshared += cnt;
+      const int c = cnt == 2 ? 0 : cnt == 4 ? 1 : cnt == 8 ? 2 : -1;
+      if (c >= 0) {
+        fix[c].insert(fix[c].end(), dst, dst + cnt);
+      } else {
+        sIds.insert(sIds.end(), dst, dst + cnt);
+        sOff.push_back((int32_t)sIds.size());
+      }
     }
   }
-  o->NsharedGather = (int32_t)sOff.size() - 1;
-  o->Nshared = (int32_t)sIds.size();
+  o->Ngen = (int32_t)sOff.size() - 1;
   if (verbose)
     fprintf(stderr, "nbx_ogs_setup: N=%d Nlocal=%d groups=%d shared groups=%d shared nodes=%d\n", N, Nlocal, G,
             o->NsharedGather, o->Nshared);
@@ -233,6 +386,24 @@ extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose
     fail(NBX_ERR_HIP, "ogs_setup: upload failed");
     ogs_release(o);
     return nullptr;
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (fix[c].empty()) continue;
+    o->Nfix[c] = (int32_t)(fix[c].size() / (2 << c));
+    if (hipMalloc(&o->d_fix[c], sizeof(int32_t) * fix[c].size()) != hipSuccess ||
+        hipMemcpy(o->d_fix[c], fix[c].data(), sizeof(int32_t) * fix[c].size(), hipMemcpyHostToDevice) != hipSuccess) {
+      fail(NBX_ERR_HIP, "ogs_setup: class upload failed");
+      ogs_release(o);
+      return nullptr;
+    }
+  }
+  if (degFits && N > 0) {
+    if (hipMalloc(&o->d_degree, (size_t)N) != hipSuccess ||
+        hipMemcpy(o->d_degree, deg.data(), (size_t)N, hipMemcpyHostToDevice) != hipSuccess) {
+      fail(NBX_ERR_HIP, "ogs_setup: degree upload failed");
+      ogs_release(o);
+      return nullptr;
+    }
   }
   return o;
 }

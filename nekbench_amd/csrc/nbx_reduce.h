@@ -5,9 +5,11 @@
 //   accumulating with an fma chain in v order;
 //   256-lane tree: halving tree inside each wave64 (lane 0 ends with
 //   ((a0+a32)+(a16+a48))+...), then (S0 + S2) + (S1 + S3) over the waves;
-//   the partials of all tiles are combined by the last tile to finish
-//   (atomic ticket): lane t adds partials t, t+256, ... in order, then the
-//   same 256-lane tree.
+//   a second one-block kernel combines the tile partials: lane t adds
+//   partials t, t+256, ... in order, then the same 256-lane tree.
+// (A single-pass "last block finishes" variant needs a device-scope fence
+// per tile, which on this multi-XCD part writes back L2 each time; the
+// kernel boundary is the cheaper synchronisation.)
 // The CPU oracle restates this order, so GPU and CPU sums agree bit for bit.
 #pragma once
 #include "nbx_internal.h"
@@ -39,36 +41,39 @@ __device__ __forceinline__ void block_sum256(double (&v)[NV], double* s4)
   }
 }
 
-// Called by every block after it has its NV block sums in lane 0 of v.
-// Writes them to partials[c*nblocks + b]; the last block to arrive combines all
-// partials in fixed order and returns true with the totals in lane 0 of v.
-// ticket must be 0 on entry and is 0 again after the last block.
+// Second phase: one 256-lane block combines the nb partials of each of the
+// NV values (partials[c*nb + b]) in fixed order; totals end in lane 0 of v.
 template <int NV>
-__device__ __forceinline__ bool last_block_combine(double (&v)[NV], double* __restrict__ partials,
-                                                   unsigned int* __restrict__ ticket, double* s4)
+__device__ __forceinline__ void combine_partials(double (&v)[NV], const double* __restrict__ partials,
+                                                 unsigned int nb, double* s4)
 {
-  __shared__ bool s_last;
-  const unsigned int nb = gridDim.x;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int c = 0; c < NV; ++c) partials[(size_t)c * nb + blockIdx.x] = v[c];
-    __threadfence();
-    const unsigned int prev = atomicAdd(ticket, 1u);
-    s_last = (prev == nb - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
 #pragma unroll
   for (int c = 0; c < NV; ++c) {
+    // loads batched 8 deep (latency-bound single block); additions stay in i order
     double acc = 0.0;
-    for (unsigned int i = threadIdx.x; i < nb; i += 256)
-      acc = acc + __hip_atomic_load(&partials[(size_t)c * nb + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double* pc = partials + (size_t)c * nb;
+    unsigned int i = threadIdx.x;
+    for (; i + 7 * 256 < nb; i += 8 * 256) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = pc[i + u * 256];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = acc + t[u];
+    }
+    for (; i < nb; i += 256) acc = acc + pc[i];
     v[c] = acc;
   }
   block_sum256<NV>(v, s4);
-  if (threadIdx.x == 0) atomicExch(ticket, 0u);
-  return true;
+}
+
+// First phase epilogue: lane 0 of each block stores its NV block sums.
+template <int NV>
+__device__ __forceinline__ void store_partials(const double (&v)[NV], double* __restrict__ partials)
+{
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) partials[(size_t)c * gridDim.x + blockIdx.x] = v[c];
+  }
 }
 
 }  // namespace nbx

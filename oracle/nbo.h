@@ -125,8 +125,9 @@ double nbo_bp_wdot(const nbo_bp* bp, const double* x, const double* y);
 /* BPPCG (solve.cpp:44-113) in the library's order: returns iterations (iter-1).
  * jacobi: z = invDiagA r with invDiagA = 1/gs(local diag) (solve.cpp:146-165);
  * tol <= 0 runs maxit iterations (FIXED ITERATION COUNT).  hist_rdotr[it],
- * hist_alpha[it] (may be NULL, length >= maxit+2) record each iteration. */
-int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, double* r, double* x, double tol, int maxit,
+ * hist_alpha[it] (may be NULL, length >= maxit+2) record each iteration.
+ * fused: pAp from the local operator output (the library's fused schedule). */
+int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r, double* x, double tol, int maxit,
                double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st);
 
 #ifdef __cplusplus

@@ -251,11 +251,11 @@ class OracleBP:
         f.restype = C.c_double
         return f(C.byref(self.struct), p(x), p(y))
 
-    def pcg(self, r, x, tol, maxit, jacobi=True, flexible=False):
+    def pcg(self, r, x, tol, maxit, jacobi=True, flexible=False, fused=True):
         hr = np.zeros(maxit + 2)
         ha = np.zeros(maxit + 2)
         st = PCGStats()
-        lib().nbo_bp_pcg(C.byref(self.struct), int(jacobi), int(flexible), p(r), p(x), C.c_double(tol), maxit,
+        lib().nbo_bp_pcg(C.byref(self.struct), int(jacobi), int(flexible), int(fused), p(r), p(x), C.c_double(tol), maxit,
                          p(hr), p(ha), C.byref(st))
         return st, hr, ha
 

@@ -9,6 +9,10 @@
  * library's arithmetic order: one fma per element-wise update, dots in the
  * fixed tile tree, scalars evaluated as C expressions in the reference's form.
  *
+ * fused != 0 restates the library's fused schedule: pAp = sum_local p (A_L p)
+ * taken from the operator's local output before the gather-scatter (equal to
+ * the weighted dot in real arithmetic for continuous, masked p).
+ *
  * Pinning: the operator is pinned through nbo_axhelm / nbo_gs_many; the loop
  * by CG invariants on small problems (orthogonality of residuals, A-conjugacy
  * of directions, monotone A-norm error), by the dense direct solve of the
@@ -67,7 +71,39 @@ double nbo_bp_wdot(const nbo_bp* bp, const double* x, const double* y)
   return out[0];
 }
 
-int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, double* r, double* x, double tol, int maxit,
+/* Element shares of pAp = sum_local p (A_L p) in the library's fused order:
+ * lane (i,j) runs an fma chain over fields then k; the Nq*Nq lanes are summed
+ * by a halving tree over the next power of two (zero padded); the element
+ * shares are combined like tile partials. */
+static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
+{
+  const int Nq = bp->N + 1, Nq2 = Nq * Nq, Np = Nq2 * Nq;
+  int P2 = 1;
+  while (P2 < Nq2) P2 *= 2;
+  double* part = (double*)malloc(sizeof(double) * (bp->E > 0 ? bp->E : 1));
+#pragma omp parallel for schedule(static)
+  for (int32_t e = 0; e < bp->E; ++e) {
+    double lanes[256];
+    for (int t = 0; t < P2; ++t) lanes[t] = 0.0;
+    for (int t = 0; t < Nq2; ++t) {
+      double acc = 0.0;
+      for (int f = 0; f < bp->Nf; ++f)
+        for (int k = 0; k < Nq; ++k) {
+          const int64_t n = (int64_t)e * Np + (int64_t)k * Nq2 + t + (int64_t)f * bp->offset;
+          acc = fma(p[n], Ap[n], acc);
+        }
+      lanes[t] = acc;
+    }
+    for (int h = P2 / 2; h >= 1; h >>= 1)
+      for (int t = 0; t < h; ++t) lanes[t] = lanes[t] + lanes[t + h];
+    part[e] = lanes[0];
+  }
+  const double r = nbo_tree_sum_partials(bp->E, part);
+  free(part);
+  return r;
+}
+
+int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r, double* x, double tol, int maxit,
                double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
 {
   const int Np = (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
@@ -120,8 +156,17 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, double* r, double* x,
     /* p = z + beta p (:100) */
     for (int64_t n = 0; n < Nall; ++n) p[n] = fma(beta, p[n], zsrc[n]);
     /* Ap = A p; pAp = (p.Ap)_w; alpha = rdotz/pAp (:102-104) */
-    nbo_bp_apply(bp, p, Ap);
-    const double pAp = nbo_bp_wdot(bp, p, Ap);
+    double pAp;
+    if (fused) {  /* pAp from the local operator output, then gs + mask */
+      nbo_axhelm(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, p, Ap);
+      pAp = fused_pap(bp, p, Ap);
+      nbo_gs_many(bp->ogs, Ap, bp->Nf, bp->offset, NBO_DOUBLE, NBO_ADD);
+      for (int f = 0; f < bp->Nf; ++f)
+        for (int32_t m = 0; m < bp->Nmasked; ++m) Ap[bp->maskIds[m] + (int64_t)f * bp->offset] = 0.0;
+    } else {
+      nbo_bp_apply(bp, p, Ap);
+      pAp = nbo_bp_wdot(bp, p, Ap);
+    }
     alpha = rdotz1 / pAp;
     if (hist_alpha) hist_alpha[it] = alpha;
     if (!isfinite(alpha)) { breakdown = 1; iterations = it - 1; break; }

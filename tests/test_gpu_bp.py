@@ -26,13 +26,13 @@ CASES = [
 ]
 
 
-def make(nbx, nbo, case):
+def make(nbx, nbo, case, fused=True):
     N, dims, Nf, bpmode, l1, precon, flex, periodic, rhs_kind = case
     kw = dict(perturb=0.0 if periodic else 0.1, seed=31 + N, periodic=periodic)
     if periodic:
         kw["extent"] = (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
     bp = nbx.BP(nbx.Box.make(N, *dims, **kw), Nfields=Nf, bpmode=bpmode, lambda1=l1, dirichlet=not periodic,
-                precon=precon, flexible=flex, rhs_kind=rhs_kind)
+                precon=precon, flexible=flex, rhs_kind=rhs_kind, fused=fused)
     ob = nbo.OracleBP(nbo.Box.make(N, *dims, **kw), Nf=Nf, bpmode=bpmode, lambda1=l1, dirichlet=not periodic,
                       rhs_kind=rhs_kind)
     return bp, ob
@@ -57,10 +57,11 @@ def test_bp_setup_and_operator(nbx, nbo, dev, case):
     assert pAp == ob.wdot(ob.rhs.copy(), ref)
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "refsched"])
 @pytest.mark.parametrize("case", CASES, ids=[f"case{i}" for i in range(len(CASES))])
-def test_bp_pcg_iterates_match_oracle(nbx, nbo, dev, case):
+def test_bp_pcg_iterates_match_oracle(nbx, nbo, dev, case, fused):
     import torch
-    bp, ob = make(nbx, nbo, case)
+    bp, ob = make(nbx, nbo, case, fused)
     precon, flex = case[5], case[6]
     n = bp.Nfields * bp.fieldOffset
     maxit, tol = 60, 1e-10
@@ -69,7 +70,7 @@ def test_bp_pcg_iterates_match_oracle(nbx, nbo, dev, case):
     st = bp.pcg(r, x, tol, maxit)
     hr, ha = bp.history(maxit + 2)
     rr, xr = ob.rhs.copy(), np.zeros(n)
-    ost, ohr, oha = ob.pcg(rr, xr, tol, maxit, jacobi=bool(precon), flexible=flex)
+    ost, ohr, oha = ob.pcg(rr, xr, tol, maxit, jacobi=bool(precon), flexible=flex, fused=fused)
     assert st.iterations == ost.iterations and st.converged == ost.converged
     k = st.iterations + 1
     assert np.array_equal(hr[1:k + 1], ohr[1:k + 1]), "rdotr history differs"

@@ -95,3 +95,21 @@ def test_manufactured_solution(nbo):
         exact = np.cos(np.pi * m["x"]) * np.cos(np.pi * m["y"]) * np.cos(np.pi * m["z"])
         errs.append(np.abs(x - exact).max())
     assert errs[1] < errs[0] / 10 and errs[1] < 1e-3
+
+
+def test_fused_schedule_matches_reference_schedule(nbo):
+    """pAp from the local operator output equals the weighted dot after the
+    gather-scatter up to rounding: same iteration counts, iterates within
+    rounding growth."""
+    for kw in (dict(bpmode=True), dict(bpmode=False, lambda1=0.4)):
+        box = nbo.Box.make(4, 3, 3, 2, perturb=0.1, seed=10)
+        ob = nbo.OracleBP(box, **kw)
+        out = []
+        for fused in (False, True):
+            r, x = ob.rhs.copy(), np.zeros(ob.offset)
+            st, hr, ha = ob.pcg(r, x, 1e-10, 400, jacobi=True, fused=fused)
+            out.append((st.iterations, x, ha[1:st.iterations + 1]))
+        assert abs(out[0][0] - out[1][0]) <= 1
+        assert np.abs(out[0][1] - out[1][1]).max() < 1e-9 * np.abs(out[0][1]).max()
+        n = min(len(out[0][2]), len(out[1][2]), 10)
+        assert np.allclose(out[0][2][:n], out[1][2][:n], rtol=1e-12, atol=0)
