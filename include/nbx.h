@@ -182,6 +182,7 @@ typedef struct {
   double rdotr0;        /* weighted |r|^2 at iteration 1 */
   double rdotr;         /* weighted |r|^2 at exit */
   double elapsed_s;     /* wall time of the solve on the device */
+  int cycles;           /* refinement cycles (1 for nbx_bp_pcg) */
 } nbx_pcg_stats;
 
 nbx_bp* nbx_bp_setup(const nbx_bp_options* opt, void* stream);
@@ -198,6 +199,17 @@ int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double* q, double
  * stopping test rdotr <= max(tol^2 rdotr_1, tol^2) (solve.cpp:85-90). */
 int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
                nbx_pcg_stats* stats);
+/* Mixed-precision solve with iterative refinement: the true residual
+ * r = b - A x is formed with the FP64 operator; each cycle solves A_lo d = r
+ * by PCG from d = 0 to the relative tolerance inner_tol with the operator in
+ * op_prec (NBX_F64, NBX_F32, or NBX_BF16 = bf16 ggeo with FP32 arithmetic)
+ * while the vectors, dot products, gather-scatter sums, updates and the
+ * stopping test stay FP64; then x += d.  Stops when |r|_w <= tol |r_0|_w
+ * (true residual), after max_cycles cycles or maxit inner iterations in
+ * total.  Returns the total inner iterations; stats->rdotr is the final
+ * true residual. */
+int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
+                     nbx_prec op_prec, double inner_tol, int max_cycles, nbx_pcg_stats* stats);
 /* per-iteration history of the last solve (host outputs, length >= iterations+1):
  * rdotr and alpha per iteration; returns the count written */
 int nbx_bp_history(const nbx_bp* bp, double* rdotr, double* alpha, int capacity);

@@ -61,7 +61,7 @@ class BPOptions(C.Structure):
 
 class PCGStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("rdotr0", C.c_double),
-                ("rdotr", C.c_double), ("elapsed_s", C.c_double)]
+                ("rdotr", C.c_double), ("elapsed_s", C.c_double), ("cycles", C.c_int)]
 
 
 _P = C.c_void_p
@@ -95,6 +95,7 @@ _SIGS = {
     "nbx_bp_ax_operator": (_I, [_P, _P, _P, _P, _P]),
     "nbx_bp_pcg": (_I, [_P, _P, _P, _P, _D, _I, C.POINTER(PCGStats)]),
     "nbx_bp_history": (_I, [_P, _P, _P, _I]),
+    "nbx_bp_pcg_mixed": (_I, [_P, _P, _P, _P, _D, _I, _I, _D, _I, C.POINTER(PCGStats)]),
 }
 
 
@@ -300,6 +301,14 @@ class BP:
         rc = lib().nbx_bp_pcg(self.h, ptr(lam), ptr(r), ptr(x), tol, maxit, C.byref(st))
         if rc < 0:
             raise NbxError(f"nbx_bp_pcg failed ({rc}): {last_error()}")
+        return st
+
+    def pcg_mixed(self, r, x, tol, maxit, prec=NBX_F32, inner_tol=1e-5, max_cycles=20, lam=None) -> PCGStats:
+        st = PCGStats()
+        rc = lib().nbx_bp_pcg_mixed(self.h, ptr(lam), ptr(r), ptr(x), tol, maxit, prec, inner_tol, max_cycles,
+                                    C.byref(st))
+        if rc < 0:
+            raise NbxError(f"nbx_bp_pcg_mixed failed ({rc}): {last_error()}")
         return st
 
     def history(self, n):

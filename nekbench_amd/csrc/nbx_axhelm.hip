@@ -31,6 +31,20 @@ __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { r
 template <>
 __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Load of a geometric factor / lambda stored as G, converted to the math type T.
+// bf16 is stored as its 16 high bits (uint16_t).
+template <typename T, typename G>
+__device__ __forceinline__ T ldg(const G* p);
+template <>
+__device__ __forceinline__ double ldg<double, double>(const double* p) { return ldnt(p); }
+template <>
+__device__ __forceinline__ float ldg<float, float>(const float* p) { return ldnt(p); }
+template <>
+__device__ __forceinline__ float ldg<float, uint16_t>(const uint16_t* p)
+{
+  return __uint_as_float((uint32_t)ldnt(p) << 16);
+}
+
 // Per-workgroup state of the operator: LDS slabs and the lane's D vectors.
 template <typename T, int NQ>
 struct AxLane {
@@ -53,11 +67,11 @@ struct AxLane {
 
 // One field of one element: out[k] = (A q)(i,j,k) for the lane's column rq.
 // g, l0, l1 point at the lane's node (k = 0) of ggeo, lambda0, lambda1.
-template <typename T, int NQ, bool BK>
+template <typename T, int NQ, bool BK, typename G = T>
 __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&rq)[NQ], T (&out)[NQ],
                                               const T* __restrict__ s_D, T* s_q, T* s_Gr, T* s_Gs,
-                                              const T* __restrict__ g, const T* __restrict__ l0,
-                                              const T* __restrict__ l1)
+                                              const G* __restrict__ g, const G* __restrict__ l0,
+                                              const G* __restrict__ l1)
 {
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
@@ -77,13 +91,14 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
       qt = fmaT(s_D[k * NQ + m], rq[m], qt);
     }
     const int n = k * NQ2;
-    const T G00 = ldnt(&g[kG00 * NP + n]), G01 = ldnt(&g[kG01 * NP + n]), G02 = ldnt(&g[kG02 * NP + n]);
-    const T G11 = ldnt(&g[kG11 * NP + n]), G12 = ldnt(&g[kG12 * NP + n]), G22 = ldnt(&g[kG22 * NP + n]);
+    const T G00 = ldg<T, G>(&g[kG00 * NP + n]), G01 = ldg<T, G>(&g[kG01 * NP + n]);
+    const T G02 = ldg<T, G>(&g[kG02 * NP + n]), G11 = ldg<T, G>(&g[kG11 * NP + n]);
+    const T G12 = ldg<T, G>(&g[kG12 * NP + n]), G22 = ldg<T, G>(&g[kG22 * NP + n]);
     T Gr = fmaT(G02, qt, fmaT(G01, qs, G00 * qr));
     T Gs = fmaT(G12, qt, fmaT(G11, qs, G01 * qr));
     T Gt = fmaT(G22, qt, fmaT(G12, qs, G02 * qr));
     if (!BK) {
-      const T lam0 = ldnt(&l0[n]);
+      const T lam0 = ldg<T, G>(&l0[n]);
       Gr = lam0 * Gr;
       Gs = lam0 * Gs;
       Gt = lam0 * Gt;
@@ -106,7 +121,7 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
     T res = rArs[k] + rAt[k];
     if (!BK) {
       const int n = k * NQ2;
-      res = fmaT(ldnt(&g[kGWJ * NP + n]) * ldnt(&l1[n]), rq[k], res);
+      res = fmaT(ldg<T, G>(&g[kGWJ * NP + n]) * ldg<T, G>(&l1[n]), rq[k], res);
     }
     out[k] = res;
   }
@@ -151,10 +166,10 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm(const int32_t Nelements, cons
 // Lane (i,j) accumulates fma(p, Ap) over fields then k; the NQ*NQ lanes are
 // summed by a halving tree over the next power of two (zero padded); the
 // element partials go to partials[e].
-template <int NQ, int NF, bool BK>
+template <int NQ, int NF, bool BK, typename T, typename G>
 __global__ __launch_bounds__(NQ* NQ) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
-                                                       const double* __restrict__ ggeo, const double* __restrict__ D,
-                                                       const double* __restrict__ lambda,
+                                                       const G* __restrict__ ggeo, const T* __restrict__ D,
+                                                       const G* __restrict__ lambda,
                                                        const double* __restrict__ z, double* __restrict__ p,
                                                        double* __restrict__ Ap, double* __restrict__ partials,
                                                        const PcgFlags* __restrict__ S)
@@ -164,30 +179,34 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm_pcg(const int32_t Nelements, 
   constexpr int P2 = NQ2 <= 1 ? 1 : (NQ2 <= 2 ? 2 : (NQ2 <= 4 ? 4 : (NQ2 <= 8 ? 8 : (NQ2 <= 16 ? 16 : (NQ2 <= 32 ? 32 : (NQ2 <= 64 ? 64 : (NQ2 <= 128 ? 128 : 256)))))));
   if (S->done) return;
   const double beta = S->beta;
-  __shared__ double s_D[NQ2], s_q[NQ2], s_Gr[NQ2], s_Gs[NQ2], s_red[P2];
+  __shared__ T s_D[NQ2], s_q[NQ2], s_Gr[NQ2], s_Gs[NQ2];
+  __shared__ double s_red[P2];
   const int64_t e = blockIdx.x;
   s_D[threadIdx.x] = D[threadIdx.x];
   __syncthreads();
-  AxLane<double, NQ> L;
+  AxLane<T, NQ> L;
   L.init(s_D);
-  const double* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
-  const double* l0 = lambda + e * NP + L.t;
-  const double* l1 = lambda + e * NP + offset + L.t;
+  const G* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
+  const G* l0 = lambda + e * NP + L.t;
+  const G* l1 = lambda + e * NP + offset + L.t;
   double acc = 0.0;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const int64_t base = e * NP + (int64_t)f * offset + L.t;
-    double rq[NQ], out[NQ];
+    double pk[NQ];
+    T rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      rq[k] = __builtin_fma(beta, p[base + k * NQ2], ldnt(&z[base + k * NQ2]));
-      p[base + k * NQ2] = rq[k];
+      pk[k] = __builtin_fma(beta, p[base + k * NQ2], ldnt(&z[base + k * NQ2]));
+      p[base + k * NQ2] = pk[k];
+      rq[k] = (T)pk[k];
     }
-    axhelm_column<double, NQ, BK>(L, rq, out, s_D, s_q, s_Gr, s_Gs, g, l0, l1);
+    axhelm_column<T, NQ, BK, G>(L, rq, out, s_D, s_q, s_Gr, s_Gs, g, l0, l1);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      stnt(out[k], &Ap[base + k * NQ2]);
-      acc = __builtin_fma(rq[k], out[k], acc);
+      const double o = (double)out[k];
+      stnt(o, &Ap[base + k * NQ2]);
+      acc = __builtin_fma(pk[k], o, acc);
     }
   }
   double sum;
@@ -326,24 +345,29 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
   }
 }
 
-int axhelm_pcg_launch(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const double* ggeo,
-                      const double* D, const double* lambda, const double* z, double* p, double* Ap,
-                      double* partials, const PcgFlags* S, hipStream_t s)
+template <typename T, typename G>
+static int axhelm_pcg_launch_t(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const void* ggeo,
+                               const void* D, const void* lambda, const double* z, double* p, double* Ap,
+                               double* partials, const PcgFlags* S, hipStream_t s)
 {
-#define NBX_AXP(NV)                                                                                           \
-  case NV:                                                                                                    \
-    if (Nf == 1 && bk)                                                                                        \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, true>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,     \
-                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
-    else if (Nf == 1)                                                                                         \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, false>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,    \
-                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
-    else if (bk)                                                                                              \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,     \
-                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
-    else                                                                                                      \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, s,    \
-                         Nelements, offset, ggeo, D, lambda, z, p, Ap, partials, S);                          \
+#define NBX_AXP(NV)                                                                                          \
+  case NV:                                                                                                   \
+    if (Nf == 1 && bk)                                                                                       \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, true, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, \
+                         s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,       \
+                         partials, S);                                                                       \
+    else if (Nf == 1)                                                                                        \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, false, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)),   \
+                         0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
+                         partials, S);                                                                       \
+    else if (bk)                                                                                             \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, \
+                         s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,       \
+                         partials, S);                                                                       \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)),   \
+                         0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
+                         partials, S);                                                                       \
     break;
   if (Nf != 1 && Nf != 3) return fail(NBX_ERR_UNSUPPORTED, "axhelm_pcg: Nfields=%d", Nf);
   switch (N) {
@@ -368,6 +392,25 @@ int axhelm_pcg_launch(int N, int Nf, int bk, int32_t Nelements, int32_t offset, 
 #undef NBX_AXP
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
+}
+
+int axhelm_pcg_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32_t offset, const void* ggeo,
+                      const void* D, const void* lambda, const double* z, double* p, double* Ap,
+                      double* partials, const PcgFlags* S, hipStream_t s)
+{
+  switch (prec) {
+    case NBX_F64:
+      return axhelm_pcg_launch_t<double, double>(N, Nf, bk, Nelements, offset, ggeo, D, lambda, z, p, Ap, partials,
+                                                 S, s);
+    case NBX_F32:
+      return axhelm_pcg_launch_t<float, float>(N, Nf, bk, Nelements, offset, ggeo, D, lambda, z, p, Ap, partials,
+                                               S, s);
+    case NBX_BF16:
+      return axhelm_pcg_launch_t<float, uint16_t>(N, Nf, bk, Nelements, offset, ggeo, D, lambda, z, p, Ap,
+                                                  partials, S, s);
+    default:
+      return fail(NBX_ERR_UNSUPPORTED, "axhelm_pcg: precision %d", (int)prec);
+  }
 }
 
 }  // namespace nbx

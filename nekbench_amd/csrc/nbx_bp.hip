@@ -36,7 +36,7 @@ namespace nbx {
 struct PcgScalars {
   int done, pad_;
   double beta;
-  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, pAp, zdotAp;
+  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, pAp, zdotAp, abs_floor;
   int iter, iterations, converged, breakdown, maxit, fixed, flexible;
 };
 static_assert(offsetof(PcgScalars, done) == offsetof(PcgFlags, done) &&
@@ -60,7 +60,7 @@ __device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, doub
   hist_rdotr[it] = rdotr;
   if (it == 1) {
     S->rdotr0 = rdotr;
-    S->TOL = fmax(S->tol2 * rdotr, S->tol2);  // solve.cpp:85
+    S->TOL = fmax(S->tol2 * rdotr, S->abs_floor);  // solve.cpp:85 (abs_floor = tol^2 there)
   }
   if (!isfinite(rdotr)) {
     S->done = 1;
@@ -203,6 +203,32 @@ __global__ __launch_bounds__(256) void k_vec_inv(const int64_t N, double* __rest
   if (n < N) v[n] = 1.0 / v[n];
 }
 
+// operand copies for the low-precision operators: float, and bf16 as the
+// round-to-nearest-even high half of the float (double -> float -> bf16)
+__global__ __launch_bounds__(256) void k_cvt_f32(const int64_t n, const double* __restrict__ a, float* __restrict__ b)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(256) void k_cvt_bf16(const int64_t n, const double* __restrict__ a,
+                                                  uint16_t* __restrict__ b)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t u = __float_as_uint((float)a[i]);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);  // finite inputs only (ggeo, lambda)
+    b[i] = (uint16_t)(r >> 16);
+  }
+}
+
+// x += d
+__global__ __launch_bounds__(256) void k_axpy1(const int64_t n, const double* __restrict__ d, double* __restrict__ x)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + d[i];
+}
+
 inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace nbx
@@ -219,6 +245,14 @@ struct nbx_bp {
   double *D = nullptr, *ggeo = nullptr, *lambda = nullptr, *invDiagA = nullptr, *rhs = nullptr;
   double *p = nullptr, *z = nullptr, *Ap = nullptr, *Ax = nullptr;
   double* elem_partials = nullptr;  // per-element shares of pAp (fused schedule)
+  // low-precision operator operands (built on first use)
+  float* ggeo32 = nullptr;
+  float* D32 = nullptr;
+  float* lambda32 = nullptr;
+  uint16_t* ggeo16 = nullptr;
+  uint16_t* lambda16 = nullptr;
+  double *b = nullptr, *d = nullptr;   // refinement: right-hand side copy, correction
+  nbx_prec op_prec = NBX_F64;         // operator precision of the PCG being run
   int32_t* maskIds = nullptr;
   ReduceWork red;
   PcgScalars* S = nullptr;
@@ -243,6 +277,8 @@ static void bp_release(nbx_bp* bp)
   if (!bp) return;
   (void)hipDeviceSynchronize();
   for (void* p : bp->owned) (void)hipFree(p);
+  for (void* p : {(void*)bp->ggeo32, (void*)bp->D32, (void*)bp->lambda32, (void*)bp->ggeo16, (void*)bp->lambda16})
+    if (p) (void)hipFree(p);
   if (bp->maskIds) (void)hipFree(bp->maskIds);
   if (bp->S) (void)hipFree(bp->S);
   if (bp->hS) (void)hipHostFree(bp->hS);
@@ -292,7 +328,8 @@ static int bp_setup_impl(nbx_bp* bp)
       (rc = bp->alloc(&bp->lambda, 2 * (int64_t)bp->fieldOffset)) || (rc = bp->alloc(&bp->invDiagA, bp->Nlocal)) ||
       (rc = bp->alloc(&bp->rhs, bp->Nall)) || (rc = bp->alloc(&bp->p, bp->Nall)) ||
       (rc = bp->alloc(&bp->z, bp->Nall)) || (rc = bp->alloc(&bp->Ap, bp->Nall)) || (rc = bp->alloc(&bp->Ax, bp->Nall)) ||
-      (rc = bp->alloc(&bp->elem_partials, E)))
+      (rc = bp->alloc(&bp->elem_partials, E)) || (rc = bp->alloc(&bp->b, bp->Nall)) ||
+      (rc = bp->alloc(&bp->d, bp->Nall)))
     return rc;
   NBX_HIP(hipMemcpy(bp->D, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
   NBX_HIP(hipMemcpy(bp->ggeo, ggeo.data(), sizeof(double) * ggeo.size(), hipMemcpyHostToDevice));
@@ -465,9 +502,21 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
     hipLaunchKernelGGL(k_pcg_finish_dots<2>, dim3(1), dim3(256), 0, bp->stream, nb, bp->red.partials, bp->S,
                        bp->hist_rdotr);
   NBX_HIP_LAUNCH();
-  if (o.fused) {
+  if (o.fused || bp->op_prec != NBX_F64) {
     // p = z + beta p, Ap = A p and pAp element shares in one pass; then gs + mask
-    int rc = axhelm_pcg_launch(bp->N, bp->Nf, o.bpmode ? 1 : 0, bp->E, bp->fieldOffset, bp->ggeo, bp->D, lambda,
+    const void* g = bp->ggeo;
+    const void* Dm = bp->D;
+    const void* lam = lambda;
+    if (bp->op_prec == NBX_F32) {
+      g = bp->ggeo32;
+      Dm = bp->D32;
+      lam = bp->lambda32;
+    } else if (bp->op_prec == NBX_BF16) {
+      g = bp->ggeo16;
+      Dm = bp->D32;
+      lam = bp->lambda16;
+    }
+    int rc = axhelm_pcg_launch(bp->N, bp->Nf, o.bpmode ? 1 : 0, bp->op_prec, bp->E, bp->fieldOffset, g, Dm, lam,
                                zsrc, bp->p, bp->Ap, bp->elem_partials, reinterpret_cast<const PcgFlags*>(bp->S),
                                bp->stream);
     if (rc) return rc;
@@ -495,12 +544,12 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
   return NBX_SUCCESS;
 }
 
-extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
-                          nbx_pcg_stats* stats)
+// PCG core (BPPCG, solve.cpp:44-113).  initial: r = r - A x first (x may be
+// nonzero); otherwise r is taken as the residual of x = 0.  abs_floor is the
+// absolute part of the stopping threshold (tol^2 in BPPCG, 0 for inner solves).
+static int bp_pcg_core(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, double abs_floor,
+                       int maxit, bool initial, bool jacobi_setup, PcgScalars* out)
 {
-  if (!bp || !r || !x) return fail(NBX_ERR_INVALID, "bp_pcg: bad arguments");
-  if (maxit < 0) return fail(NBX_ERR_INVALID, "bp_pcg: maxit < 0");
-  if (!lambda) lambda = bp->lambda;
   if (maxit + 2 > bp->hist_cap) {
     NBX_HIP(hipStreamSynchronize(bp->stream));
     if (bp->hist_rdotr) (void)hipFree(bp->hist_rdotr);
@@ -515,18 +564,19 @@ extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x
   s0.rdotz1 = 1.0;  // solve.cpp:58-60
   s0.rdotz2 = 0.0;
   s0.tol2 = tol * tol;
+  s0.abs_floor = abs_floor;
   s0.maxit = maxit;
   s0.fixed = tol > 0 ? 0 : 1;  // solve() sets FIXED ITERATION COUNT when tol <= 0 (main.cpp:17-22)
   s0.flexible = bp->opt.flexible ? 1 : 0;
   *bp->hS = s0;
   NBX_HIP(hipMemcpyAsync(bp->S, bp->hS, sizeof(PcgScalars), hipMemcpyHostToDevice, bp->stream));
-  NBX_HIP(hipEventRecord(bp->ev0, bp->stream));
-  // Ax = A x; r = r - Ax  (solve.cpp:68-69)
-  int rc = bp_apply(bp, lambda, x, bp->Ax, nullptr);
-  if (rc) return rc;
-  rc = nbx_scaled_add((nbx_dlong)bp->Nall, -1.0, bp->Ax, 1.0, r, bp->stream);
-  if (rc) return rc;
-  if (bp->opt.precon == NBX_PRECON_JACOBI && (rc = bp_update_jacobi(bp, lambda))) return rc;  // solve.cpp:72
+  int rc;
+  if (initial) {  // Ax = A x; r = r - Ax  (solve.cpp:68-69)
+    if ((rc = bp_apply(bp, lambda, x, bp->Ax, nullptr))) return rc;
+    if ((rc = nbx_scaled_add((nbx_dlong)bp->Nall, -1.0, bp->Ax, 1.0, r, bp->stream))) return rc;
+  }
+  if (jacobi_setup && bp->opt.precon == NBX_PRECON_JACOBI && (rc = bp_update_jacobi(bp, lambda)))
+    return rc;  // solve.cpp:72
   // iterations in chunks; the device flag ends the work, the host only polls
   const int chunk = 16;
   int enqueued = 0;
@@ -539,21 +589,126 @@ extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x
     NBX_HIP(hipStreamSynchronize(bp->stream));
     if (bp->hS->done || enqueued > maxit) break;
   }
+  *out = *bp->hS;
+  bp->last_iters = out->iterations;
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
+                          nbx_pcg_stats* stats)
+{
+  if (!bp || !r || !x) return fail(NBX_ERR_INVALID, "bp_pcg: bad arguments");
+  if (maxit < 0) return fail(NBX_ERR_INVALID, "bp_pcg: maxit < 0");
+  if (!lambda) lambda = bp->lambda;
+  bp->op_prec = NBX_F64;
+  NBX_HIP(hipEventRecord(bp->ev0, bp->stream));
+  PcgScalars S{};
+  int rc = bp_pcg_core(bp, lambda, r, x, tol, tol * tol, maxit, true, true, &S);
+  if (rc) return rc;
   NBX_HIP(hipEventRecord(bp->ev1, bp->stream));
   NBX_HIP(hipEventSynchronize(bp->ev1));
   float ms = 0.f;
   NBX_HIP(hipEventElapsedTime(&ms, bp->ev0, bp->ev1));
-  const PcgScalars& S = *bp->hS;
-  bp->last_iters = S.iterations;
   if (stats) {
     stats->iterations = S.iterations;
     stats->converged = S.converged;
     stats->rdotr0 = S.rdotr0;
     stats->rdotr = S.rdotr;
     stats->elapsed_s = ms * 1e-3;
+    stats->cycles = 1;
   }
   if (S.breakdown) return fail(NBX_ERR_DIVERGED, "bp_pcg: breakdown at iteration %d", S.iter);
   return S.iterations;
+}
+
+static int bp_prepare_precision(nbx_bp* bp, nbx_prec prec, const double* lambda)
+{
+  const int64_t ng = (int64_t)bp->E * 7 * bp->Np, nl = 2 * (int64_t)bp->fieldOffset;
+  if (prec == NBX_F32 || prec == NBX_BF16) {
+    if (!bp->D32) {
+      NBX_HIP(hipMalloc(&bp->D32, sizeof(float) * bp->Nq * bp->Nq));
+      hipLaunchKernelGGL(k_cvt_f32, grid_for(bp->Nq * bp->Nq), dim3(256), 0, bp->stream, (int64_t)bp->Nq * bp->Nq,
+                         bp->D, bp->D32);
+    }
+  }
+  if (prec == NBX_F32) {
+    if (!bp->ggeo32) {
+      NBX_HIP(hipMalloc(&bp->ggeo32, sizeof(float) * ng));
+      NBX_HIP(hipMalloc(&bp->lambda32, sizeof(float) * nl));
+      hipLaunchKernelGGL(k_cvt_f32, grid_for(ng), dim3(256), 0, bp->stream, ng, bp->ggeo, bp->ggeo32);
+    }
+    hipLaunchKernelGGL(k_cvt_f32, grid_for(nl), dim3(256), 0, bp->stream, nl, lambda, bp->lambda32);
+  } else if (prec == NBX_BF16) {
+    if (!bp->ggeo16) {
+      NBX_HIP(hipMalloc(&bp->ggeo16, sizeof(uint16_t) * ng));
+      NBX_HIP(hipMalloc(&bp->lambda16, sizeof(uint16_t) * nl));
+      hipLaunchKernelGGL(k_cvt_bf16, grid_for(ng), dim3(256), 0, bp->stream, ng, bp->ggeo, bp->ggeo16);
+    }
+    hipLaunchKernelGGL(k_cvt_bf16, grid_for(nl), dim3(256), 0, bp->stream, nl, lambda, bp->lambda16);
+  } else if (prec != NBX_F64) {
+    return fail(NBX_ERR_UNSUPPORTED, "bp: operator precision %d", (int)prec);
+  }
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
+                                nbx_prec op_prec, double inner_tol, int max_cycles, nbx_pcg_stats* stats)
+{
+  if (!bp || !r || !x || tol <= 0 || inner_tol <= 0 || max_cycles < 1 || maxit < 0)
+    return fail(NBX_ERR_INVALID, "bp_pcg_mixed: bad arguments");
+  if (!lambda) lambda = bp->lambda;
+  int rc = bp_prepare_precision(bp, op_prec, lambda);
+  if (rc) return rc;
+  NBX_HIP(hipEventRecord(bp->ev0, bp->stream));
+  NBX_HIP(hipMemcpyAsync(bp->b, r, sizeof(double) * bp->Nall, hipMemcpyDeviceToDevice, bp->stream));
+  const double* w = nbx_ogs_inv_degree(bp->ogs);
+  int total = 0, cycles = 0;
+  double rr0 = 0.0, rr = 0.0;
+  bool conv = false, breakdown = false;
+  for (;;) {
+    // true residual in FP64: r = b - A x
+    NBX_HIP(hipMemcpyAsync(r, bp->b, sizeof(double) * bp->Nall, hipMemcpyDeviceToDevice, bp->stream));
+    if ((rc = bp_apply(bp, lambda, x, bp->Ax, nullptr))) return rc;
+    if ((rc = nbx_scaled_add((nbx_dlong)bp->Nall, -1.0, bp->Ax, 1.0, r, bp->stream))) return rc;
+    if ((rc = launch_wdot(bp->Nlocal, bp->fieldOffset, bp->Nf, w, r, r, false, bp->red, bp->red.result, bp->stream)))
+      return rc;
+    NBX_HIP(hipMemcpyAsync(&bp->hS->rdotr, bp->red.result, sizeof(double), hipMemcpyDeviceToHost, bp->stream));
+    NBX_HIP(hipStreamSynchronize(bp->stream));
+    rr = bp->hS->rdotr;
+    if (cycles == 0) rr0 = rr;
+    if (rr <= tol * tol * rr0) {
+      conv = true;
+      break;
+    }
+    if (cycles >= max_cycles || total >= maxit || breakdown) break;
+    // inner solve A_lo d = r from d = 0 (operator in op_prec, vectors, dots and updates FP64)
+    NBX_HIP(hipMemsetAsync(bp->d, 0, sizeof(double) * bp->Nall, bp->stream));
+    bp->op_prec = op_prec;
+    PcgScalars S{};
+    rc = bp_pcg_core(bp, lambda, r, bp->d, inner_tol, 0.0, maxit - total, false, cycles == 0, &S);
+    bp->op_prec = NBX_F64;
+    if (rc) return rc;
+    total += S.iterations;
+    breakdown = S.breakdown != 0;
+    hipLaunchKernelGGL(k_axpy1, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, bp->d, x);
+    NBX_HIP_LAUNCH();
+    ++cycles;
+    if (S.iterations == 0) break;  // no progress possible
+  }
+  NBX_HIP(hipEventRecord(bp->ev1, bp->stream));
+  NBX_HIP(hipEventSynchronize(bp->ev1));
+  float ms = 0.f;
+  NBX_HIP(hipEventElapsedTime(&ms, bp->ev0, bp->ev1));
+  if (stats) {
+    stats->iterations = total;
+    stats->converged = conv ? 1 : 0;
+    stats->rdotr0 = rr0;
+    stats->rdotr = rr;
+    stats->elapsed_s = ms * 1e-3;
+    stats->cycles = cycles;
+  }
+  return total;
 }
 
 extern "C" int nbx_bp_history(const nbx_bp* bp, double* rdotr, double* alpha, int capacity)

@@ -41,9 +41,11 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
 int gs_launch(const nbx_ogs* o, void* v, int k, int32_t stride, nbx_type type, nbx_op op, const int* skip,
               hipStream_t s);
 int mask_launch(int32_t Nmasked, const int32_t* maskIds, double* q, const int* skip, hipStream_t s);
-// p = z + beta p; Ap = A p (local); partials[e] = element share of pAp
-int axhelm_pcg_launch(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const double* ggeo,
-                      const double* D, const double* lambda, const double* z, double* p, double* Ap,
+// p = z + beta p (FP64); Ap = A p (local) with the operator in precision prec
+// (F64: double ggeo/D/lambda; F32: float; BF16: bf16 ggeo/lambda, float D and
+// float arithmetic); partials[e] = element share of pAp (FP64)
+int axhelm_pcg_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32_t offset, const void* ggeo,
+                      const void* D, const void* lambda, const double* z, double* p, double* Ap,
                       double* partials, const PcgFlags* S, hipStream_t s);
 // double add gather-scatter of every field plus the Dirichlet mask, one launch
 int gs_add_mask_launch(const nbx_ogs* o, double* v, int k, int32_t stride, int32_t Nmasked,

@@ -130,6 +130,12 @@ double nbo_bp_wdot(const nbo_bp* bp, const double* x, const double* y);
 int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r, double* x, double tol, int maxit,
                double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st);
 
+/* Mixed precision with iterative refinement (library: nbx_bp_pcg_mixed):
+ * prec 0 = FP64, 1 = FP32 operator, 2 = bf16 ggeo/lambda with FP32 arithmetic.
+ * st->iter holds the number of refinement cycles. */
+int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, double* r, double* x, double tol,
+                     int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -259,6 +259,12 @@ class OracleBP:
                          p(hr), p(ha), C.byref(st))
         return st, hr, ha
 
+    def pcg_mixed(self, r, x, tol, maxit, prec=1, inner_tol=1e-5, max_cycles=20, jacobi=True, flexible=False):
+        st = PCGStats()
+        lib().nbo_bp_pcg_mixed(C.byref(self.struct), int(jacobi), int(flexible), int(prec), p(r), p(x),
+                               C.c_double(tol), maxit, C.c_double(inner_tol), max_cycles, C.byref(st))
+        return st
+
     def __del__(self):
         try:
             ogs_free(self.ogs)

@@ -21,6 +21,7 @@
  */
 #include <math.h>
 #include <stdlib.h>
+#include <stdint.h>
 #include <string.h>
 #include "nbo.h"
 
@@ -103,8 +104,56 @@ static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
   return r;
 }
 
-int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r, double* x, double tol, int maxit,
-               double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
+/* operands of a low-precision operator: float copies (prec 1) or bf16
+ * values held in float (prec 2, round-to-nearest-even of the float) */
+typedef struct { float *g, *D, *lam; } lo_ops;
+
+static float to_bf16f(double v)
+{
+  union { float f; uint32_t u; } a;
+  a.f = (float)v;
+  a.u = (a.u + 0x7FFFu + ((a.u >> 16) & 1u)) & 0xFFFF0000u;
+  return a.f;
+}
+
+static void lo_make(const nbo_bp* bp, int prec, lo_ops* lo)
+{
+  const int Nq = bp->N + 1, Np = Nq * Nq * Nq;
+  const int64_t ng = (int64_t)bp->E * 7 * Np, nl = 2 * (int64_t)bp->offset;
+  lo->g = (float*)malloc(sizeof(float) * ng);
+  lo->lam = (float*)malloc(sizeof(float) * nl);
+  lo->D = (float*)malloc(sizeof(float) * Nq * Nq);
+  for (int i = 0; i < Nq * Nq; ++i) lo->D[i] = (float)bp->D[i];
+  for (int64_t i = 0; i < ng; ++i) lo->g[i] = prec == 2 ? to_bf16f(bp->ggeo[i]) : (float)bp->ggeo[i];
+  for (int64_t i = 0; i < nl; ++i) lo->lam[i] = prec == 2 ? to_bf16f(bp->lambda[i]) : (float)bp->lambda[i];
+}
+
+static void lo_free(lo_ops* lo) { free(lo->g); free(lo->lam); free(lo->D); }
+
+/* Ap (FP64) = A_lo p: p rounded to float, the float operator, result widened */
+static void lo_axhelm(const nbo_bp* bp, const lo_ops* lo, const double* p, double* Ap)
+{
+  const int64_t Nall = (int64_t)bp->Nf * bp->offset;
+  float* p32 = (float*)malloc(sizeof(float) * Nall);
+  float* a32 = (float*)malloc(sizeof(float) * Nall);
+  for (int64_t n = 0; n < Nall; ++n) p32[n] = (float)p[n];
+  nbo_axhelm_f32(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, lo->g, lo->D, lo->lam, p32, a32);
+  for (int64_t n = 0; n < Nall; ++n) Ap[n] = (double)a32[n];
+  free(p32); free(a32);
+}
+
+static void local_jacobi(const nbo_bp* bp, double* invDiag)
+{
+  const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  nbo_jacobi_local(bp->N, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, invDiag);
+  nbo_gs_many(bp->ogs, invDiag, 1, bp->offset, NBO_DOUBLE, NBO_ADD);
+  for (int64_t n = 0; n < N; ++n) invDiag[n] = 1.0 / invDiag[n];
+}
+
+/* PCG core; lo == NULL: FP64 operator.  initial: r -= A x first. */
+static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const lo_ops* lo, const double* invDiag,
+                    double* r, double* x, double tol, double abs_floor, int maxit, int initial,
+                    double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
 {
   const int Np = (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
   const int64_t N = (int64_t)bp->E * Np;
@@ -114,18 +163,12 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
   double* z = (double*)calloc(Nall, sizeof(double));
   double* Ap = (double*)calloc(Nall, sizeof(double));
   double* Ax = (double*)calloc(Nall, sizeof(double));
-  double* invDiag = NULL;
   double rdotz1 = 1.0, rdotz2 = 0.0, alpha = 0.0, beta = 0.0, TOL = 0.0, rdotr = 0.0, rdotr0 = 0.0;
   int converged = 0, breakdown = 0, iterations = maxit, it;
 
-  /* Ax = A x; r = r - Ax (solve.cpp:68-69) */
-  nbo_bp_apply(bp, x, Ax);
-  for (int64_t n = 0; n < Nall; ++n) r[n] = -1.0 * Ax[n] + 1.0 * r[n];
-  if (jacobi) {  /* solve.cpp:72, 146-165 */
-    invDiag = (double*)malloc(sizeof(double) * N);
-    nbo_jacobi_local(bp->N, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, invDiag);
-    nbo_gs_many(bp->ogs, invDiag, 1, bp->offset, NBO_DOUBLE, NBO_ADD);
-    for (int64_t n = 0; n < N; ++n) invDiag[n] = 1.0 / invDiag[n];
+  if (initial) {  /* Ax = A x; r = r - Ax (solve.cpp:68-69) */
+    nbo_bp_apply(bp, x, Ax);
+    for (int64_t n = 0; n < Nall; ++n) r[n] = -1.0 * Ax[n] + 1.0 * r[n];
   }
   for (it = 1;; ++it) {
     if (it > maxit) { iterations = maxit; break; }
@@ -148,7 +191,7 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
     rdotz1 = d[0];
     rdotr = d[1];
     if (hist_rdotr) hist_rdotr[it] = rdotr;
-    if (it == 1) { rdotr0 = rdotr; TOL = fmax(tol * tol * rdotr, tol * tol); }  /* solve.cpp:85 */
+    if (it == 1) { rdotr0 = rdotr; TOL = fmax(tol * tol * rdotr, abs_floor); }  /* solve.cpp:85 */
     if (!isfinite(rdotr)) { breakdown = 1; iterations = it - 1; break; }
     if (rdotr <= TOL && !fixed) { converged = 1; iterations = it - 1; break; }  /* :90 */
     if (flexible) beta = -alpha * d[2] / rdotz2;                                    /* :92-95 */
@@ -157,8 +200,9 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
     for (int64_t n = 0; n < Nall; ++n) p[n] = fma(beta, p[n], zsrc[n]);
     /* Ap = A p; pAp = (p.Ap)_w; alpha = rdotz/pAp (:102-104) */
     double pAp;
-    if (fused) {  /* pAp from the local operator output, then gs + mask */
-      nbo_axhelm(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, p, Ap);
+    if (fused || lo) {  /* pAp from the local operator output, then gs + mask */
+      if (lo) lo_axhelm(bp, lo, p, Ap);
+      else nbo_axhelm(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, p, Ap);
       pAp = fused_pap(bp, p, Ap);
       nbo_gs_many(bp->ogs, Ap, bp->Nf, bp->offset, NBO_DOUBLE, NBO_ADD);
       for (int f = 0; f < bp->Nf; ++f)
@@ -180,6 +224,67 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
     st->iterations = iterations; st->converged = converged; st->breakdown = breakdown; st->iter = it;
     st->rdotr0 = rdotr0; st->rdotr = rdotr; st->TOL = TOL;
   }
-  free(p); free(z); free(Ap); free(Ax); free(invDiag);
+  free(p); free(z); free(Ap); free(Ax);
   return iterations;
+}
+
+int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r, double* x, double tol, int maxit,
+               double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
+{
+  const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  double* invDiag = NULL;
+  if (jacobi) {
+    /* the initial residual is formed before the Jacobi setup (solve.cpp:68-72);
+     * the diagonal does not depend on r, so the order does not matter */
+    invDiag = (double*)malloc(sizeof(double) * N);
+    local_jacobi(bp, invDiag);
+  }
+  const int its = pcg_core(bp, jacobi, flexible, fused, NULL, invDiag, r, x, tol, tol * tol, maxit, 1,
+                           hist_rdotr, hist_alpha, st);
+  free(invDiag);
+  return its;
+}
+
+/* Iterative refinement around PCG with a low-precision operator (library's
+ * nbx_bp_pcg_mixed): r = b - A64 x; each cycle solves A_lo d = r from d = 0
+ * to inner_tol (no absolute floor), x += d; stop on the true residual. */
+int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, double* r, double* x, double tol,
+                     int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st)
+{
+  const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  const int64_t Nall = (int64_t)bp->Nf * bp->offset;
+  lo_ops lo = {0};
+  if (prec != 0) lo_make(bp, prec, &lo);
+  double* b = (double*)malloc(sizeof(double) * Nall);
+  double* Ax = (double*)malloc(sizeof(double) * Nall);
+  double* d = (double*)malloc(sizeof(double) * Nall);
+  double* invDiag = NULL;
+  memcpy(b, r, sizeof(double) * Nall);
+  int total = 0, cycles = 0, conv = 0, breakdown = 0;
+  double rr0 = 0.0, rr = 0.0;
+  for (;;) {
+    nbo_bp_apply(bp, x, Ax);
+    for (int64_t n = 0; n < Nall; ++n) r[n] = -1.0 * Ax[n] + 1.0 * b[n];
+    rr = nbo_bp_wdot(bp, r, r);
+    if (cycles == 0) rr0 = rr;
+    if (rr <= tol * tol * rr0) { conv = 1; break; }
+    if (cycles >= max_cycles || total >= maxit || breakdown) break;
+    if (jacobi && !invDiag) { invDiag = (double*)malloc(sizeof(double) * N); local_jacobi(bp, invDiag); }
+    memset(d, 0, sizeof(double) * Nall);
+    nbo_pcg_stats s;
+    pcg_core(bp, jacobi, flexible, 1, prec ? &lo : NULL, invDiag, r, d, inner_tol, 0.0, maxit - total, 0,
+             NULL, NULL, &s);
+    total += s.iterations;
+    breakdown = s.breakdown;
+    for (int64_t n = 0; n < Nall; ++n) x[n] = x[n] + d[n];
+    ++cycles;
+    if (s.iterations == 0) break;
+  }
+  if (st) {
+    st->iterations = total; st->converged = conv; st->breakdown = breakdown; st->iter = cycles;
+    st->rdotr0 = rr0; st->rdotr = rr; st->TOL = tol * tol * rr0;
+  }
+  free(b); free(Ax); free(d); free(invDiag);
+  if (prec != 0) lo_free(&lo);
+  return total;
 }
