@@ -57,6 +57,45 @@ def cpu_baseline(N, NX, NY, NZ, iters):
     }
 
 
+def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8):
+    """Time to a 1e-8 relative FP64 residual on the same problem: FP64 Jacobi
+    PCG without the iteration cap, and the mixed-precision refinement solvers.
+    Each solver runs once untimed, then once timed; the error of each
+    solution is reported against the FP64 solution."""
+    import torch
+    out = {}
+    x64 = None
+
+    def run(name, fn):
+        nonlocal x64
+        for timed in (False, True):
+            r.copy_(b)
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = fn()
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+        rel = (st.rdotr / st.rdotr0) ** 0.5 if st.rdotr0 > 0 else 0.0
+        rec = {"time_s": round(t, 6), "iterations": st.iterations, "cycles": st.cycles,
+               "converged": bool(st.converged), "rel_residual": rel}
+        if x64 is None:
+            x64 = x.clone()
+        else:
+            rec["err_vs_fp64"] = float((x - x64).abs().max() / x64.abs().max())
+        out[name] = rec
+        log(f"[bench] {name}: {rec}")
+
+    run("fp64_jacobi", lambda: bp.pcg(r, x, tol, 20000))
+    run("mixed_f32_jacobi", lambda: bp.pcg_mixed(r, x, tol, 20000, prec=nbx.NBX_F32, inner_tol=1e-5, max_cycles=20))
+    run("mixed_bf16_jacobi", lambda: bp.pcg_mixed(r, x, tol, 20000, prec=nbx.NBX_BF16, inner_tol=2e-2,
+                                                  max_cycles=40))
+    base = out["fp64_jacobi"]["time_s"]
+    for k, v in out.items():
+        v["speedup_vs_fp64_jacobi"] = round(base / v["time_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +107,8 @@ def main():
     ap.add_argument("--ax-launches", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the time-to-1e-8 comparison of the FP64, mixed and learned solvers")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short fixed sequence for rocprofv3 counter runs: a 1 GiB device copy (calibration), "
                          "5 axhelm launches, one 10-iteration PCG solve; prints no JSON")
@@ -148,6 +189,8 @@ def main():
     log(f"[bench] axhelm BK N=7 E={E}: {ax_s * 1e6:.1f} us/launch, {ax_gbs:.0f} GB/s, "
         f"{E * N ** 3 / ax_s / 1e9:.1f} GDOF/s")
 
+    compare = None if args.no_compare else solver_comparison(nbx, bp, b, r, x, log)
+
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists() and NX == 32:
@@ -160,7 +203,8 @@ def main():
                                f"tol {args.tol:g} or {args.maxit} iterations", "elements": E, "N": N,
                    "iterations": its, "converged": bool(st.converged),
                    "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
-                   "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "parallelism": "single GPU"},
+                   "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "parallelism": "single GPU",
+                   "time_to_1e-8": compare},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_axhelm<double,8,1,BK>", "bytes_per_launch": ax_bytes,

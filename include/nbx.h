@@ -156,7 +156,7 @@ int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, doub
  * diagonal and the PCG workspace; r and x are the caller's. */
 typedef struct nbx_bp nbx_bp;
 
-typedef enum { NBX_PRECON_COPY = 0, NBX_PRECON_JACOBI = 1 } nbx_precon;
+typedef enum { NBX_PRECON_COPY = 0, NBX_PRECON_JACOBI = 1, NBX_PRECON_LEARNED = 2 } nbx_precon;
 
 typedef struct {
   nbx_box box;
@@ -183,6 +183,7 @@ typedef struct {
   double rdotr;         /* weighted |r|^2 at exit */
   double elapsed_s;     /* wall time of the solve on the device */
   int cycles;           /* refinement cycles (1 for nbx_bp_pcg) */
+  int fallback_iteration; /* learned preconditioner: iteration of the switch to Jacobi (0 = none) */
 } nbx_pcg_stats;
 
 nbx_bp* nbx_bp_setup(const nbx_bp_options* opt, void* stream);
@@ -210,6 +211,41 @@ int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double to
  * true residual. */
 int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
                      nbx_prec op_prec, double inner_tol, int max_cycles, nbx_pcg_stats* stats);
+/* ======================= learned preconditioner ==========================
+ * A per-element network over the 8^3 GLL nodes (N = 7, one field): inputs
+ * the residual (scaled by the element's max |r|) and the 7 geometric factors
+ * (each divided by its RMS over the mesh); three layers (3x3x3 convolution of
+ * the residual plus a pointwise map of the factors -> 32 channels, ReLU;
+ * 32 -> 32 pointwise, ReLU; 3x3x3 convolution 32 -> 1); bf16 weights, FP32
+ * arithmetic.  Output z_e = invDiag (r + s f), assembled by gather-scatter
+ * with weights 1/multiplicity.  The zero-initialised output layer makes the
+ * untrained preconditioner Jacobi.  Used by nbx_bp_pcg with
+ * precon = NBX_PRECON_LEARNED inside flexible PCG; after a stall (|r|^2 not
+ * below ratio x its value window iterations earlier) the solve continues with
+ * Jacobi.  No counterpart in the reference (new feature of this build). */
+typedef struct nbx_net nbx_net;
+int nbx_net_num_params(void);
+nbx_net* nbx_net_create(uint64_t seed, void* stream);
+void nbx_net_free(nbx_net* net);  /* detaches it from every BP handle it is attached to */
+int nbx_net_get_params(const nbx_net* net, float* host);      /* FP32 master weights */
+int nbx_net_set_params(nbx_net* net, const float* host);
+int nbx_net_save(const nbx_net* net, const char* path);        /* header, seed, params, checksum */
+nbx_net* nbx_net_load(const char* path, void* stream);
+int nbx_bp_attach_net(nbx_bp* bp, nbx_net* net);  /* the handle uses the network until either is freed */
+int nbx_bp_set_fallback(nbx_bp* bp, int stall_window, double stall_ratio);
+int nbx_bp_update_jacobi(nbx_bp* bp);                          /* invDiagA of the handle */
+/* element-local output (weighted: times 1/multiplicity) for r (device) */
+int nbx_net_apply(nbx_bp* bp, const double* r, double* zl, int weighted);
+/* training pairs (r_k, x* - x_k) from Jacobi PCG snapshots; returns the count */
+int nbx_net_generate_pairs(nbx_net* net, nbx_bp* bp, int nseeds, uint64_t seed0, const int* snapshots, int nsnap,
+                           double tol_exact);
+int nbx_net_set_pairs(nbx_net* net, nbx_bp* bp, int npairs, const float* R, const float* E);
+/* loss sum_samples sum_n (z - e)^2 / sum_pair e^2 and its parameter gradient
+ * over (pair, element) samples (host int pairs), chunk samples per workgroup */
+int nbx_net_gradient(nbx_net* net, nbx_bp* bp, const int32_t* samples, int nsamples, int chunk, float* grad,
+                     double* loss);
+int nbx_net_train(nbx_net* net, nbx_bp* bp, int steps, int batch, float lr, uint64_t seed, double* loss_hist);
+
 /* per-iteration history of the last solve (host outputs, length >= iterations+1):
  * rdotr and alpha per iteration; returns the count written */
 int nbx_bp_history(const nbx_bp* bp, double* rdotr, double* alpha, int capacity);

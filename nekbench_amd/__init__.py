@@ -19,7 +19,7 @@ LIB_PATH = PKG / "libnbx.so"
 NBX_F64, NBX_F32, NBX_BF16, NBX_F16 = 0, 1, 2, 3
 NBX_DOUBLE, NBX_FLOAT, NBX_INT, NBX_LONG = 0, 1, 2, 3
 NBX_ADD, NBX_MUL, NBX_MIN, NBX_MAX = 0, 1, 2, 3
-PRECON_COPY, PRECON_JACOBI = 0, 1
+PRECON_COPY, PRECON_JACOBI, PRECON_LEARNED = 0, 1, 2
 # reference ggeo factor order (core/params.h:55-62)
 GWJID, G00ID, G01ID, G02ID, G11ID, G12ID, G22ID, NGGEO = 0, 1, 2, 3, 4, 5, 6, 7
 
@@ -61,7 +61,8 @@ class BPOptions(C.Structure):
 
 class PCGStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("rdotr0", C.c_double),
-                ("rdotr", C.c_double), ("elapsed_s", C.c_double), ("cycles", C.c_int)]
+                ("rdotr", C.c_double), ("elapsed_s", C.c_double), ("cycles", C.c_int),
+                ("fallback_iteration", C.c_int)]
 
 
 _P = C.c_void_p
@@ -355,3 +356,117 @@ def _wrap_device(addr: int, n: int, dtype):
     if rc != 0:
         raise NbxError(f"hipMemcpy failed ({rc})")
     return out
+
+
+_SIGS.update({
+    "nbx_net_num_params": (_I, []),
+    "nbx_net_create": (_P, [C.c_uint64, _P]),
+    "nbx_net_free": (None, [_P]),
+    "nbx_net_get_params": (_I, [_P, _P]),
+    "nbx_net_set_params": (_I, [_P, _P]),
+    "nbx_net_save": (_I, [_P, C.c_char_p]),
+    "nbx_net_load": (_P, [C.c_char_p, _P]),
+    "nbx_bp_attach_net": (_I, [_P, _P]),
+    "nbx_bp_set_fallback": (_I, [_P, _I, _D]),
+    "nbx_bp_update_jacobi": (_I, [_P]),
+    "nbx_net_apply": (_I, [_P, _P, _P, _I]),
+    "nbx_net_generate_pairs": (_I, [_P, _P, _I, C.c_uint64, _P, _I, _D]),
+    "nbx_net_set_pairs": (_I, [_P, _P, _I, _P, _P]),
+    "nbx_net_gradient": (_I, [_P, _P, _P, _I, _I, _P, _P]),
+    "nbx_net_train": (_I, [_P, _P, _I, _I, C.c_float, C.c_uint64, _P]),
+})
+
+
+class Net:
+    """Learned element preconditioner (nbx_net_*)."""
+
+    def __init__(self, seed=1, handle=None, stream=None):
+        l = lib()
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name, None)
+            if f is not None:
+                f.restype, f.argtypes = res, args
+        self.h = handle or l.nbx_net_create(seed, stream_ptr(stream))
+        if not self.h:
+            raise NbxError(f"nbx_net_create failed: {last_error()}")
+        self.nparams = l.nbx_net_num_params()
+
+    @classmethod
+    def load(cls, path, stream=None):
+        l = lib()
+        l.nbx_net_load.restype, l.nbx_net_load.argtypes = _P, [C.c_char_p, _P]
+        h = l.nbx_net_load(str(path).encode(), stream_ptr(stream))
+        if not h:
+            raise NbxError(f"nbx_net_load failed: {last_error()}")
+        return cls(handle=h)
+
+    def save(self, path):
+        check(lib().nbx_net_save(self.h, str(path).encode()), "nbx_net_save")
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, dtype=np.float32)
+        check(lib().nbx_net_get_params(self.h, ptr(out)), "nbx_net_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().nbx_net_set_params(self.h, ptr(p)), "nbx_net_set_params")
+
+    def attach(self, bp: "BP"):
+        check(lib().nbx_bp_attach_net(bp.h, self.h), "nbx_bp_attach_net")
+        bp._net = self  # the handle uses the weights: keep them alive as long as it
+
+    def apply(self, bp: "BP", r, zl, weighted=True):
+        check(lib().nbx_net_apply(bp.h, ptr(r), ptr(zl), int(weighted)), "nbx_net_apply")
+
+    def generate_pairs(self, bp: "BP", nseeds, seed0, snapshots, tol_exact=1e-11) -> int:
+        snaps = np.ascontiguousarray(snapshots, dtype=np.int32)
+        k = lib().nbx_net_generate_pairs(self.h, bp.h, nseeds, seed0, ptr(snaps), snaps.size, tol_exact)
+        if k < 0:
+            raise NbxError(f"nbx_net_generate_pairs failed: {last_error()}")
+        return k
+
+    def set_pairs(self, bp: "BP", R, E):
+        """R, E: device float32 tensors of shape (npairs, local nodes)."""
+        assert R.dim() == 2 and R.shape == E.shape
+        check(lib().nbx_net_set_pairs(self.h, bp.h, R.shape[0], ptr(R), ptr(E)), "nbx_net_set_pairs")
+
+    def gradient(self, bp: "BP", samples, chunk=16):
+        s = np.ascontiguousarray(samples, dtype=np.int32).reshape(-1, 2)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_double(0.0)
+        check(lib().nbx_net_gradient(self.h, bp.h, ptr(s), s.shape[0], chunk, ptr(g), C.byref(loss)),
+              "nbx_net_gradient")
+        return g, loss.value
+
+    def train(self, bp: "BP", steps, batch=2048, lr=1e-3, seed=1):
+        hist = np.zeros(max(steps, 1))
+        check(lib().nbx_net_train(self.h, bp.h, steps, batch, C.c_float(lr), seed, ptr(hist)), "nbx_net_train")
+        return hist[:steps]
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().nbx_net_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _bp_set_fallback(self, window, ratio):
+    l = lib()
+    l.nbx_bp_set_fallback.restype, l.nbx_bp_set_fallback.argtypes = _I, [_P, _I, _D]
+    check(l.nbx_bp_set_fallback(self.h, window, ratio), "nbx_bp_set_fallback")
+
+
+def _bp_update_jacobi(self):
+    l = lib()
+    l.nbx_bp_update_jacobi.restype, l.nbx_bp_update_jacobi.argtypes = _I, [_P]
+    check(l.nbx_bp_update_jacobi(self.h), "nbx_bp_update_jacobi")
+
+
+BP.set_fallback = _bp_set_fallback
+BP.update_jacobi = _bp_update_jacobi

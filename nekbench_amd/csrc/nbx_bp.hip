@@ -21,7 +21,8 @@
 // every element-wise operation is the same single fma in the same order.
 // The dots use the fixed reduction tree of nbx_reduce.h.  The CPU oracle
 // (nbo_bp_pcg) restates exactly this arithmetic.
-#include "nbx_blas.h"
+#include "nbx_bp_internal.h"
+#include "nbx_net.h"
 #include <chrono>
 #include <cstddef>
 #include <cmath>
@@ -32,16 +33,7 @@
 
 namespace nbx {
 
-// the leading members are PcgFlags (nbx_internal.h), read by operator kernels
-struct PcgScalars {
-  int done, pad_;
-  double beta;
-  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, pAp, zdotAp, abs_floor;
-  int iter, iterations, converged, breakdown, maxit, fixed, flexible;
-};
-static_assert(offsetof(PcgScalars, done) == offsetof(PcgFlags, done) &&
-                  offsetof(PcgScalars, beta) == offsetof(PcgFlags, beta),
-              "PcgScalars must start with PcgFlags");
+
 
 __device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, double rdotr, double zdotAp,
                                                double* hist_rdotr)
@@ -73,6 +65,14 @@ __device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, doub
     S->converged = 1;
     S->iterations = it - 1;
     return;
+  }
+  // learned preconditioner: Jacobi from the next iteration on when the
+  // residual has not fallen below stall_ratio of its value stall_window
+  // iterations ago (plain Jacobi PCG is the fallback)
+  if (S->learned && !S->fallback && it > S->stall_window &&
+      !(rdotr <= S->stall_ratio * hist_rdotr[it - S->stall_window])) {
+    S->fallback = 1;
+    S->fallback_iter = it;
   }
   if (S->flexible)
     S->beta = -S->alpha * S->zdotAp / S->rdotz2;  // solve.cpp:92-95
@@ -196,6 +196,51 @@ __global__ __launch_bounds__(256) void k_pcg_finish_pap(const unsigned int nb, c
   }
 }
 
+// x += alpha p; r -= alpha Ap of the previous iteration (BPUpdatePCG)
+__global__ __launch_bounds__(256) void k_pcg_update(const int64_t Nall, const double* __restrict__ p,
+                                                    const double* __restrict__ Ap, double* __restrict__ x,
+                                                    double* __restrict__ r, const PcgScalars* S)
+{
+  if (S->done || S->iter == 0) return;
+  const double alpha = S->alpha;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < Nall) {
+    x[n] = __builtin_fma(alpha, ldnt(&p[n]), ldnt(&x[n]));
+    r[n] = __builtin_fma(-alpha, ldnt(&Ap[n]), ldnt(&r[n]));
+  }
+}
+
+// tile partials of <r,z>_w, <r,r>_w, <z,Ap>_w for a given (assembled) z
+template <bool DEG>
+__global__ __launch_bounds__(256) void k_pcg_dots_z(const int32_t N, const int32_t offset, const int Nf,
+                                                    const double* __restrict__ w, const uint8_t* __restrict__ deg,
+                                                    const double* __restrict__ r, const double* __restrict__ z,
+                                                    const double* __restrict__ Ap, double* __restrict__ partials,
+                                                    const PcgScalars* S)
+{
+  if (S->done) return;
+  __shared__ double s4[3 * 4];
+  double v[3] = {0.0, 0.0, 0.0};
+  const int64_t b0 = (int64_t)blockIdx.x * kRedTile + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kRedPerLane; ++u) {
+    const int64_t id = b0 + u * kRedLanes;
+    if (id < N) {
+      const double wid = weight<DEG>(w, deg, id);
+      for (int f = 0; f < Nf; ++f) {
+        const int64_t iid = id + (int64_t)f * offset;
+        const double rf = ldnt(&r[iid]);
+        const double zf = ldnt(&z[iid]);
+        v[0] = __builtin_fma(wid * rf, zf, v[0]);
+        v[1] = __builtin_fma(wid * rf, rf, v[1]);
+        v[2] = __builtin_fma(wid * zf, ldnt(&Ap[iid]), v[2]);
+      }
+    }
+  }
+  block_sum256<3>(v, s4);
+  store_partials<3>(v, partials);
+}
+
 // vecInv (nekBone/kernel/utils.okl:605-613)
 __global__ __launch_bounds__(256) void k_vec_inv(const int64_t N, double* __restrict__ v)
 {
@@ -235,43 +280,6 @@ inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 using namespace nbx;
 
-struct nbx_bp {
-  nbx_bp_options opt{};
-  int N = 0, Nq = 0, Np = 0, Nf = 1;
-  int32_t E = 0, Nlocal = 0, fieldOffset = 0, Nmasked = 0;
-  int64_t Nall = 0;
-  hipStream_t stream = nullptr;
-  nbx_ogs* ogs = nullptr;
-  double *D = nullptr, *ggeo = nullptr, *lambda = nullptr, *invDiagA = nullptr, *rhs = nullptr;
-  double *p = nullptr, *z = nullptr, *Ap = nullptr, *Ax = nullptr;
-  double* elem_partials = nullptr;  // per-element shares of pAp (fused schedule)
-  // low-precision operator operands (built on first use)
-  float* ggeo32 = nullptr;
-  float* D32 = nullptr;
-  float* lambda32 = nullptr;
-  uint16_t* ggeo16 = nullptr;
-  uint16_t* lambda16 = nullptr;
-  double *b = nullptr, *d = nullptr;   // refinement: right-hand side copy, correction
-  nbx_prec op_prec = NBX_F64;         // operator precision of the PCG being run
-  int32_t* maskIds = nullptr;
-  ReduceWork red;
-  PcgScalars* S = nullptr;
-  PcgScalars* hS = nullptr;  // pinned host copy
-  double *hist_rdotr = nullptr, *hist_alpha = nullptr;
-  int hist_cap = 0;
-  int last_iters = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<void*> owned;
-
-  int alloc(double** ptr, int64_t n)
-  {
-    NBX_HIP(hipMalloc((void**)ptr, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
-    NBX_HIP(hipMemset(*ptr, 0, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
-    owned.push_back(*ptr);
-    return NBX_SUCCESS;
-  }
-};
-
 static void bp_release(nbx_bp* bp)
 {
   if (!bp) return;
@@ -286,6 +294,9 @@ static void bp_release(nbx_bp* bp)
   if (bp->ev1) (void)hipEventDestroy(bp->ev1);
   bp->red.release();
   if (bp->ogs) nbx_ogs_free(bp->ogs);
+  if (bp->feat) (void)hipFree(bp->feat);
+  if (bp->zl) (void)hipFree(bp->zl);
+  if (bp->net) nbx::net_detach(bp->net, bp);
   delete bp;
 }
 
@@ -483,6 +494,26 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
   const double* w = nbx_ogs_inv_degree(bp->ogs);
   const uint8_t* deg = ogs_degree(bp->ogs);
   const double* zsrc = jac ? bp->z : r;
+  if (o.precon == NBX_PRECON_LEARNED) {
+    // update; z = gs(w invDiag (r + s f(r))) (Jacobi after a stall); dots; finish
+    hipLaunchKernelGGL(k_pcg_update, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, bp->p, bp->Ap, x, r,
+                       bp->S);
+    int rc = net_apply_learned(bp, r, bp->z, &bp->S->fallback, &bp->S->done);
+    if (rc) return rc;
+    rc = gs_add_mask_launch(bp->ogs, bp->z, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, &bp->S->done,
+                            bp->stream);
+    if (rc) return rc;
+    if (deg)
+      hipLaunchKernelGGL(k_pcg_dots_z<true>, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, bp->Nf,
+                         w, deg, r, bp->z, bp->Ap, bp->red.partials, bp->S);
+    else
+      hipLaunchKernelGGL(k_pcg_dots_z<false>, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset,
+                         bp->Nf, w, deg, r, bp->z, bp->Ap, bp->red.partials, bp->S);
+    hipLaunchKernelGGL(k_pcg_finish_dots<3>, dim3(1), dim3(256), 0, bp->stream, nb, bp->red.partials, bp->S,
+                       bp->hist_rdotr);
+    NBX_HIP_LAUNCH();
+    zsrc = bp->z;
+  } else {
 #define NBX_PD(J, F, DG)                                                                                         \
   hipLaunchKernelGGL((k_pcg_update_precon_dots<J, F, DG>), dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal,        \
                      bp->fieldOffset, bp->Nf, w, deg, bp->invDiagA, bp->p, bp->Ap, x, r, bp->z, bp->red.partials,  \
@@ -502,7 +533,8 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
     hipLaunchKernelGGL(k_pcg_finish_dots<2>, dim3(1), dim3(256), 0, bp->stream, nb, bp->red.partials, bp->S,
                        bp->hist_rdotr);
   NBX_HIP_LAUNCH();
-  if (o.fused || bp->op_prec != NBX_F64) {
+  }
+  if (o.fused || bp->op_prec != NBX_F64 || o.precon == NBX_PRECON_LEARNED) {
     // p = z + beta p, Ap = A p and pAp element shares in one pass; then gs + mask
     const void* g = bp->ggeo;
     const void* Dm = bp->D;
@@ -568,6 +600,13 @@ static int bp_pcg_core(nbx_bp* bp, const double* lambda, double* r, double* x, d
   s0.maxit = maxit;
   s0.fixed = tol > 0 ? 0 : 1;  // solve() sets FIXED ITERATION COUNT when tol <= 0 (main.cpp:17-22)
   s0.flexible = bp->opt.flexible ? 1 : 0;
+  if (bp->opt.precon == NBX_PRECON_LEARNED) {
+    if (!bp->net) return fail(NBX_ERR_INVALID, "bp_pcg: LEARNED preconditioner without an attached network");
+    s0.learned = 1;
+    s0.flexible = 1;  // nonlinear, switching preconditioner: flexible beta (solve.cpp:92-95)
+    s0.stall_window = bp->stall_window;
+    s0.stall_ratio = bp->stall_ratio;
+  }
   *bp->hS = s0;
   NBX_HIP(hipMemcpyAsync(bp->S, bp->hS, sizeof(PcgScalars), hipMemcpyHostToDevice, bp->stream));
   int rc;
@@ -575,7 +614,8 @@ static int bp_pcg_core(nbx_bp* bp, const double* lambda, double* r, double* x, d
     if ((rc = bp_apply(bp, lambda, x, bp->Ax, nullptr))) return rc;
     if ((rc = nbx_scaled_add((nbx_dlong)bp->Nall, -1.0, bp->Ax, 1.0, r, bp->stream))) return rc;
   }
-  if (jacobi_setup && bp->opt.precon == NBX_PRECON_JACOBI && (rc = bp_update_jacobi(bp, lambda)))
+  if (jacobi_setup && (bp->opt.precon == NBX_PRECON_JACOBI || bp->opt.precon == NBX_PRECON_LEARNED) &&
+      (rc = bp_update_jacobi(bp, lambda)))
     return rc;  // solve.cpp:72
   // iterations in chunks; the device flag ends the work, the host only polls
   const int chunk = 16;
@@ -616,6 +656,7 @@ extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x
     stats->rdotr = S.rdotr;
     stats->elapsed_s = ms * 1e-3;
     stats->cycles = 1;
+    stats->fallback_iteration = S.fallback ? S.fallback_iter : 0;
   }
   if (S.breakdown) return fail(NBX_ERR_DIVERGED, "bp_pcg: breakdown at iteration %d", S.iter);
   return S.iterations;
@@ -707,6 +748,7 @@ extern "C" int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, dou
     stats->rdotr = rr;
     stats->elapsed_s = ms * 1e-3;
     stats->cycles = cycles;
+    stats->fallback_iteration = 0;
   }
   return total;
 }
@@ -720,4 +762,28 @@ extern "C" int nbx_bp_history(const nbx_bp* bp, double* rdotr, double* alpha, in
   if (rdotr) NBX_HIP(hipMemcpy(rdotr, bp->hist_rdotr, sizeof(double) * n, hipMemcpyDeviceToHost));
   if (alpha) NBX_HIP(hipMemcpy(alpha, bp->hist_alpha, sizeof(double) * n, hipMemcpyDeviceToHost));
   return n;
+}
+
+int bp_solve_plain(nbx_bp* bp, nbx_precon precon, double* r, double* x, double tol, int maxit, int* iterations)
+{
+  const nbx_precon keep = bp->opt.precon;
+  const int keepFlex = bp->opt.flexible;
+  bp->opt.precon = precon;
+  bp->opt.flexible = 0;
+  bp->op_prec = NBX_F64;
+  PcgScalars S{};
+  const int rc = bp_pcg_core(bp, bp->lambda, r, x, tol, tol * tol, maxit, true, true, &S);
+  bp->opt.precon = keep;
+  bp->opt.flexible = keepFlex;
+  if (iterations) *iterations = S.iterations;
+  return rc;
+}
+
+int bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out)
+{
+  std::vector<double> r(bp->Nall);
+  for (int f = 0; f < bp->Nf; ++f) nbx_urand_fill(seed, 3 + f, bp->Nlocal, r.data() + (int64_t)f * bp->fieldOffset);
+  NBX_HIP(hipMemcpyAsync(out, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, bp->stream));
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  return gs_add_mask_launch(bp->ogs, out, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, nullptr, bp->stream);
 }

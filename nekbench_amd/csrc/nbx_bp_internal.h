@@ -1,0 +1,84 @@
+// Internal state of the BP driver, shared by nbx_bp.hip and nbx_learned.hip.
+#pragma once
+#include "nbx_blas.h"
+#include <vector>
+
+#include <cstddef>
+
+struct nbx_net;
+
+namespace nbx {
+// the leading members are PcgFlags (nbx_internal.h), read by operator kernels
+struct PcgScalars {
+  int done, pad_;
+  double beta;
+  double rdotz1, rdotz2, rdotr, rdotr0, TOL, tol2, alpha, pAp, zdotAp, abs_floor;
+  int iter, iterations, converged, breakdown, maxit, fixed, flexible;
+  int learned, fallback, stall_window, fallback_iter;  // learned preconditioner state
+  double stall_ratio;
+};
+static_assert(offsetof(PcgScalars, done) == offsetof(PcgFlags, done) &&
+                  offsetof(PcgScalars, beta) == offsetof(PcgFlags, beta),
+              "PcgScalars must start with PcgFlags");
+}  // namespace nbx
+
+struct nbx_bp {
+  nbx_bp_options opt{};
+  int N = 0, Nq = 0, Np = 0, Nf = 1;
+  int32_t E = 0, Nlocal = 0, fieldOffset = 0, Nmasked = 0;
+  int64_t Nall = 0;
+  hipStream_t stream = nullptr;
+  nbx_ogs* ogs = nullptr;
+  double *D = nullptr, *ggeo = nullptr, *lambda = nullptr, *invDiagA = nullptr, *rhs = nullptr;
+  double *p = nullptr, *z = nullptr, *Ap = nullptr, *Ax = nullptr;
+  double* elem_partials = nullptr;  // per-element shares of pAp (fused schedule)
+  // low-precision operator operands (built on first use)
+  float* ggeo32 = nullptr;
+  float* D32 = nullptr;
+  float* lambda32 = nullptr;
+  uint16_t* ggeo16 = nullptr;
+  uint16_t* lambda16 = nullptr;
+  double *b = nullptr, *d = nullptr;   // refinement: right-hand side copy, correction
+  nbx_prec op_prec = NBX_F64;         // operator precision of the PCG being run
+  int32_t* maskIds = nullptr;
+  nbx::ReduceWork red;
+  nbx::PcgScalars* S = nullptr;
+  nbx::PcgScalars* hS = nullptr;  // pinned host copy
+  double *hist_rdotr = nullptr, *hist_alpha = nullptr;
+  int hist_cap = 0;
+  int last_iters = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // learned preconditioner (attached network, features, per-element output)
+  nbx_net* net = nullptr;
+  float* feat = nullptr;    // normalised geometric factors [E][7][Np]
+  double* zl = nullptr;     // element-local preconditioner output
+  int stall_window = 20;    // iterations compared for the stagnation test
+  double stall_ratio = 0.9; // rdotr(it) > ratio * rdotr(it - window) -> Jacobi from then on
+  std::vector<void*> owned;
+
+  int alloc(double** ptr, int64_t n)
+  {
+    NBX_HIP(hipMalloc((void**)ptr, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+    NBX_HIP(hipMemset(*ptr, 0, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+    owned.push_back(*ptr);
+    return NBX_SUCCESS;
+  }
+};
+
+
+// FP64 PCG on the handle's operator with an explicit preconditioner choice
+// (used to generate training data); x is updated in place, r becomes the
+// residual.  tol <= 0 runs exactly maxit iterations.
+int bp_solve_plain(nbx_bp* bp, nbx_precon precon, double* r, double* x, double tol, int maxit, int* iterations);
+// seeded right-hand side of the handle's problem (U[-1,1) per local node,
+// stream 3 + field, masked, assembled) into out (device, Nall)
+int bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out);
+
+namespace nbx {
+// element-local learned preconditioner output w invDiag (r + s f(r)) of the
+// attached network into out (Jacobi when *fallback)
+int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallback, const int* skip);
+// forget a handle that is being freed (the network keeps a list of the
+// handles it is attached to and clears their pointer when it is freed first)
+void net_detach(nbx_net* net, nbx_bp* bp);
+}  // namespace nbx

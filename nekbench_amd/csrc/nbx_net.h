@@ -1,0 +1,26 @@
+// Learned element preconditioner: parameter layout and launchers (nbx_net.hip).
+#pragma once
+#include "nbx_internal.h"
+
+namespace nbx {
+
+// parameters (float, bf16-representable in the forward copy), in this order:
+// Wr[27][32] residual taps, Wg[7][32] geometry, b1[32], W2[32][32] (in x out),
+// b2[32], W3[27][32] output taps, b3
+constexpr int kOffWr = 0, kOffWg = 864, kOffB1 = 1088, kOffW2 = 1120, kOffB2 = 2144, kOffW3 = 2176, kOffB3 = 3040;
+constexpr int kNetParams = 3041;
+constexpr int kNetParamsPadded = 3042;  // loss (double) follows, 8-byte aligned
+constexpr int kNetPartial = 3044;       // floats per workgroup partial
+
+int net_apply_launch(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
+                     const double* w, double* zl, const int* fallback, const int* skip, hipStream_t s);
+size_t net_train_lds_bytes();
+int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64_t pairStride, const float* R,
+                     const float* Ev, const double* invNorm, const float* W, const float* feat, const double* invDiag,
+                     float* partials, hipStream_t s);
+int net_grad_sum_launch(int32_t nblocks, const float* partials, float* grad, double* loss, hipStream_t s);
+int net_adam_launch(float* master, float* m, float* v, const float* grad, float* wbf, float lr, int step,
+                    hipStream_t s);
+int net_features_launch(int32_t E, const double* ggeo, const double* invScale, float* feat, hipStream_t s);
+
+}  // namespace nbx

@@ -116,6 +116,7 @@ typedef struct {
 typedef struct {
   int iterations, converged, breakdown, iter;
   double rdotr0, rdotr, TOL;
+  int fallback_iter;
 } nbo_pcg_stats;
 
 /* Aq = mask(gs(axhelm(q))) per field (BPAxOperator, solve.cpp:181-235 without the dot) */
@@ -135,6 +136,18 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
  * st->iter holds the number of refinement cycles. */
 int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, double* r, double* x, double tol,
                      int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st);
+
+/* ---- learned element preconditioner (nbo_net.c) -------------------------- */
+/* zl = w invDiag (r + s f(r)) per element (w NULL: weight 1) */
+void nbo_net_apply(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
+                   const double* w, double* zl);
+void nbo_net_features(int32_t E, int Np, const double* ggeo, float* feat);
+double nbo_sumsq_inv_f(int64_t n, const float* a);
+void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, int64_t pairStride,
+                      const float* R, const float* Ev, const double* invNorm, const float* W, const float* feat,
+                      const double* invDiag, float* grad, double* loss);
+int nbo_bp_pcg_learned(const nbo_bp* bp, const float* W, const float* feat, int window, double ratio, double* r,
+                       double* x, double tol, int maxit, double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st);
 
 #ifdef __cplusplus
 }

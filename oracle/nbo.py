@@ -203,7 +203,7 @@ class BPStruct(C.Structure):
 
 class PCGStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("breakdown", C.c_int), ("iter", C.c_int),
-                ("rdotr0", C.c_double), ("rdotr", C.c_double), ("TOL", C.c_double)]
+                ("rdotr0", C.c_double), ("rdotr", C.c_double), ("TOL", C.c_double), ("fallback_iter", C.c_int)]
 
 
 class OracleBP:
@@ -265,8 +265,57 @@ class OracleBP:
                                C.c_double(tol), maxit, C.c_double(inner_tol), max_cycles, C.byref(st))
         return st
 
+    def invdiag(self):
+        d = jacobi_local(self.N, self.E, self.offset, self.mesh["ggeo"], self.mesh["D"], self.lam)
+        gs_many(self.ogs, d, 1, self.offset)
+        return 1.0 / d
+
+    def pcg_learned(self, r, x, tol, maxit, W, feat, window=20, ratio=0.9):
+        hr = np.zeros(maxit + 2)
+        ha = np.zeros(maxit + 2)
+        st = PCGStats()
+        lib().nbo_bp_pcg_learned(C.byref(self.struct), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat), window,
+                                 C.c_double(ratio), p(r), p(x), C.c_double(tol), maxit, p(hr), p(ha), C.byref(st))
+        return st, hr, ha
+
     def __del__(self):
         try:
             ogs_free(self.ogs)
         except Exception:
             pass
+
+
+def sumsq_inv_f(a):
+    f = lib().nbo_sumsq_inv_f
+    f.restype = C.c_double
+    return f(C.c_int64(a.size), p(np.ascontiguousarray(a, dtype=np.float32)))
+
+
+def net_features(E, Np, ggeo):
+    feat = np.empty(E * 7 * Np, dtype=np.float32)
+    lib().nbo_net_features(C.c_int32(E), Np, p(ggeo), p(feat))
+    return feat
+
+
+def net_apply(E, W, feat, r, invDiag, w=None):
+    zl = np.zeros(E * 512)
+    lib().nbo_net_apply(C.c_int32(E), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat), p(r), p(invDiag), p(w),
+                        p(zl))
+    return zl
+
+
+def net_gradient(samples, chunk, pairStride, R, Ev, invNorm, W, feat, invDiag):
+    s = np.ascontiguousarray(samples, dtype=np.int32).reshape(-1, 2)
+    g = np.zeros(3041, dtype=np.float32)
+    loss = C.c_double(0.0)
+    lib().nbo_net_gradient(C.c_int32(s.shape[0]), C.c_int32(chunk), p(s), C.c_int64(pairStride), p(R), p(Ev),
+                           p(invNorm), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat), p(invDiag), p(g),
+                           C.byref(loss))
+    return g, loss.value
+
+
+def bf16_round(x):
+    """float32 -> nearest-even bf16 value (as float32)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32)

@@ -150,10 +150,19 @@ static void local_jacobi(const nbo_bp* bp, double* invDiag)
   for (int64_t n = 0; n < N; ++n) invDiag[n] = 1.0 / invDiag[n];
 }
 
-/* PCG core; lo == NULL: FP64 operator.  initial: r -= A x first. */
-static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const lo_ops* lo, const double* invDiag,
-                    double* r, double* x, double tol, double abs_floor, int maxit, int initial,
-                    double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
+typedef struct {
+  const float* W;      /* network weights (bf16 values) */
+  const float* feat;   /* normalised geometric factors [E][7][512] */
+  int window;          /* stall test: rdotr(it) > ratio * rdotr(it - window) -> Jacobi */
+  double ratio;
+} learned_ctx;
+
+/* PCG core; lo == NULL: FP64 operator.  initial: r -= A x first.
+ * ln != NULL: learned preconditioner (flexible beta, stall fallback). */
+static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, const lo_ops* lo,
+                        const double* invDiag, const learned_ctx* ln, double* r, double* x, double tol,
+                        double abs_floor, int maxit, int initial, double* hist_rdotr, double* hist_alpha,
+                        nbo_pcg_stats* st)
 {
   const int Np = (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
   const int64_t N = (int64_t)bp->E * Np;
@@ -164,7 +173,12 @@ static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const
   double* Ap = (double*)calloc(Nall, sizeof(double));
   double* Ax = (double*)calloc(Nall, sizeof(double));
   double rdotz1 = 1.0, rdotz2 = 0.0, alpha = 0.0, beta = 0.0, TOL = 0.0, rdotr = 0.0, rdotr0 = 0.0;
-  int converged = 0, breakdown = 0, iterations = maxit, it;
+  int converged = 0, breakdown = 0, iterations = maxit, it, fallback = 0, fallback_iter = 0;
+  double* hist_own = NULL;
+  if (ln) {
+    flexible = 1;
+    if (!hist_rdotr) hist_rdotr = hist_own = (double*)calloc((size_t)maxit + 2, sizeof(double));
+  }
 
   if (initial) {  /* Ax = A x; r = r - Ax (solve.cpp:68-69) */
     nbo_bp_apply(bp, x, Ax);
@@ -175,7 +189,25 @@ static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const
     /* z = M r; (r.z)_w, (r.r)_w [, (z.Ap)_w]  (solve.cpp:77-83, 94) */
     double d[3];
     const double* zsrc = jacobi ? z : r;
-    TILE_TREE(3, N, d, {
+    if (ln) {  /* z = gs(w invDiag (r + s f(r))), Jacobi after a stall */
+      if (fallback) {
+        for (int64_t n = 0; n < N; ++n) z[n] = bp->w[n] * (invDiag[n] * r[n]);
+      } else {
+        nbo_net_apply(bp->E, ln->W, ln->feat, r, invDiag, bp->w, z);
+      }
+      nbo_gs_many(bp->ogs, z, bp->Nf, bp->offset, NBO_DOUBLE, NBO_ADD);
+      for (int32_t m = 0; m < bp->Nmasked; ++m) z[bp->maskIds[m]] = 0.0;
+      zsrc = z;
+    }
+    if (ln) TILE_TREE(3, N, d, {
+      const double wid = bp->w[id];
+      const double rf = r[id];
+      const double zf = z[id];
+      acc[0] = fma(wid * rf, zf, acc[0]);
+      acc[1] = fma(wid * rf, rf, acc[1]);
+      acc[2] = fma(wid * zf, Ap[id], acc[2]);
+    });
+    else TILE_TREE(3, N, d, {
       const double wid = bp->w[id];
       for (int f = 0; f < bp->Nf; ++f) {
         const int64_t iid = id + (int64_t)f * bp->offset;
@@ -194,13 +226,17 @@ static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const
     if (it == 1) { rdotr0 = rdotr; TOL = fmax(tol * tol * rdotr, abs_floor); }  /* solve.cpp:85 */
     if (!isfinite(rdotr)) { breakdown = 1; iterations = it - 1; break; }
     if (rdotr <= TOL && !fixed) { converged = 1; iterations = it - 1; break; }  /* :90 */
+    if (ln && !fallback && it > ln->window && !(rdotr <= ln->ratio * hist_rdotr[it - ln->window])) {
+      fallback = 1;
+      fallback_iter = it;
+    }
     if (flexible) beta = -alpha * d[2] / rdotz2;                                    /* :92-95 */
     else beta = (it == 1) ? 0.0 : rdotz1 / rdotz2;                                 /* :97 */
     /* p = z + beta p (:100) */
     for (int64_t n = 0; n < Nall; ++n) p[n] = fma(beta, p[n], zsrc[n]);
     /* Ap = A p; pAp = (p.Ap)_w; alpha = rdotz/pAp (:102-104) */
     double pAp;
-    if (fused || lo) {  /* pAp from the local operator output, then gs + mask */
+    if (fused || lo || ln) {  /* pAp from the local operator output, then gs + mask */
       if (lo) lo_axhelm(bp, lo, p, Ap);
       else nbo_axhelm(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, p, Ap);
       pAp = fused_pap(bp, p, Ap);
@@ -223,9 +259,18 @@ static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const
   if (st) {
     st->iterations = iterations; st->converged = converged; st->breakdown = breakdown; st->iter = it;
     st->rdotr0 = rdotr0; st->rdotr = rdotr; st->TOL = TOL;
+    st->fallback_iter = fallback ? fallback_iter : 0;
   }
-  free(p); free(z); free(Ap); free(Ax);
+  free(p); free(z); free(Ap); free(Ax); free(hist_own);
   return iterations;
+}
+
+static int pcg_core(const nbo_bp* bp, int jacobi, int flexible, int fused, const lo_ops* lo, const double* invDiag,
+                    double* r, double* x, double tol, double abs_floor, int maxit, int initial,
+                    double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
+{
+  return pcg_core_ext(bp, jacobi, flexible, fused, lo, invDiag, NULL, r, x, tol, abs_floor, maxit, initial,
+                      hist_rdotr, hist_alpha, st);
 }
 
 int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r, double* x, double tol, int maxit,
@@ -287,4 +332,18 @@ int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, doubl
   free(b); free(Ax); free(d); free(invDiag);
   if (prec != 0) lo_free(&lo);
   return total;
+}
+
+/* BPPCG with the learned preconditioner (library: nbx_bp_pcg, precon LEARNED) */
+int nbo_bp_pcg_learned(const nbo_bp* bp, const float* W, const float* feat, int window, double ratio, double* r,
+                       double* x, double tol, int maxit, double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st)
+{
+  const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  double* invDiag = (double*)malloc(sizeof(double) * N);
+  local_jacobi(bp, invDiag);
+  const learned_ctx ln = {W, feat, window, ratio};
+  const int its = pcg_core_ext(bp, 1, 1, 1, NULL, invDiag, &ln, r, x, tol, tol * tol, maxit, 1, hist_rdotr,
+                               hist_alpha, st);
+  free(invDiag);
+  return its;
 }

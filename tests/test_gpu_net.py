@@ -1,0 +1,209 @@
+"""Learned element preconditioner on the GPU against the oracle
+(oracle/nbo_net.c): forward pass, loss and parameter gradient bit for bit,
+the learned flexible PCG iterate for iterate (including the fall-back to
+Jacobi after a stall), checkpoint round trip, and training on pairs from the
+FP64 PCG lowering the loss; the trained network still solves to the FP64
+solution."""
+import numpy as np
+import pytest
+
+from test_oracle_net import random_net
+
+pytestmark = pytest.mark.gpu
+
+
+def H(t):
+    return t.cpu().numpy()
+
+
+def make(nbx, nbo, dims=(3, 3, 2), seed=21, precon=None):
+    kw = dict(perturb=0.1, seed=seed)
+    bp = nbx.BP(nbx.Box.make(7, *dims, **kw), bpmode=True,
+                precon=nbx.PRECON_LEARNED if precon is None else precon)
+    ob = nbo.OracleBP(nbo.Box.make(7, *dims, **kw), bpmode=True)
+    return bp, ob
+
+
+def net_with(nbx, nbo, bp, W):
+    net = nbx.Net(seed=3)
+    net.set_params(W)
+    net.attach(bp)
+    bp.update_jacobi()
+    return net
+
+
+def test_apply_matches_oracle(nbx, nbo, dev):
+    import torch
+    bp, ob = make(nbx, nbo)
+    rng = np.random.default_rng(1)
+    W = nbo.bf16_round(random_net(rng))
+    net = net_with(nbx, nbo, bp, W)
+    n = bp.fieldOffset
+    invDiag = ob.invdiag()
+    assert np.array_equal(H(bp.device_array("invDiagA", n)), invDiag)
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    r = torch.tensor(rng.standard_normal(n) * 1e-3, device=dev)
+    zl = torch.zeros_like(r)
+    for weighted in (True, False):
+        net.apply(bp, r, zl, weighted)
+        exp = nbo.net_apply(ob.E, W, feat, H(r), invDiag, ob.invd if weighted else None)
+        assert np.array_equal(H(zl), exp), weighted
+
+
+def test_params_are_rounded_to_bf16_for_the_forward(nbx, nbo, dev):
+    import torch
+    bp, ob = make(nbx, nbo, dims=(2, 2, 1))
+    W = random_net(np.random.default_rng(2))  # not bf16-representable
+    net = net_with(nbx, nbo, bp, W)
+    assert np.array_equal(net.params(), W)  # FP32 master copy kept
+    r = torch.tensor(np.random.default_rng(3).standard_normal(bp.fieldOffset), device=dev)
+    zl = torch.zeros_like(r)
+    net.apply(bp, r, zl, True)
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    exp = nbo.net_apply(ob.E, nbo.bf16_round(W), feat, H(r), ob.invdiag(), ob.invd)
+    assert np.array_equal(H(zl), exp)
+
+
+@pytest.mark.parametrize("chunk", [16, 5, 1])
+def test_gradient_matches_oracle(nbx, nbo, dev, chunk):
+    import torch
+    bp, ob = make(nbx, nbo, dims=(2, 2, 2))
+    rng = np.random.default_rng(7)
+    W = nbo.bf16_round(random_net(rng))
+    net = net_with(nbx, nbo, bp, W)
+    n, npairs = bp.fieldOffset, 3
+    R = (rng.standard_normal((npairs, n)) * 1e-2).astype(np.float32)
+    Ev = (rng.standard_normal((npairs, n)) * 1e-1).astype(np.float32)
+    net.set_pairs(bp, torch.tensor(R, device=dev), torch.tensor(Ev, device=dev))
+    invNorm = np.array([nbo.sumsq_inv_f(Ev[k]) for k in range(npairs)])
+    samples = np.stack([rng.integers(0, npairs, 37), rng.integers(0, ob.E, 37)], axis=1).astype(np.int32)
+    g, loss = net.gradient(bp, samples, chunk)
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    og, oloss = nbo.net_gradient(samples, chunk, n, R.ravel(), Ev.ravel(), invNorm, W, feat, ob.invdiag())
+    assert loss == oloss
+    assert np.array_equal(g, og)
+
+
+@pytest.mark.parametrize("scale3", [0.05, 0.5])
+def test_learned_pcg_matches_oracle(nbx, nbo, dev, scale3):
+    import torch
+    bp, ob = make(nbx, nbo)
+    W = nbo.bf16_round(random_net(np.random.default_rng(4), scale3=scale3))
+    net = net_with(nbx, nbo, bp, W)
+    n = bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-8, 600)
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    xr = np.zeros(n)
+    ost, _, _ = ob.pcg_learned(ob.rhs.copy(), xr, 1e-8, 600, W, feat)
+    assert st.iterations == ost.iterations and st.converged == ost.converged == 1
+    assert st.fallback_iteration == ost.fallback_iter
+    assert st.rdotr == ost.rdotr
+    assert np.array_equal(H(x), xr)
+
+
+def test_learned_pcg_stall_fallback_matches_oracle(nbx, nbo, dev):
+    import torch
+    bp, ob = make(nbx, nbo)
+    W = nbo.bf16_round(random_net(np.random.default_rng(4), scale3=40.0))
+    net = net_with(nbx, nbo, bp, W)
+    bp.set_fallback(5, 0.999)
+    n = bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-8, 3000)
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    xr = np.zeros(n)
+    ost, _, _ = ob.pcg_learned(ob.rhs.copy(), xr, 1e-8, 3000, W, feat, window=5, ratio=0.999)
+    assert ost.fallback_iter > 5 and st.fallback_iteration == ost.fallback_iter
+    assert st.iterations == ost.iterations and st.converged == 1
+    assert np.array_equal(H(x), xr)
+
+
+def test_zero_network_is_flexible_jacobi(nbx, nbo, dev):
+    import torch
+    bp, _ = make(nbx, nbo)
+    net = nbx.Net(seed=5)  # output layer starts at zero
+    net.attach(bp)
+    n = bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-8, 600)
+    bj2 = nbx.BP(nbx.Box.make(7, 3, 3, 2, perturb=0.1, seed=21), bpmode=True, precon=nbx.PRECON_JACOBI,
+                 flexible=True, fused=True)
+    r2 = bj2.device_array("rhs", n).clone()
+    x2 = torch.zeros_like(r2)
+    st2 = bj2.pcg(r2, x2, 1e-8, 600)
+    assert st.iterations == st2.iterations
+    assert (x - x2).abs().max().item() <= 1e-12 * x2.abs().max().item()
+
+
+def test_checkpoint_round_trip(nbx, dev, tmp_path):
+    net = nbx.Net(seed=9)
+    W = random_net(np.random.default_rng(6))
+    net.set_params(W)
+    path = tmp_path / "net.nbx"
+    net.save(path)
+    back = nbx.Net.load(path)
+    assert np.array_equal(back.params(), W)
+    raw = bytearray(path.read_bytes())
+    raw[100] ^= 0x10
+    bad = tmp_path / "bad.nbx"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(nbx.NbxError):
+        nbx.Net.load(bad)
+    with pytest.raises(nbx.NbxError):
+        nbx.Net.load(tmp_path / "missing.nbx")
+
+
+def test_learned_requires_n7_single_field(nbx, dev):
+    net = nbx.Net(seed=1)
+    bp5 = nbx.BP(nbx.Box.make(5, 2, 2, 2), bpmode=True, precon=nbx.PRECON_LEARNED)
+    with pytest.raises(nbx.NbxError):
+        net.attach(bp5)
+    bp3 = nbx.BP(nbx.Box.make(7, 2, 2, 2), Nfields=3, precon=nbx.PRECON_LEARNED)
+    with pytest.raises(nbx.NbxError):
+        net.attach(bp3)
+
+
+def test_learned_without_network_fails(nbx, dev):
+    import torch
+    bp = nbx.BP(nbx.Box.make(7, 2, 2, 2), bpmode=True, precon=nbx.PRECON_LEARNED)
+    r = bp.device_array("rhs", bp.fieldOffset).clone()
+    with pytest.raises(nbx.NbxError):
+        bp.pcg(r, torch.zeros_like(r), 1e-8, 10)
+    # a network freed while attached detaches itself: the solve reports an error
+    net = nbx.Net(seed=2)
+    net.attach(bp)
+    bp._net = None
+    net.free()
+    with pytest.raises(nbx.NbxError, match="network"):
+        bp.pcg(r, torch.zeros_like(r), 1e-8, 10)
+    zl = torch.zeros_like(r)
+    with pytest.raises(nbx.NbxError):
+        nbx.Net(seed=3).apply(bp, r, zl)
+
+
+def test_training_lowers_loss(nbx, dev):
+    import torch
+    box = nbx.Box.make(7, 4, 4, 4, perturb=0.1, seed=5)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
+    net = nbx.Net(seed=1)
+    net.attach(bp)
+    k = net.generate_pairs(bp, nseeds=6, seed0=1000, snapshots=[0, 5, 15, 40], tol_exact=1e-10)
+    assert k == 24
+    hist = net.train(bp, steps=300, batch=1024, lr=2e-3, seed=3)
+    print("loss", hist[:3], hist[-3:])
+    assert hist[-20:].mean() < 0.8 * hist[:3].mean()
+    n = bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-8, 2000)
+    bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, fused=True)
+    rj = bj.device_array("rhs", n).clone()
+    xj = torch.zeros_like(rj)
+    sj = bj.pcg(rj, xj, 1e-8, 2000)
+    print("iterations learned", st.iterations, "jacobi", sj.iterations, "fallback", st.fallback_iteration)
+    assert st.converged and sj.converged
+    assert (x - xj).abs().max().item() <= 1e-6 * xj.abs().max().item()
