@@ -241,10 +241,17 @@ int nbx_net_generate_pairs(nbx_net* net, nbx_bp* bp, int nseeds, uint64_t seed0,
                            double tol_exact);
 int nbx_net_set_pairs(nbx_net* net, nbx_bp* bp, int npairs, const float* R, const float* E);
 /* loss sum_samples sum_n (z - e)^2 / sum_pair e^2 and its parameter gradient
- * over (pair, element) samples (host int pairs), chunk samples per workgroup */
-int nbx_net_gradient(nbx_net* net, nbx_bp* bp, const int32_t* samples, int nsamples, int chunk, float* grad,
-                     double* loss);
+ * over (pair, element) samples (host int pairs), chunk samples per workgroup.
+ * G (device, local vector, may be NULL): when given, the loss is external and
+ * G[e*Np+n] is its derivative with respect to the unweighted element output
+ * invDiag (r + s f(r)); the returned loss is then 0 */
+int nbx_net_gradient(nbx_net* net, nbx_bp* bp, const int32_t* samples, int nsamples, int chunk, const double* G,
+                     float* grad, double* loss);
+/* Adam on the L2 error loss above, batch random (pair, element) samples a step */
 int nbx_net_train(nbx_net* net, nbx_bp* bp, int steps, int batch, float lr, uint64_t seed, double* loss_hist);
+/* Adam on the energy-norm loss: one random pair a step, all its elements,
+ * L = d^T A d / e^T A e with d = mask(gs(w invDiag (r + s f(r)))) - e */
+int nbx_net_train_energy(nbx_net* net, nbx_bp* bp, int steps, float lr, uint64_t seed, double* loss_hist);
 
 /* per-iteration history of the last solve (host outputs, length >= iterations+1):
  * rdotr and alpha per iteration; returns the count written */

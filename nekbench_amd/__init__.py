@@ -372,8 +372,9 @@ _SIGS.update({
     "nbx_net_apply": (_I, [_P, _P, _P, _I]),
     "nbx_net_generate_pairs": (_I, [_P, _P, _I, C.c_uint64, _P, _I, _D]),
     "nbx_net_set_pairs": (_I, [_P, _P, _I, _P, _P]),
-    "nbx_net_gradient": (_I, [_P, _P, _P, _I, _I, _P, _P]),
+    "nbx_net_gradient": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "nbx_net_train": (_I, [_P, _P, _I, _I, C.c_float, C.c_uint64, _P]),
+    "nbx_net_train_energy": (_I, [_P, _P, _I, C.c_float, C.c_uint64, _P]),
 })
 
 
@@ -431,17 +432,23 @@ class Net:
         assert R.dim() == 2 and R.shape == E.shape
         check(lib().nbx_net_set_pairs(self.h, bp.h, R.shape[0], ptr(R), ptr(E)), "nbx_net_set_pairs")
 
-    def gradient(self, bp: "BP", samples, chunk=16):
+    def gradient(self, bp: "BP", samples, chunk=16, G=None):
         s = np.ascontiguousarray(samples, dtype=np.int32).reshape(-1, 2)
         g = np.zeros(self.nparams, dtype=np.float32)
         loss = C.c_double(0.0)
-        check(lib().nbx_net_gradient(self.h, bp.h, ptr(s), s.shape[0], chunk, ptr(g), C.byref(loss)),
+        check(lib().nbx_net_gradient(self.h, bp.h, ptr(s), s.shape[0], chunk, ptr(G), ptr(g), C.byref(loss)),
               "nbx_net_gradient")
         return g, loss.value
 
     def train(self, bp: "BP", steps, batch=2048, lr=1e-3, seed=1):
         hist = np.zeros(max(steps, 1))
         check(lib().nbx_net_train(self.h, bp.h, steps, batch, C.c_float(lr), seed, ptr(hist)), "nbx_net_train")
+        return hist[:steps]
+
+    def train_energy(self, bp: "BP", steps, lr=1e-3, seed=1):
+        hist = np.zeros(max(steps, 1))
+        check(lib().nbx_net_train_energy(self.h, bp.h, steps, C.c_float(lr), seed, ptr(hist)),
+              "nbx_net_train_energy")
         return hist[:steps]
 
     def free(self):

@@ -445,7 +445,7 @@ extern "C" const void* nbx_bp_device_array(const nbx_bp* bp, const char* name)
 extern "C" nbx_ogs* nbx_bp_ogs(nbx_bp* bp) { return bp ? bp->ogs : nullptr; }
 
 // Aq = mask(gs(axhelm(q)))  (BPAxOperator without the dot)
-static int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* Aq, const int* skip)
+int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* Aq, const int* skip)
 {
   const nbx_bp_options& o = bp->opt;
   int rc = axhelm_launch(bp->N, bp->Nf, o.bpmode ? 1 : 0, NBX_F64, bp->E, bp->fieldOffset, nullptr, bp->ggeo,

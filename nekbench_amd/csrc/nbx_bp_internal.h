@@ -70,6 +70,8 @@ struct nbx_bp {
 // (used to generate training data); x is updated in place, r becomes the
 // residual.  tol <= 0 runs exactly maxit iterations.
 int bp_solve_plain(nbx_bp* bp, nbx_precon precon, double* r, double* x, double tol, int maxit, int* iterations);
+// Aq = mask(gs(axhelm(q))) with the handle's FP64 operator (no dot)
+int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* Aq, const int* skip);
 // seeded right-hand side of the handle's problem (U[-1,1) per local node,
 // stream 3 + field, masked, assembled) into out (device, Nall)
 int bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out);

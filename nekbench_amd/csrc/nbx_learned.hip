@@ -30,6 +30,11 @@ struct nbx_net {
   int npairs = 0, pair_cap = 0;
   int64_t pairStride = 0;
   std::vector<nbx_bp*> attached;  // handles whose bp->net points here
+  // energy-norm training: three local vectors and 1 / e^T A e per pair
+  double* work = nullptr;
+  int64_t work_n = 0;
+  double* invNormE = nullptr;
+  int energy_pairs = 0;
 };
 
 namespace {
@@ -87,7 +92,8 @@ void net_release(nbx_net* n)
   (void)hipDeviceSynchronize();
   for (nbx_bp* bp : n->attached) bp->net = nullptr;  // LEARNED solves now fail with an error
   for (void* p : {(void*)n->master, (void*)n->wbf, (void*)n->m, (void*)n->v, (void*)n->grad, (void*)n->loss,
-                  (void*)n->partials, (void*)n->samples, (void*)n->R, (void*)n->Ev, (void*)n->invNorm})
+                  (void*)n->partials, (void*)n->samples, (void*)n->R, (void*)n->Ev, (void*)n->invNorm,
+                  (void*)n->work, (void*)n->invNormE})
     if (p) (void)hipFree(p);
   delete n;
 }
@@ -126,6 +132,44 @@ __global__ __launch_bounds__(256) void k_sumsq_f(const int64_t n, const float* _
   }
   nbx::block_sum256<1>(v, s4);
   if (threadIdx.x == 0) *out = 1.0 / v[0];
+}
+
+__global__ __launch_bounds__(256) void k_f2d(const int64_t n, const float* __restrict__ a, double* __restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)a[i];
+}
+
+// z <- z - e (e float)
+__global__ __launch_bounds__(256) void k_sub_f(const int64_t n, double* __restrict__ z, const float* __restrict__ e)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = z[i] - (double)e[i];
+}
+
+// G = 2 wn w Ad in place of Ad (derivative of wn d^T A d per local copy)
+__global__ __launch_bounds__(256) void k_energy_grad(const int64_t n, const double* __restrict__ wn,
+                                                     const double* __restrict__ w, double* __restrict__ Ad)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Ad[i] = 2.0 * wn[0] * (w[i] * Ad[i]);
+}
+
+__global__ void k_scalar_mul(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = a[0] * b[0];
+}
+
+__global__ void k_scalar_inv(const double* __restrict__ a, double* __restrict__ out)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = 1.0 / a[0];
+}
+
+// samples (p, e) for every element of pair p
+__global__ __launch_bounds__(256) void k_pair_samples(const int32_t p, const int32_t E, int2* __restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < E) out[i] = make_int2(p, i);
 }
 
 // training batch: sample i = (mix64(key+c0+2i) mod npairs, mix64(key+c0+2i+1) mod E)
@@ -369,6 +413,7 @@ extern "C" int nbx_net_generate_pairs(nbx_net* n, nbx_bp* bp, int nseeds, uint64
   for (void* p : {(void*)b, (void*)xs, (void*)r, (void*)x}) (void)hipFree(p);
   if (rc) return rc;
   n->npairs = k;
+  n->energy_pairs = 0;
   return k;
 }
 
@@ -385,6 +430,7 @@ extern "C" int nbx_net_set_pairs(nbx_net* n, nbx_bp* bp, int npairs, const float
                        n->Ev + (int64_t)k * n->pairStride, n->invNorm + k);
   NBX_HIP(hipStreamSynchronize(bp->stream));
   n->npairs = npairs;
+  n->energy_pairs = 0;
   return NBX_SUCCESS;
 }
 
@@ -406,12 +452,12 @@ static int ensure_work(nbx_net* n, int nsamples, int chunk)
   return NBX_SUCCESS;
 }
 
-// gradient over the nsamples samples already in n->samples
-static int run_gradient(nbx_net* n, nbx_bp* bp, int nsamples, int chunk)
+// gradient over the nsamples samples already in n->samples (G: energy mode)
+static int run_gradient(nbx_net* n, nbx_bp* bp, int nsamples, int chunk, const double* G = nullptr)
 {
   const int nblocks = (nsamples + chunk - 1) / chunk;
   int rc = net_train_launch(nsamples, chunk, n->samples, n->pairStride, n->R, n->Ev, n->invNorm, n->wbf, bp->feat,
-                            bp->invDiagA, n->partials, bp->stream);
+                            bp->invDiagA, G, n->partials, bp->stream);
   if (rc) return rc;
   return net_grad_sum_launch(nblocks, n->partials, n->grad, n->loss, bp->stream);
 }
@@ -419,7 +465,7 @@ static int run_gradient(nbx_net* n, nbx_bp* bp, int nsamples, int chunk)
 // Gradient of the training loss over the given (pair, element) samples, in
 // the library's fixed order (chunk samples per workgroup); host outputs.
 extern "C" int nbx_net_gradient(nbx_net* n, nbx_bp* bp, const int32_t* samples, int nsamples, int chunk,
-                                float* grad_host, double* loss_host)
+                                const double* G, float* grad_host, double* loss_host)
 {
   if (!n || !bp || !samples || nsamples < 1 || chunk < 1 || !bp->feat || n->npairs < 1)
     return fail(NBX_ERR_INVALID, "net_gradient: bad arguments (network attached, pairs set?)");
@@ -429,7 +475,7 @@ extern "C" int nbx_net_gradient(nbx_net* n, nbx_bp* bp, const int32_t* samples, 
   int rc = ensure_work(n, nsamples, chunk);
   if (rc) return rc;
   NBX_HIP(hipMemcpy(n->samples, samples, sizeof(int2) * nsamples, hipMemcpyHostToDevice));
-  if ((rc = run_gradient(n, bp, nsamples, chunk))) return rc;
+  if ((rc = run_gradient(n, bp, nsamples, chunk, G))) return rc;
   if (grad_host) NBX_HIP(hipMemcpyAsync(grad_host, n->grad, sizeof(float) * kNetParams, hipMemcpyDeviceToHost, bp->stream));
   if (loss_host) NBX_HIP(hipMemcpyAsync(loss_host, n->loss, sizeof(double), hipMemcpyDeviceToHost, bp->stream));
   NBX_HIP(hipStreamSynchronize(bp->stream));
@@ -461,6 +507,81 @@ extern "C" int nbx_net_train(nbx_net* n, nbx_bp* bp, int steps, int batch, float
       NBX_HIP(hipMemcpyAsync(&l, n->loss, sizeof(double), hipMemcpyDeviceToHost, bp->stream));
       NBX_HIP(hipStreamSynchronize(bp->stream));
       loss_hist[st] = l / batch;
+    }
+  }
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  return NBX_SUCCESS;
+}
+
+// Energy-norm training.  One pair per step, all its elements in one batch:
+//   z = mask(gs(w invDiag (r + s f(r)))),  d = z - e,
+//   L = d^T A d / e^T A e   (weighted dots over local nodes, FP64 operator),
+// i.e. the A-norm of the error left by one preconditioned Richardson step,
+// relative to the error before it.  dL/dz_local = 2 w A d / e^T A e is
+// back-propagated through every element.  loss_hist (host, may be null)
+// receives each step's L.
+extern "C" int nbx_net_train_energy(nbx_net* n, nbx_bp* bp, int steps, float lr, uint64_t seed, double* loss_hist)
+{
+  if (!n || !bp || steps < 0 || !bp->feat || n->npairs < 1 || !(lr > 0) || bp->Nf != 1 ||
+      n->pairStride != bp->Nlocal)
+    return fail(NBX_ERR_INVALID, "net_train_energy: bad arguments (network attached, pairs of this mesh?)");
+  const int64_t Nall = bp->Nall, Nl = bp->Nlocal;
+  if (n->work_n < Nall) {
+    if (n->work) (void)hipFree(n->work);
+    n->work = nullptr;
+    NBX_HIP(hipMalloc(&n->work, sizeof(double) * 3 * Nall));
+    n->work_n = Nall;
+  }
+  double* rd = n->work;
+  double* zb = n->work + Nall;
+  double* Ad = n->work + 2 * Nall;
+  NBX_HIP(hipMemsetAsync(n->work, 0, sizeof(double) * 3 * Nall, bp->stream));
+  const double* w = nbx_ogs_inv_degree(bp->ogs);
+  const dim3 g((unsigned)((Nl + 255) / 256)), b(256);
+  if (n->energy_pairs != n->npairs) {  // 1 / e^T A e for every pair
+    if (n->invNormE) (void)hipFree(n->invNormE);
+    n->invNormE = nullptr;
+    NBX_HIP(hipMalloc(&n->invNormE, sizeof(double) * n->npairs));
+    for (int p = 0; p < n->npairs; ++p) {
+      hipLaunchKernelGGL(k_f2d, g, b, 0, bp->stream, Nl, n->Ev + (int64_t)p * n->pairStride, zb);
+      int rc = bp_apply(bp, bp->lambda, zb, Ad, nullptr);
+      if (rc) return rc;
+      rc = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, w, zb, Ad, false, bp->red, bp->red.result, bp->stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_scalar_inv, dim3(1), dim3(64), 0, bp->stream, bp->red.result, n->invNormE + p);
+    }
+    NBX_HIP_LAUNCH();
+    n->energy_pairs = n->npairs;
+  }
+  const int chunk = 16;
+  int rc = ensure_work(n, bp->E, chunk);
+  if (rc) return rc;
+  const uint64_t key = mix64(seed ^ mix64(93));
+  for (int st = 0; st < steps; ++st) {
+    const int p = (int)(mix64(key + (uint64_t)st) % (uint64_t)n->npairs);
+    const float* Rp = n->R + (int64_t)p * n->pairStride;
+    const float* Ep = n->Ev + (int64_t)p * n->pairStride;
+    hipLaunchKernelGGL(k_f2d, g, b, 0, bp->stream, Nl, Rp, rd);
+    if ((rc = net_apply_launch(bp->E, n->wbf, bp->feat, rd, bp->invDiagA, w, zb, nullptr, nullptr, bp->stream)))
+      return rc;
+    if ((rc = gs_add_mask_launch(bp->ogs, zb, 1, bp->fieldOffset, bp->Nmasked, bp->maskIds, nullptr, bp->stream)))
+      return rc;
+    hipLaunchKernelGGL(k_sub_f, g, b, 0, bp->stream, Nl, zb, Ep);
+    if ((rc = bp_apply(bp, bp->lambda, zb, Ad, nullptr))) return rc;
+    if ((rc = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, w, zb, Ad, false, bp->red, bp->red.result, bp->stream)))
+      return rc;
+    hipLaunchKernelGGL(k_energy_grad, g, b, 0, bp->stream, Nl, n->invNormE + p, w, Ad);
+    hipLaunchKernelGGL(k_pair_samples, dim3((bp->E + 255) / 256), b, 0, bp->stream, p, bp->E, n->samples);
+    NBX_HIP_LAUNCH();
+    if ((rc = run_gradient(n, bp, bp->E, chunk, Ad))) return rc;
+    hipLaunchKernelGGL(k_scalar_mul, dim3(1), dim3(64), 0, bp->stream, bp->red.result, n->invNormE + p, n->loss);
+    n->steps += 1;
+    if ((rc = net_adam_launch(n->master, n->m, n->v, n->grad, n->wbf, lr, n->steps, bp->stream))) return rc;
+    if (loss_hist) {
+      double l;
+      NBX_HIP(hipMemcpyAsync(&l, n->loss, sizeof(double), hipMemcpyDeviceToHost, bp->stream));
+      NBX_HIP(hipStreamSynchronize(bp->stream));
+      loss_hist[st] = l;
     }
   }
   NBX_HIP(hipStreamSynchronize(bp->stream));

@@ -17,7 +17,7 @@ int net_apply_launch(int32_t E, const float* W, const float* feat, const double*
 size_t net_train_lds_bytes();
 int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64_t pairStride, const float* R,
                      const float* Ev, const double* invNorm, const float* W, const float* feat, const double* invDiag,
-                     float* partials, hipStream_t s);
+                     const double* G, float* partials, hipStream_t s);
 int net_grad_sum_launch(int32_t nblocks, const float* partials, float* grad, double* loss, hipStream_t s);
 int net_adam_launch(float* master, float* m, float* v, const float* grad, float* wbf, float lr, int step,
                     hipStream_t s);

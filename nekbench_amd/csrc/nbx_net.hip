@@ -30,6 +30,10 @@
 namespace nbx {
 
 constexpr int kP = 512, kC = 32, kG = 7, kTaps = 27;
+// row stride of the [channel][node] LDS tiles of the training kernel: odd, so
+// the 32 lanes of a parameter gradient that read 32 channel rows at the same
+// node hit 32 different banks
+constexpr int kRow = kP + 1;
 
 __device__ __forceinline__ int tap_neighbor(int n, int tap)
 {
@@ -152,17 +156,21 @@ __global__ __launch_bounds__(512) void k_net_apply(const int32_t E, const float*
 //   L = sum_samples inv_norm[pair] sum_n (z_n - e_n)^2
 // in lane-owned registers (parameter q is owned by lane q mod 512 of its
 // tensor); one partial per workgroup goes to partials[block][kNetParams+1].
+// With G non-null the loss is defined outside (energy norm of the assembled
+// error) and G[e*512+n] holds its derivative with respect to the unweighted
+// element output invDiag (r + s y) at local node n of element e; the kernel
+// then back-propagates dL/dy = G invDiag s and reports no loss.
 __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const int32_t chunk,
                                                    const int2* __restrict__ samples, const int64_t pairStride,
                                                    const float* __restrict__ R, const float* __restrict__ Ev,
                                                    const double* __restrict__ invNorm, const float* __restrict__ W,
                                                    const float* __restrict__ feat, const double* __restrict__ invDiag,
-                                                   float* __restrict__ partials)
+                                                   const double* __restrict__ G, float* __restrict__ partials)
 {
   extern __shared__ float lds[];
-  float* R1 = lds;                    // [32][512]: h2 -> h1 -> dpre1
-  float* R2 = R1 + kC * kP;           // [32][512]: T / dT -> dpre2
-  float* s_rh = R2 + kC * kP;         // [512]
+  float* R1 = lds;                    // [32][kRow]: h2 -> h1 -> dpre1
+  float* R2 = R1 + kC * kRow;         // [32][kRow]: T / dT -> dpre2
+  float* s_rh = R2 + kC * kRow;       // [512]
   float* s_g = s_rh + kP;             // [7][512]
   float* s_dy = s_g + kG * kP;        // [512]
   double* s_red = (double*)(s_dy + kP);  // [8]
@@ -177,7 +185,7 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
     const int64_t e = pe.y;
     const int64_t off = (int64_t)pe.x * pairStride + e * kP + n;
     const double rn = (double)R[off];
-    const double en = (double)Ev[off];
+    const double en = G ? 0.0 : (double)Ev[off];
     const double di = invDiag[e * kP + n];
     __syncthreads();  // previous sample done with LDS
     const double s = element_scale(rn, s_red);
@@ -194,19 +202,19 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
     float T[kTaps];
     net_forward_lane(W, n, s_rh, g, a, T);
 #pragma unroll
-    for (int tap = 0; tap < kTaps; ++tap) R2[tap * kP + n] = T[tap];
+    for (int tap = 0; tap < kTaps; ++tap) R2[tap * kRow + n] = T[tap];
     __syncthreads();
     float y = W[kOffB3];
 #pragma unroll
     for (int tap = 0; tap < kTaps; ++tap) {
       const int nb = tap_neighbor(n, tap);
-      if (nb >= 0) y = y + R2[tap * kP + nb];
+      if (nb >= 0) y = y + R2[tap * kRow + nb];
     }
     const double z = di * (rn + s * (double)y);
     const double res = z - en;
-    const double wn = invNorm[pe.x];
+    const double wn = G ? 0.0 : invNorm[pe.x];
     // loss: lane values, fixed tree (per-wave halving, waves in order)
-    {
+    if (!G) {
       double l = res * res * wn;
 #pragma unroll
       for (int h = 32; h >= 1; h >>= 1) l += __shfl_xor(l, h, 64);
@@ -219,7 +227,7 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
         loss = loss + t;
       }
     }
-    const float dy = (float)(2.0 * res * wn * di * s);
+    const float dy = G ? (float)(G[e * kP + n] * di * s) : (float)(2.0 * res * wn * di * s);
     s_dy[n] = dy;
     __syncthreads();
     // dT[tap] = dy at the output node m = n - tap
@@ -239,9 +247,9 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
     }
     __syncthreads();  // everyone has read T from R2
 #pragma unroll
-    for (int c = 0; c < kC; ++c) R1[c * kP + n] = a.h2[c];
+    for (int c = 0; c < kC; ++c) R1[c * kRow + n] = a.h2[c];
 #pragma unroll
-    for (int tap = 0; tap < kTaps; ++tap) R2[tap * kP + n] = dT[tap];
+    for (int tap = 0; tap < kTaps; ++tap) R2[tap * kRow + n] = dT[tap];
     __syncthreads();
     // dW3[tap][c] = sum_n h2[n][c] dT[n][tap]
 #pragma unroll
@@ -250,7 +258,7 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
       if (q < kTaps * kC) {
         const int tap = q / kC, c = q % kC;
         float acc = 0.f;
-        for (int m = 0; m < kP; ++m) acc = __builtin_fmaf(R1[c * kP + m], R2[tap * kP + m], acc);
+        for (int m = 0; m < kP; ++m) acc = __builtin_fmaf(R1[c * kRow + m], R2[tap * kRow + m], acc);
         gW3[u] = gW3[u] + acc;
       }
     }
@@ -262,8 +270,8 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < kC; ++c) {
-      R1[c * kP + n] = a.h1[c];
-      R2[c * kP + n] = dpre2[c];
+      R1[c * kRow + n] = a.h1[c];
+      R2[c * kRow + n] = dpre2[c];
     }
     __syncthreads();
     // dW2[ci][c] = sum_n h1[n][ci] dpre2[n][c]; db2[c] = sum_n dpre2[n][c]
@@ -272,12 +280,12 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
       const int q = n + u * kP;
       const int ci = q / kC, c = q % kC;
       float acc = 0.f;
-      for (int m = 0; m < kP; ++m) acc = __builtin_fmaf(R1[ci * kP + m], R2[c * kP + m], acc);
+      for (int m = 0; m < kP; ++m) acc = __builtin_fmaf(R1[ci * kRow + m], R2[c * kRow + m], acc);
       gW2[u] = gW2[u] + acc;
     }
     if (n < kC) {
       float acc = 0.f;
-      for (int m = 0; m < kP; ++m) acc = acc + R2[n * kP + m];
+      for (int m = 0; m < kP; ++m) acc = acc + R2[n * kRow + m];
       gb2 = gb2 + acc;
     }
     // dpre1 = relu'(h1) (W2 dpre2)
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
     }
     __syncthreads();  // R1 (h1) reads done
 #pragma unroll
-    for (int c = 0; c < kC; ++c) R1[c * kP + n] = dpre1[c];
+    for (int c = 0; c < kC; ++c) R1[c * kRow + n] = dpre1[c];
     __syncthreads();
     // dWr[tap][c] = sum_n dpre1[n][c] rh[n+tap]; dWg[q][c] = sum_n dpre1[n][c] g[n][q]; db1
 #pragma unroll
@@ -302,7 +310,7 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
         float acc = 0.f;
         for (int m = 0; m < kP; ++m) {
           const int nb = tap_neighbor(m, tap);
-          if (nb >= 0) acc = __builtin_fmaf(R1[c * kP + m], s_rh[nb], acc);
+          if (nb >= 0) acc = __builtin_fmaf(R1[c * kRow + m], s_rh[nb], acc);
         }
         gWr[u] = gWr[u] + acc;
       }
@@ -310,12 +318,12 @@ __global__ __launch_bounds__(512) void k_net_train(const int32_t nsamples, const
     if (n < kG * kC) {
       const int q = n / kC, c = n % kC;
       float acc = 0.f;
-      for (int m = 0; m < kP; ++m) acc = __builtin_fmaf(R1[c * kP + m], s_g[q * kP + m], acc);
+      for (int m = 0; m < kP; ++m) acc = __builtin_fmaf(R1[c * kRow + m], s_g[q * kP + m], acc);
       gWg = gWg + acc;
     }
     if (n < kC) {
       float acc = 0.f;
-      for (int m = 0; m < kP; ++m) acc = acc + R1[n * kP + m];
+      for (int m = 0; m < kP; ++m) acc = acc + R1[n * kRow + m];
       gb1 = gb1 + acc;
     }
   }
@@ -405,12 +413,12 @@ int net_apply_launch(int32_t E, const float* W, const float* feat, const double*
 
 size_t net_train_lds_bytes()
 {
-  return sizeof(float) * (2 * kC * kP + kP + kG * kP + kP) + sizeof(double) * (kP / 64);
+  return sizeof(float) * (2 * kC * kRow + kP + kG * kP + kP) + sizeof(double) * (kP / 64);
 }
 
 int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64_t pairStride, const float* R,
                      const float* Ev, const double* invNorm, const float* W, const float* feat, const double* invDiag,
-                     float* partials, hipStream_t s)
+                     const double* G, float* partials, hipStream_t s)
 {
   const int nblocks = (nsamples + chunk - 1) / chunk;
   const size_t lds = net_train_lds_bytes();
@@ -420,7 +428,7 @@ int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64
     attr_set = true;
   }
   hipLaunchKernelGGL(k_net_train, dim3(nblocks), dim3(kP), lds, s, nsamples, chunk, samples, pairStride, R, Ev,
-                     invNorm, W, feat, invDiag, partials);
+                     invNorm, W, feat, invDiag, G, partials);
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
 }

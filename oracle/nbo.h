@@ -145,7 +145,7 @@ void nbo_net_features(int32_t E, int Np, const double* ggeo, float* feat);
 double nbo_sumsq_inv_f(int64_t n, const float* a);
 void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, int64_t pairStride,
                       const float* R, const float* Ev, const double* invNorm, const float* W, const float* feat,
-                      const double* invDiag, float* grad, double* loss);
+                      const double* invDiag, const double* Gz, float* grad, double* loss);
 int nbo_bp_pcg_learned(const nbo_bp* bp, const float* W, const float* feat, int window, double ratio, double* r,
                        double* x, double tol, int maxit, double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st);
 

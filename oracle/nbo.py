@@ -304,13 +304,15 @@ def net_apply(E, W, feat, r, invDiag, w=None):
     return zl
 
 
-def net_gradient(samples, chunk, pairStride, R, Ev, invNorm, W, feat, invDiag):
+def net_gradient(samples, chunk, pairStride, R, Ev, invNorm, W, feat, invDiag, G=None):
+    """G given: gradient of an external loss whose derivative with respect to
+    the unweighted element output invDiag (r + s y) is G (local vector)."""
     s = np.ascontiguousarray(samples, dtype=np.int32).reshape(-1, 2)
     g = np.zeros(3041, dtype=np.float32)
     loss = C.c_double(0.0)
     lib().nbo_net_gradient(C.c_int32(s.shape[0]), C.c_int32(chunk), p(s), C.c_int64(pairStride), p(R), p(Ev),
-                           p(invNorm), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat), p(invDiag), p(g),
-                           C.byref(loss))
+                           p(invNorm), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat), p(invDiag), p(G),
+                           p(g), C.byref(loss))
     return g, loss.value
 
 

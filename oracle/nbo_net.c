@@ -106,7 +106,7 @@ void nbo_net_apply(int32_t E, const float* W, const float* feat, const double* r
  * in order.  R, Ev: float pairs with stride pairStride; grad[NPAR]. */
 void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, int64_t pairStride,
                       const float* R, const float* Ev, const double* invNorm, const float* W, const float* feat,
-                      const double* invDiag, float* grad, double* loss_out)
+                      const double* invDiag, const double* Gz, float* grad, double* loss_out)
 {
   const int nblocks = (nsamples + chunk - 1) / chunk;
   float* part = (float*)calloc((size_t)nblocks * NPAR, sizeof(float));
@@ -130,25 +130,30 @@ void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, i
         const float* Ep = Ev + (int64_t)p * pairStride + (int64_t)e * P;
         for (int n = 0; n < P; ++n) rr[n] = (double)Rp[n];
         elem_forward(W, rr, feat + (int64_t)e * G * P, a);
-        const double wn = invNorm[p];
+        const double wn = Gz ? 0.0 : invNorm[p];
         double lane[P];
         for (int n = 0; n < P; ++n) {
           const double di = invDiag[(int64_t)e * P + n];
+          if (Gz) {  /* external loss: dL/du given, u = invDiag (r + s y) */
+            dy[n] = (float)(Gz[(int64_t)e * P + n] * di * a->s);
+            continue;
+          }
           const double z = di * (rr[n] + a->s * (double)a->y[n]);
           const double res = z - (double)Ep[n];
           lane[n] = res * res * wn;
           dy[n] = (float)(2.0 * res * wn * di * a->s);
         }
-        /* per-wave halving tree, waves added in order */
-        double t = 0.0;
-        for (int w = 0; w < P / 64; ++w) {
-          double v[64];
-          for (int l = 0; l < 64; ++l) v[l] = lane[w * 64 + l];
-          for (int h = 32; h >= 1; h >>= 1)
-            for (int l = 0; l < h; ++l) v[l] = v[l] + v[l + h];
-          t = (w == 0) ? v[0] : t + v[0];
+        if (!Gz) { /* per-wave halving tree, waves added in order */
+          double t = 0.0;
+          for (int w = 0; w < P / 64; ++w) {
+            double v[64];
+            for (int l = 0; l < 64; ++l) v[l] = lane[w * 64 + l];
+            for (int h = 32; h >= 1; h >>= 1)
+              for (int l = 0; l < h; ++l) v[l] = v[l] + v[l + h];
+            t = (w == 0) ? v[0] : t + v[0];
+          }
+          loss = loss + t;
         }
-        loss = loss + t;
         for (int n = 0; n < P; ++n) {
           for (int tap = 0; tap < TAPS; ++tap) {
             const int m = tap_nb(n, 26 - tap);

@@ -84,6 +84,38 @@ def test_gradient_matches_oracle(nbx, nbo, dev, chunk):
     assert np.array_equal(g, og)
 
 
+def test_external_loss_gradient_matches_oracle(nbx, nbo, dev):
+    import torch
+    bp, ob = make(nbx, nbo, dims=(2, 2, 2))
+    rng = np.random.default_rng(17)
+    W = nbo.bf16_round(random_net(rng))
+    net = net_with(nbx, nbo, bp, W)
+    n = bp.fieldOffset
+    R = (rng.standard_normal((2, n)) * 1e-2).astype(np.float32)
+    Ev = np.zeros((2, n), dtype=np.float32)
+    net.set_pairs(bp, torch.tensor(R, device=dev), torch.tensor(Ev + 1, device=dev))
+    G = rng.standard_normal(n)
+    samples = np.stack([np.ones(ob.E, np.int32), np.arange(ob.E, dtype=np.int32)], axis=1)
+    g, loss = net.gradient(bp, samples, 16, G=torch.tensor(G, device=dev))
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    og, _ = nbo.net_gradient(samples, 16, n, R.ravel(), Ev.ravel() + 1, np.ones(2), W, feat, ob.invdiag(), G=G)
+    assert loss == 0.0
+    assert np.array_equal(g, og)
+
+
+def test_energy_training_lowers_energy_loss(nbx, dev):
+    box = nbx.Box.make(7, 4, 4, 4, perturb=0.1, seed=5)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
+    net = nbx.Net(seed=1)
+    net.attach(bp)
+    net.generate_pairs(bp, nseeds=4, seed0=1000, snapshots=[0, 3, 10, 30], tol_exact=1e-10)
+    hist = net.train_energy(bp, steps=200, lr=2e-3, seed=3)
+    print("energy loss", hist[:3], hist[-3:])
+    # at zero output layer the loss is that of one Jacobi step, below 1
+    assert 0 < hist[0] < 1.0
+    assert hist[-20:].mean() < 0.9 * hist[:3].mean()
+
+
 @pytest.mark.parametrize("scale3", [0.05, 0.5])
 def test_learned_pcg_matches_oracle(nbx, nbo, dev, scale3):
     import torch

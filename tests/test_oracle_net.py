@@ -137,6 +137,43 @@ def test_gradient_matches_finite_differences(nbo, data):
         assert np.linalg.norm(fd) > 0, name
 
 
+def np_functional(W, samples, R, G, feat, invDiag, stride):
+    """sum_samples sum_n G[e, n] u_n with u = invDiag (r + s y)"""
+    tot = 0.0
+    for p, e in samples:
+        r = R[p * stride + e * NP_: p * stride + (e + 1) * NP_].astype(np.float64)
+        y, s = np_forward(W, r, feat[e * G_ * NP_:(e + 1) * G_ * NP_].reshape(G_, NP_))
+        sl = slice(e * NP_, (e + 1) * NP_)
+        tot += np.sum(G[sl] * invDiag[sl] * (r + s * y))
+    return tot
+
+
+def test_external_loss_gradient_matches_finite_differences(nbo, data):
+    """G mode (energy-norm training): the gradient of sum G u over the
+    parameters, u the unweighted element output."""
+    rng = np.random.default_rng(15)
+    W = random_net(rng)
+    G = rng.standard_normal(data["E"] * NP_)
+    samples = np.array([[1, 0], [0, 1], [1, 2], [0, 2]], dtype=np.int32)
+    g, loss = nbo.net_gradient(samples, 3, data["stride"], data["R"], data["Ev"], np.ones(2), W, data["feat"],
+                               data["invDiag"], G=G)
+    assert loss == 0.0
+    args = (samples, data["R"], G, data["feat"], data["invDiag"], data["stride"])
+    W64 = W.astype(np.float64)
+    for name, (a, b) in {"Wr": (OWR, OWG), "Wg": (OWG, OB1), "b1": (OB1, OW2), "W2": (OW2, OB2),
+                         "b2": (OB2, OW3), "W3": (OW3, OB3), "b3": (OB3, NPAR)}.items():
+        idx = np.arange(a, b) if b - a <= 32 else rng.choice(np.arange(a, b), 16, replace=False)
+        fd = np.empty(idx.size)
+        for m, q in enumerate(idx):
+            h = 1e-6 * max(1.0, abs(W64[q]))
+            Wp, Wm = W64.copy(), W64.copy()
+            Wp[q] += h
+            Wm[q] -= h
+            fd[m] = (np_functional(Wp, *args) - np_functional(Wm, *args)) / (2 * h)
+        err = np.linalg.norm(g[idx] - fd) / max(np.linalg.norm(fd), 1e-30)
+        assert err < 1e-3, (name, err)
+
+
 def test_gradient_chunking_changes_only_rounding(nbo, data):
     W = random_net(np.random.default_rng(8))
     samples = np.array([[0, 0], [1, 2], [1, 1], [0, 2]], dtype=np.int32)
