@@ -196,6 +196,14 @@ const void* nbx_bp_device_array(const nbx_bp* bp, const char* name);
 nbx_ogs* nbx_bp_ogs(nbx_bp* bp);
 /* BPAxOperator: Aq = gs(axhelm(q)) (+ mask); returns pAp = <q, Aq>_w through pAp_host */
 int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double* q, double* Aq, double* pAp_host);
+/* BPZeroMean (nekBone/solve.cpp:115-134): q -= unweighted mean of the
+ * E-vector over its Nf fields, on the handle's stream.  nbx_bp_pcg applies it
+ * to r before and x after the solve when nbx_bp_all_neumann() is 1: lambda1
+ * is 0 (setup.cpp:210-214) and no Dirichlet mask is set (the reference never
+ * masks, so it projects every lambda1 = 0 problem). */
+int nbx_bp_zero_mean(nbx_bp* bp, double* q);
+int nbx_bp_all_neumann(const nbx_bp* bp);
+
 /* BPPCG: solves A x = r in place (r becomes the residual), tol relative,
  * stopping test rdotr <= max(tol^2 rdotr_1, tol^2) (solve.cpp:85-90). */
 int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,

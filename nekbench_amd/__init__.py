@@ -312,6 +312,13 @@ class BP:
             raise NbxError(f"nbx_bp_pcg_mixed failed ({rc}): {last_error()}")
         return st
 
+    def zero_mean(self, q):
+        check(lib().nbx_bp_zero_mean(self.h, ptr(q)), "nbx_bp_zero_mean")
+
+    @property
+    def all_neumann(self) -> bool:
+        return bool(lib().nbx_bp_all_neumann(self.h))
+
     def history(self, n):
         rd = np.zeros(n)
         al = np.zeros(n)
@@ -360,6 +367,8 @@ def _wrap_device(addr: int, n: int, dtype):
 
 _SIGS.update({
     "nbx_net_num_params": (_I, []),
+    "nbx_bp_zero_mean": (_I, [_P, _P]),
+    "nbx_bp_all_neumann": (_I, [_P]),
     "nbx_net_create": (_P, [C.c_uint64, _P]),
     "nbx_net_free": (None, [_P]),
     "nbx_net_get_params": (_I, [_P, _P]),

@@ -18,4 +18,8 @@ struct ReduceWork {
 int launch_wdot(int32_t N, int32_t offset, int Nf, const double* w, const double* x, const double* y, bool xx,
                 ReduceWork& ws, double* out_dev, hipStream_t s);
 
+// q -= (unweighted sum of the E-vector over Nf fields) / count, on the whole
+// Nf*offset range (BPZeroMean, nekBone/solve.cpp:115-134); device only
+int launch_zero_mean(int32_t N, int32_t offset, int Nf, double count, double* q, ReduceWork& ws, hipStream_t s);
+
 }  // namespace nbx

@@ -1,6 +1,7 @@
 // BLAS-1 kernels of the nekBone PCG loop (nekBone/kernel/utils.okl) with the
 // deterministic reduction order of nbx_reduce.h.
 #include "nbx_blas.h"
+#include <algorithm>
 #include <mutex>
 
 namespace nbx {
@@ -55,6 +56,54 @@ __global__ __launch_bounds__(256) void k_scaled_add(const int32_t N, const doubl
 {
   const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n < N) y[n] = alpha * x[n] + beta * y[n];
+}
+
+// unweighted sum over the fields of the E-vector (sum, nekBone/kernel/utils.okl:212-262)
+__global__ __launch_bounds__(256) void k_sum(const int32_t N, const int32_t offset, const int Nf,
+                                             const double* __restrict__ x, double* __restrict__ partials)
+{
+  __shared__ double s4[4];
+  double v[1] = {0.0};
+  const int64_t b0 = (int64_t)blockIdx.x * kRedTile + threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < kRedPerLane; ++u) {
+    const int64_t id = b0 + u * kRedLanes;
+    if (id < N)
+      for (int f = 0; f < Nf; ++f) v[0] = v[0] + x[id + (int64_t)f * offset];
+  }
+  block_sum256<1>(v, s4);
+  store_partials<1>(v, partials);
+}
+
+// mean = total / count (BPZeroMean, nekBone/solve.cpp:129-131)
+__global__ __launch_bounds__(256) void k_mean_finish(const unsigned int nb, const double* __restrict__ partials,
+                                                     const double count, double* __restrict__ out)
+{
+  __shared__ double s4[4];
+  double v[1];
+  combine_partials<1>(v, partials, nb, s4);
+  if (threadIdx.x == 0) out[0] = v[0] / count;
+}
+
+// y[n] += -(*mean) for n < N (addScalar, nekBone/kernel/utils.okl:29-36)
+__global__ __launch_bounds__(256) void k_sub_scalar(const int64_t N, const double* __restrict__ mean,
+                                                    double* __restrict__ y)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) y[n] += -mean[0];
+}
+
+int launch_zero_mean(int32_t N, int32_t offset, int Nf, double count, double* q, ReduceWork& ws, hipStream_t s)
+{
+  if (N <= 0) return fail(NBX_ERR_INVALID, "zero_mean: N=%d", N);
+  const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
+  if (nb > ws.capacity) return fail(NBX_ERR_INVALID, "zero_mean: workspace too small (%u < %u)", ws.capacity, nb);
+  const int64_t Nall = Nf > 1 ? (int64_t)Nf * offset : (int64_t)std::max(N, offset);
+  hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, s, N, offset, Nf, q, ws.partials);
+  hipLaunchKernelGGL(k_mean_finish, dim3(1), dim3(256), 0, s, nb, ws.partials, count, ws.result + 7);
+  hipLaunchKernelGGL(k_sub_scalar, dim3((unsigned)((Nall + 255) / 256)), dim3(256), 0, s, Nall, ws.result + 7, q);
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
 }
 
 int launch_wdot(int32_t N, int32_t offset, int Nf, const double* w, const double* x, const double* y, bool xx,

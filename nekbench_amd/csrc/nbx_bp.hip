@@ -318,6 +318,10 @@ static int bp_setup_impl(nbx_bp* bp)
   bp->fieldOffset = bp->Nlocal;  // Np*(Nelements + totalHaloPairs) on one rank (setup.cpp:73-74)
   bp->Nall = (int64_t)bp->Nf * bp->fieldOffset;
   const double lambda1 = o.bpmode ? 0.0 : o.lambda1;  // setup.cpp:51-53
+  // the reference never masks, so it projects every lambda1 = 0 problem; with
+  // the Dirichlet mask applied the operator is nonsingular and no projection
+  // is made
+  bp->allNeumann = lambda1 == 0.0 && !o.dirichlet;
   const bool manufactured = o.rhs_kind == 1;
 
   // host mesh inputs
@@ -634,6 +638,17 @@ static int bp_pcg_core(nbx_bp* bp, const double* lambda, double* r, double* x, d
   return NBX_SUCCESS;
 }
 
+// q <- q - mean(q), the unweighted mean over the E-vector and its fields
+// (BPZeroMean, nekBone/solve.cpp:115-134); stream-ordered, no host wait
+extern "C" int nbx_bp_zero_mean(nbx_bp* bp, double* q)
+{
+  if (!bp || !q) return fail(NBX_ERR_INVALID, "bp_zero_mean: bad arguments");
+  return launch_zero_mean(bp->Nlocal, bp->fieldOffset, bp->Nf, (double)bp->E * bp->Nf * bp->Np, q, bp->red,
+                          bp->stream);
+}
+
+extern "C" int nbx_bp_all_neumann(const nbx_bp* bp) { return bp && bp->allNeumann ? 1 : 0; }
+
 extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
                           nbx_pcg_stats* stats)
 {
@@ -643,8 +658,10 @@ extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x
   bp->op_prec = NBX_F64;
   NBX_HIP(hipEventRecord(bp->ev0, bp->stream));
   PcgScalars S{};
-  int rc = bp_pcg_core(bp, lambda, r, x, tol, tol * tol, maxit, true, true, &S);
-  if (rc) return rc;
+  int rc;
+  if (bp->allNeumann && (rc = nbx_bp_zero_mean(bp, r))) return rc;  // main.cpp:24-25
+  if ((rc = bp_pcg_core(bp, lambda, r, x, tol, tol * tol, maxit, true, true, &S))) return rc;
+  if (bp->allNeumann && (rc = nbx_bp_zero_mean(bp, x))) return rc;  // main.cpp:30-31
   NBX_HIP(hipEventRecord(bp->ev1, bp->stream));
   NBX_HIP(hipEventSynchronize(bp->ev1));
   float ms = 0.f;

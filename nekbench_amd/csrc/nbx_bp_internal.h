@@ -27,6 +27,9 @@ struct nbx_bp {
   int N = 0, Nq = 0, Np = 0, Nf = 1;
   int32_t E = 0, Nlocal = 0, fieldOffset = 0, Nmasked = 0;
   int64_t Nall = 0;
+  // singular all-Neumann problem (lambda1 = 0, setup.cpp:210-214) without a
+  // Dirichlet mask: r and x are projected to zero mean around the solve
+  bool allNeumann = false;
   hipStream_t stream = nullptr;
   nbx_ogs* ogs = nullptr;
   double *D = nullptr, *ggeo = nullptr, *lambda = nullptr, *invDiagA = nullptr, *rhs = nullptr;

@@ -137,6 +137,8 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
 int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, double* r, double* x, double tol,
                      int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st);
 
+void nbo_bp_zero_mean(const nbo_bp* bp, double* q);
+
 /* ---- learned element preconditioner (nbo_net.c) -------------------------- */
 /* zl = w invDiag (r + s f(r)) per element (w NULL: weight 1) */
 void nbo_net_apply(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,

@@ -218,6 +218,8 @@ class OracleBP:
         self.E, self.Np = box.E, box.Np
         self.offset = self.E * self.Np
         self.lambda1 = 0.0 if bpmode else lambda1
+        # singular all-Neumann problem (setup.cpp:210-214) that carries no mask
+        self.all_neumann = self.lambda1 == 0.0 and not dirichlet
         self.bk = int(bool(bpmode))
         self.lam = np.concatenate([np.ones(self.offset), np.full(self.offset, self.lambda1)])
         self.ogs, self.off, self.gid, self.invd = ogs_setup(m["ids"])
@@ -251,12 +253,20 @@ class OracleBP:
         f.restype = C.c_double
         return f(C.byref(self.struct), p(x), p(y))
 
+    def zero_mean(self, q):
+        lib().nbo_bp_zero_mean(C.byref(self.struct), p(q))
+
     def pcg(self, r, x, tol, maxit, jacobi=True, flexible=False, fused=True):
+        """solve() of nekBone/main.cpp:4-34: BPZeroMean around BPPCG when all-Neumann."""
         hr = np.zeros(maxit + 2)
         ha = np.zeros(maxit + 2)
         st = PCGStats()
+        if self.all_neumann:
+            self.zero_mean(r)
         lib().nbo_bp_pcg(C.byref(self.struct), int(jacobi), int(flexible), int(fused), p(r), p(x), C.c_double(tol), maxit,
                          p(hr), p(ha), C.byref(st))
+        if self.all_neumann:
+            self.zero_mean(x)
         return st, hr, ha
 
     def pcg_mixed(self, r, x, tol, maxit, prec=1, inner_tol=1e-5, max_cycles=20, jacobi=True, flexible=False):

@@ -347,3 +347,18 @@ int nbo_bp_pcg_learned(const nbo_bp* bp, const float* W, const float* feat, int 
   free(invDiag);
   return its;
 }
+
+/* BPZeroMean (nekBone/solve.cpp:115-134): q -= (unweighted sum of the
+ * E-vector over its fields) / (E Nf Np) on the whole Nf*offset range, the sum
+ * in the library's tile tree */
+void nbo_bp_zero_mean(const nbo_bp* bp, double* q)
+{
+  const int Np = (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  const int64_t N = (int64_t)bp->E * Np;
+  double d[1];
+  TILE_TREE(1, N, d, {
+    for (int f = 0; f < bp->Nf; ++f) acc[0] = acc[0] + q[id + (int64_t)f * bp->offset];
+  });
+  const double mean = d[0] / ((double)bp->E * bp->Nf * Np);
+  for (int64_t i = 0; i < (int64_t)bp->Nf * bp->offset; ++i) q[i] += -mean;
+}

@@ -145,3 +145,31 @@ def test_config2_full_size_properties(nbx, dev):
     assert abs(rr_true - st.rdotr) <= 1e-6 * st.rdotr0 + 1e-3 * st.rdotr
     if st.converged:
         assert st.rdotr <= 1e-16 * st.rdotr0
+
+
+@pytest.mark.parametrize("Nf", [1, 3])
+def test_all_neumann_zero_mean_matches_oracle(nbx, nbo, dev, Nf):
+    """lambda1 = 0 without a mask: BPZeroMean on r before and on x after the
+    solve (nekBone/main.cpp:24-31); fixed iteration count as nekBone runs it."""
+    import torch
+    kw = dict(perturb=0.0, seed=5, periodic=1)
+    bp = nbx.BP(nbx.Box.make(4, 3, 3, 3, **kw), Nfields=Nf, bpmode=True, dirichlet=False,
+                precon=nbx.PRECON_COPY)
+    ob = nbo.OracleBP(nbo.Box.make(4, 3, 3, 3, **kw), Nf=Nf, bpmode=True, dirichlet=False)
+    assert bp.all_neumann and ob.all_neumann
+    n = Nf * bp.fieldOffset
+    q = torch.tensor(np.random.default_rng(2).uniform(-1, 1, n) + 0.25, device=dev)
+    qo = H(q).copy()
+    bp.zero_mean(q)
+    ob.zero_mean(qo)
+    torch.cuda.synchronize()
+    assert np.array_equal(H(q), qo)
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 0.0, 25)
+    rr, xr = ob.rhs.copy(), np.zeros(n)
+    ost, _, _ = ob.pcg(rr, xr, 0.0, 25, jacobi=False)
+    assert st.iterations == ost.iterations == 25
+    assert np.array_equal(H(x), xr)
+    assert np.array_equal(H(r), rr)
+    assert abs(xr.sum()) <= 1e-12 * np.abs(xr).sum()

@@ -113,3 +113,22 @@ def test_fused_schedule_matches_reference_schedule(nbo):
         assert np.abs(out[0][1] - out[1][1]).max() < 1e-9 * np.abs(out[0][1]).max()
         n = min(len(out[0][2]), len(out[1][2]), 10)
         assert np.allclose(out[0][2][:n], out[1][2][:n], rtol=1e-12, atol=0)
+
+
+def test_zero_mean_projection(nbo):
+    """BPZeroMean (nekBone/solve.cpp:115-134): the unweighted E-vector sum
+    over all fields divided by E*Nf*Np is subtracted everywhere."""
+    box = nbo.Box.make(3, 3, 3, 3, periodic=1)
+    for Nf in (1, 3):
+        ob = nbo.OracleBP(box, Nf=Nf, bpmode=True, dirichlet=False)
+        assert ob.all_neumann
+        q = np.random.default_rng(Nf).uniform(-1, 1, Nf * ob.offset) + 0.3
+        ref = q - q.sum() / (ob.E * Nf * ob.Np)
+        ob.zero_mean(q)
+        assert np.abs(q - ref).max() < 1e-14
+        assert abs(q.sum()) < 1e-11
+        c = np.full(Nf * ob.offset, 2.5)
+        ob.zero_mean(c)
+        assert np.all(c == 0.0)
+    assert not nbo.OracleBP(box, bpmode=False, lambda1=1.1, dirichlet=False).all_neumann
+    assert not nbo.OracleBP(nbo.Box.make(3, 2, 2, 2), bpmode=True, dirichlet=True).all_neumann
