@@ -156,7 +156,7 @@ int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, doub
  * diagonal and the PCG workspace; r and x are the caller's. */
 typedef struct nbx_bp nbx_bp;
 
-typedef enum { NBX_PRECON_COPY = 0, NBX_PRECON_JACOBI = 1, NBX_PRECON_LEARNED = 2 } nbx_precon;
+typedef enum { NBX_PRECON_COPY = 0, NBX_PRECON_JACOBI = 1, NBX_PRECON_LEARNED = 2, NBX_PRECON_MG = 3 } nbx_precon;
 
 typedef struct {
   nbx_box box;
@@ -219,6 +219,21 @@ int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double to
  * true residual. */
 int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
                      nbx_prec op_prec, double inner_tol, int max_cycles, nbx_pcg_stats* stats);
+/* ---- p-multigrid preconditioner (NBX_PRECON_MG) ---------------------------
+ * No counterpart in the reference (nekBone offers COPY and JACOBI only).
+ * Levels are the same mesh at orders orders[0] = N > orders[1] > ... >= 1;
+ * one application is a V-cycle with Chebyshev-Jacobi smoothing of the given
+ * degree on bounds [lo, hi] * lambda_max(D^-1 A) (power iteration at setup),
+ * element-local GLL interpolation between levels, and coarse_iters Jacobi-PCG
+ * iterations on the last level.  Used inside flexible PCG by nbx_bp_pcg and
+ * nbx_bp_pcg_mixed when the handle was set up with precon = NBX_PRECON_MG.
+ * One field, N <= 11.  Uses the handle's lambda. */
+int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters, double lo, double hi);
+/* order and lambda_max estimate of a level; returns the number of levels */
+int nbx_bp_mg_info(const nbx_bp* bp, int level, int* N, double* lmax);
+/* z = M r, one V-cycle (device vectors), synchronous */
+int nbx_bp_mg_apply(nbx_bp* bp, const double* r, double* z);
+
 /* ======================= learned preconditioner ==========================
  * A per-element network over the 8^3 GLL nodes (N = 7, one field): inputs
  * the residual (scaled by the element's max |r|) and the 7 geometric factors

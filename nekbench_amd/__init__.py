@@ -19,7 +19,7 @@ LIB_PATH = PKG / "libnbx.so"
 NBX_F64, NBX_F32, NBX_BF16, NBX_F16 = 0, 1, 2, 3
 NBX_DOUBLE, NBX_FLOAT, NBX_INT, NBX_LONG = 0, 1, 2, 3
 NBX_ADD, NBX_MUL, NBX_MIN, NBX_MAX = 0, 1, 2, 3
-PRECON_COPY, PRECON_JACOBI, PRECON_LEARNED = 0, 1, 2
+PRECON_COPY, PRECON_JACOBI, PRECON_LEARNED, PRECON_MG = 0, 1, 2, 3
 # reference ggeo factor order (core/params.h:55-62)
 GWJID, G00ID, G01ID, G02ID, G11ID, G12ID, G22ID, NGGEO = 0, 1, 2, 3, 4, 5, 6, 7
 
@@ -312,6 +312,21 @@ class BP:
             raise NbxError(f"nbx_bp_pcg_mixed failed ({rc}): {last_error()}")
         return st
 
+    def mg_setup(self, orders=(7, 3, 1), degree=2, coarse_iters=20, lo=0.1, hi=1.1):
+        o = np.ascontiguousarray(orders, dtype=np.int32)
+        check(lib().nbx_bp_mg_setup(self.h, o.size, ptr(o), degree, coarse_iters, lo, hi), "nbx_bp_mg_setup")
+
+    def mg_info(self, level):
+        N = C.c_int(0)
+        lam = C.c_double(0.0)
+        rc = lib().nbx_bp_mg_info(self.h, level, C.byref(N), C.byref(lam))
+        if rc < 0:
+            raise NbxError(f"nbx_bp_mg_info failed ({rc}): {last_error()}")
+        return N.value, lam.value
+
+    def mg_apply(self, r, z):
+        check(lib().nbx_bp_mg_apply(self.h, ptr(r), ptr(z)), "nbx_bp_mg_apply")
+
     def zero_mean(self, q):
         check(lib().nbx_bp_zero_mean(self.h, ptr(q)), "nbx_bp_zero_mean")
 
@@ -368,6 +383,9 @@ def _wrap_device(addr: int, n: int, dtype):
 _SIGS.update({
     "nbx_net_num_params": (_I, []),
     "nbx_bp_zero_mean": (_I, [_P, _P]),
+    "nbx_bp_mg_setup": (_I, [_P, _I, _P, _I, _I, _D, _D]),
+    "nbx_bp_mg_info": (_I, [_P, _I, _P, _P]),
+    "nbx_bp_mg_apply": (_I, [_P, _P, _P]),
     "nbx_bp_all_neumann": (_I, [_P]),
     "nbx_net_create": (_P, [C.c_uint64, _P]),
     "nbx_net_free": (None, [_P]),

@@ -297,6 +297,7 @@ static void bp_release(nbx_bp* bp)
   if (bp->feat) (void)hipFree(bp->feat);
   if (bp->zl) (void)hipFree(bp->zl);
   if (bp->net) nbx::net_detach(bp->net, bp);
+  if (bp->mg) nbx::mg_release(bp->mg);
   delete bp;
 }
 
@@ -475,7 +476,7 @@ extern "C" int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double
 }
 
 // updateJacobi (solve.cpp:146-165): local diagonal, assembled, inverted
-static int bp_update_jacobi(nbx_bp* bp, const double* lambda)
+int bp_update_jacobi(nbx_bp* bp, const double* lambda)
 {
   int rc = nbx_update_jacobi_local(bp->N, bp->E, bp->fieldOffset, nullptr, bp->ggeo, bp->D, lambda, bp->invDiagA,
                                    bp->stream);
@@ -489,7 +490,7 @@ static int bp_update_jacobi(nbx_bp* bp, const double* lambda)
   return NBX_SUCCESS;
 }
 
-static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, double* x)
+int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, double* x)
 {
   const nbx_bp_options& o = bp->opt;
   const bool jac = o.precon == NBX_PRECON_JACOBI;
@@ -498,15 +499,20 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
   const double* w = nbx_ogs_inv_degree(bp->ogs);
   const uint8_t* deg = ogs_degree(bp->ogs);
   const double* zsrc = jac ? bp->z : r;
-  if (o.precon == NBX_PRECON_LEARNED) {
-    // update; z = gs(w invDiag (r + s f(r))) (Jacobi after a stall); dots; finish
+  if (o.precon == NBX_PRECON_LEARNED || o.precon == NBX_PRECON_MG) {
+    // update; z = M r (learned: gs(w invDiag (r + s f(r))), Jacobi after a
+    // stall; multigrid: one V-cycle); dots; finish
     hipLaunchKernelGGL(k_pcg_update, grid_for(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, bp->p, bp->Ap, x, r,
                        bp->S);
-    int rc = net_apply_learned(bp, r, bp->z, &bp->S->fallback, &bp->S->done);
-    if (rc) return rc;
-    rc = gs_add_mask_launch(bp->ogs, bp->z, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, &bp->S->done,
-                            bp->stream);
-    if (rc) return rc;
+    int rc;
+    if (o.precon == NBX_PRECON_MG) {
+      if ((rc = mg_apply(bp, r, bp->z, &bp->S->done))) return rc;
+    } else {
+      if ((rc = net_apply_learned(bp, r, bp->z, &bp->S->fallback, &bp->S->done))) return rc;
+      rc = gs_add_mask_launch(bp->ogs, bp->z, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, &bp->S->done,
+                              bp->stream);
+      if (rc) return rc;
+    }
     if (deg)
       hipLaunchKernelGGL(k_pcg_dots_z<true>, dim3(nb), dim3(256), 0, bp->stream, bp->Nlocal, bp->fieldOffset, bp->Nf,
                          w, deg, r, bp->z, bp->Ap, bp->red.partials, bp->S);
@@ -538,7 +544,7 @@ static int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, dou
                        bp->hist_rdotr);
   NBX_HIP_LAUNCH();
   }
-  if (o.fused || bp->op_prec != NBX_F64 || o.precon == NBX_PRECON_LEARNED) {
+  if (o.fused || bp->op_prec != NBX_F64 || o.precon == NBX_PRECON_LEARNED || o.precon == NBX_PRECON_MG) {
     // p = z + beta p, Ap = A p and pAp element shares in one pass; then gs + mask
     const void* g = bp->ggeo;
     const void* Dm = bp->D;
@@ -611,6 +617,10 @@ static int bp_pcg_core(nbx_bp* bp, const double* lambda, double* r, double* x, d
     s0.stall_window = bp->stall_window;
     s0.stall_ratio = bp->stall_ratio;
   }
+  if (bp->opt.precon == NBX_PRECON_MG) {
+    if (!bp->mg) return fail(NBX_ERR_INVALID, "bp_pcg: MG preconditioner without nbx_bp_mg_setup");
+    s0.flexible = 1;  // the coarse solve is a fixed number of CG steps: flexible beta
+  }
   *bp->hS = s0;
   NBX_HIP(hipMemcpyAsync(bp->S, bp->hS, sizeof(PcgScalars), hipMemcpyHostToDevice, bp->stream));
   int rc;
@@ -622,7 +632,9 @@ static int bp_pcg_core(nbx_bp* bp, const double* lambda, double* r, double* x, d
       (rc = bp_update_jacobi(bp, lambda)))
     return rc;  // solve.cpp:72
   // iterations in chunks; the device flag ends the work, the host only polls
-  const int chunk = 16;
+  // each poll is a host wait; a multigrid iteration is long, so it is polled
+  // more often (the kernels of iterations past convergence exit at once)
+  const int chunk = bp->opt.precon == NBX_PRECON_MG ? 4 : 16;
   int enqueued = 0;
   for (;;) {
     const int n = std::min(chunk, maxit + 1 - enqueued);

@@ -6,6 +6,7 @@
 #include <cstddef>
 
 struct nbx_net;
+struct nbx_mg;
 
 namespace nbx {
 // the leading members are PcgFlags (nbx_internal.h), read by operator kernels
@@ -53,6 +54,7 @@ struct nbx_bp {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // learned preconditioner (attached network, features, per-element output)
   nbx_net* net = nullptr;
+  nbx_mg* mg = nullptr;     // p-multigrid preconditioner (levels own their handles)
   float* feat = nullptr;    // normalised geometric factors [E][7][Np]
   double* zl = nullptr;     // element-local preconditioner output
   int stall_window = 20;    // iterations compared for the stagnation test
@@ -73,6 +75,10 @@ struct nbx_bp {
 // (used to generate training data); x is updated in place, r becomes the
 // residual.  tol <= 0 runs exactly maxit iterations.
 int bp_solve_plain(nbx_bp* bp, nbx_precon precon, double* r, double* x, double tol, int maxit, int* iterations);
+// Jacobi diagonal of the handle's operator with this lambda into invDiagA
+int bp_update_jacobi(nbx_bp* bp, const double* lambda);
+// enqueue one PCG iteration body (device-resident scalars; no host wait)
+int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, double* x);
 // Aq = mask(gs(axhelm(q))) with the handle's FP64 operator (no dot)
 int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* Aq, const int* skip);
 // seeded right-hand side of the handle's problem (U[-1,1) per local node,
@@ -86,4 +92,7 @@ int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallb
 // forget a handle that is being freed (the network keeps a list of the
 // handles it is attached to and clears their pointer when it is freed first)
 void net_detach(nbx_net* net, nbx_bp* bp);
+// one multigrid V-cycle z = M r on the handle's stream (kernels skip when *skip)
+int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip);
+void mg_release(nbx_mg* mg);
 }  // namespace nbx

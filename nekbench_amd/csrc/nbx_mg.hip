@@ -1,0 +1,468 @@
+// p-multigrid preconditioner for the BP PCG (precon NBX_PRECON_MG).
+//
+// Levels share the mesh: level l discretises the same box at polynomial
+// order N_l (e.g. 7, 3, 1), each with its own BP handle (geometric factors,
+// gather-scatter, Dirichlet mask and Jacobi diagonal at that order).  One
+// application z = M r is a V-cycle:
+//
+//   level l < L-1:  Chebyshev-Jacobi pre-smoothing of degree k from x = 0
+//                   (k operator applies, the residual is carried along),
+//                   restriction  r_c = mask(gs_c(J^T (w res)))   per element,
+//                   x_c = V-cycle(l+1, r_c),
+//                   x += J x_c (element-local interpolation of the continuous
+//                   coarse correction, itself continuous),
+//                   Chebyshev-Jacobi post-smoothing of degree k.
+//   level L-1:      a fixed number of Jacobi-PCG iterations from x = 0.
+//
+// J is the Lagrange interpolation from the coarse GLL nodes to the fine GLL
+// nodes, applied as three 1-D contractions (x, then y, then z).  Chebyshev
+// bounds are [lo, hi] * lambda_max(D^-1 A), lambda_max from a seeded power
+// iteration at setup.  Every kernel and every sum follows a fixed order that
+// the CPU oracle restates, so V-cycles and PCG iterates are reproduced bit for
+// bit.  All work is enqueued on the handle's stream; every kernel honours the
+// outer solver's done flag so the iterations enqueued past convergence cost
+// nothing.
+#include "nbx_bp_internal.h"
+#include "nbx_blas.h"
+#include <cmath>
+#include <vector>
+
+struct nbx_mg_level {
+  nbx_bp* bp = nullptr;  // level 0: the solver's handle (not owned)
+  bool owned = false;
+  int N = 0, Nq = 0;
+  double lmax = 0.0;
+  std::vector<double> c1, c2;  // Chebyshev step coefficients
+  double *r = nullptr, *x = nullptr, *res = nullptr, *d = nullptr, *Ad = nullptr;
+  double* J = nullptr;  // [Nq][Nq of level l+1]: interpolation from the next coarser level
+};
+
+struct nbx_mg {
+  std::vector<nbx_mg_level> lv;
+  int degree = 2, coarse_iters = 20;
+  double lo = 0.1, hi = 1.1;
+  nbx::PcgScalars* coarse_template = nullptr;  // device copy of the coarse solve's initial scalars
+  std::vector<void*> owned;
+};
+
+namespace nbx {
+
+inline dim3 mg_grid(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// Chebyshev-Jacobi step (n local nodes of one field):
+//   MODE 0 (first pre-smoothing step, x = 0):  res = r; d = c2 D^-1 res; x = d
+//   MODE 1 (general step):  res -= Ad; d = c1 d + c2 D^-1 res; x += d
+//   MODE 2 (first post-smoothing step):  res -= Ad; d = c2 D^-1 res; x += d
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cheb(const int32_t n, const double c1, const double c2,
+                                              const double* __restrict__ invD, const double* __restrict__ r,
+                                              double* __restrict__ res, const double* __restrict__ Ad,
+                                              double* __restrict__ d, double* __restrict__ x,
+                                              const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ri;
+  if (MODE == 0) ri = r[i];
+  else ri = res[i] - Ad[i];
+  res[i] = ri;
+  const double t = c2 * (invD[i] * ri);
+  const double di = (MODE == 1) ? __builtin_fma(c1, d[i], t) : t;
+  d[i] = di;
+  x[i] = (MODE == 0) ? di : x[i] + di;
+}
+
+// Restriction of one element: res -= Ad (stored), u = w res, then
+// rc[a,b,c] = sum_k J[k][c] sum_j J[j][b] sum_i J[i][a] u[i,j,k]
+// (node index i + j Nq + k Nq^2; fma chains in ascending fine index).
+__global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nqc, const double* __restrict__ J,
+                                                     const double* __restrict__ w, double* __restrict__ res,
+                                                     const double* __restrict__ Ad, double* __restrict__ rc,
+                                                     const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  __shared__ double s_u[12 * 12 * 12];
+  __shared__ double s_t[12 * 12 * 12];
+  __shared__ double s_J[12 * 12];
+  const int64_t e = blockIdx.x;
+  const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
+  for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
+  for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
+    const int64_t id = e * Npf + q;
+    const double rq = res[id] - Ad[id];
+    res[id] = rq;
+    s_u[q] = w[id] * rq;
+  }
+  __syncthreads();
+  // x: t1[a + Nqc (j + Nqf k)]
+  for (int q = threadIdx.x; q < Nqc * Nqf * Nqf; q += blockDim.x) {
+    const int a = q % Nqc, jk = q / Nqc;
+    double acc = 0.0;
+    for (int i = 0; i < Nqf; ++i) acc = __builtin_fma(s_J[i * Nqc + a], s_u[i + Nqf * jk], acc);
+    s_t[q] = acc;
+  }
+  __syncthreads();
+  // y: t2[a + Nqc (b + Nqc k)] into s_u
+  for (int q = threadIdx.x; q < Nqc * Nqc * Nqf; q += blockDim.x) {
+    const int a = q % Nqc, b = (q / Nqc) % Nqc, k = q / (Nqc * Nqc);
+    double acc = 0.0;
+    for (int j = 0; j < Nqf; ++j) acc = __builtin_fma(s_J[j * Nqc + b], s_t[a + Nqc * (j + Nqf * k)], acc);
+    s_u[q] = acc;
+  }
+  __syncthreads();
+  // z: rc[a + Nqc (b + Nqc c)]
+  for (int q = threadIdx.x; q < Npc; q += blockDim.x) {
+    const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
+    double acc = 0.0;
+    for (int k = 0; k < Nqf; ++k) acc = __builtin_fma(s_J[k * Nqc + c], s_u[ab + Nqc * Nqc * k], acc);
+    rc[e * Npc + q] = acc;
+  }
+}
+
+// Prolongation of one element: v[i,j,k] = sum_c J[k][c] sum_b J[j][b] sum_a J[i][a] xc[a,b,c];
+// d = v, x += v.
+__global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc, const double* __restrict__ J,
+                                                    const double* __restrict__ xc, double* __restrict__ d,
+                                                    double* __restrict__ x, const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  __shared__ double s_u[12 * 12 * 12];
+  __shared__ double s_t[12 * 12 * 12];
+  __shared__ double s_J[12 * 12];
+  const int64_t e = blockIdx.x;
+  const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
+  for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
+  for (int q = threadIdx.x; q < Npc; q += blockDim.x) s_u[q] = xc[e * Npc + q];
+  __syncthreads();
+  // x: t1[i + Nqf (b + Nqc c)]
+  for (int q = threadIdx.x; q < Nqf * Nqc * Nqc; q += blockDim.x) {
+    const int i = q % Nqf, bc = q / Nqf;
+    double acc = 0.0;
+    for (int a = 0; a < Nqc; ++a) acc = __builtin_fma(s_J[i * Nqc + a], s_u[a + Nqc * bc], acc);
+    s_t[q] = acc;
+  }
+  __syncthreads();
+  // y: t2[i + Nqf (j + Nqf c)] into s_u
+  for (int q = threadIdx.x; q < Nqf * Nqf * Nqc; q += blockDim.x) {
+    const int i = q % Nqf, j = (q / Nqf) % Nqf, c = q / (Nqf * Nqf);
+    double acc = 0.0;
+    for (int b = 0; b < Nqc; ++b) acc = __builtin_fma(s_J[j * Nqc + b], s_t[i + Nqf * (b + Nqc * c)], acc);
+    s_u[q] = acc;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
+    const int ij = q % (Nqf * Nqf), k = q / (Nqf * Nqf);
+    double acc = 0.0;
+    for (int c = 0; c < Nqc; ++c) acc = __builtin_fma(s_J[k * Nqc + c], s_u[ij + Nqf * Nqf * c], acc);
+    const int64_t id = e * Npf + q;
+    d[id] = acc;
+    x[id] = x[id] + acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mg_fill0(const int64_t n, double* __restrict__ v,
+                                                  const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0.0;
+}
+
+// coarse solve scalars <- template; the solve is skipped when the outer one is done
+__global__ void k_mg_reset(PcgScalars* __restrict__ S, const PcgScalars* __restrict__ tmpl,
+                           const int* __restrict__ skip)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  *S = *tmpl;
+  if (skip && *skip) S->done = 1;
+}
+
+// dst = a * (invD ? invD src : src)
+__global__ __launch_bounds__(256) void k_mg_scale(const int64_t n, const double a, const double* __restrict__ invD,
+                                                  const double* __restrict__ src, double* __restrict__ dst)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = a * (invD ? invD[i] * src[i] : src[i]);
+}
+
+}  // namespace nbx
+
+using namespace nbx;
+
+namespace {
+
+// Lagrange interpolation from nodes zc (Nqc) to nodes zf (Nqf): J[i][a] =
+// prod_{m != a} (zf[i] - zc[m]) / (zc[a] - zc[m]), factors in ascending m.
+std::vector<double> interp_matrix(int Nf, int Nc)
+{
+  std::vector<double> zf(Nf + 1), wf(Nf + 1), zc(Nc + 1), wc(Nc + 1);
+  nbx_gll(Nf, zf.data(), wf.data(), nullptr);
+  nbx_gll(Nc, zc.data(), wc.data(), nullptr);
+  std::vector<double> J((size_t)(Nf + 1) * (Nc + 1));
+  for (int i = 0; i <= Nf; ++i)
+    for (int a = 0; a <= Nc; ++a) {
+      double l = 1.0;
+      for (int m = 0; m <= Nc; ++m)
+        if (m != a) l = l * ((zf[i] - zc[m]) / (zc[a] - zc[m]));
+      J[(size_t)i * (Nc + 1) + a] = l;
+    }
+  return J;
+}
+
+void cheb_coeffs(double lmax, double lo, double hi, int degree, std::vector<double>& c1, std::vector<double>& c2)
+{
+  const double a = lo * lmax, b = hi * lmax;
+  const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  c1.assign(degree, 0.0);
+  c2.assign(degree, 0.0);
+  c2[0] = 1.0 / theta;
+  for (int s = 1; s < degree; ++s) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    c1[s] = rn * rho;
+    c2[s] = 2.0 * rn / delta;
+    rho = rn;
+  }
+}
+
+int mg_alloc(nbx_mg* mg, double** p, int64_t n)
+{
+  NBX_HIP(hipMalloc((void**)p, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+  NBX_HIP(hipMemset(*p, 0, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+  mg->owned.push_back(*p);
+  return NBX_SUCCESS;
+}
+
+// lambda_max(D^-1 A) by power iteration: v0 = gs(mask(U[-1,1) seeded)),
+// normalised in the weighted norm; v <- D^-1 A v / ||D^-1 A v||_w; the
+// estimate is the last norm.  Norms are taken on the host (setup only).
+int power_lmax(nbx_bp* bp, uint64_t seed, uint64_t stream_id, int iters, double* v, double* w, double* out)
+{
+  const int64_t n = bp->Nlocal;
+  std::vector<double> h(n);
+  int rc = nbx_urand_fill(seed, stream_id, n, h.data());
+  if (rc) return rc;
+  std::vector<int32_t> mids(bp->Nmasked);
+  if (bp->Nmasked)
+    NBX_HIP(hipMemcpy(mids.data(), bp->maskIds, sizeof(int32_t) * bp->Nmasked, hipMemcpyDeviceToHost));
+  for (int32_t m : mids) h[m] = 0.0;
+  NBX_HIP(hipMemcpyAsync(v, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, bp->stream));
+  if ((rc = gs_add_mask_launch(bp->ogs, v, 1, bp->fieldOffset, 0, nullptr, nullptr, bp->stream))) return rc;
+  const double* wt = nbx_ogs_inv_degree(bp->ogs);
+  double nrm = 0.0;
+  auto norm2 = [&](const double* a) -> int {
+    int r2 = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, wt, a, a, false, bp->red, bp->red.result, bp->stream);
+    if (r2) return r2;
+    NBX_HIP(hipMemcpyAsync(&nrm, bp->red.result, sizeof(double), hipMemcpyDeviceToHost, bp->stream));
+    NBX_HIP(hipStreamSynchronize(bp->stream));
+    return NBX_SUCCESS;
+  };
+  if ((rc = norm2(v))) return rc;
+  if (!(nrm > 0)) return fail(NBX_ERR_INVALID, "mg: power iteration start vector is zero");
+  hipLaunchKernelGGL(k_mg_scale, mg_grid(n), dim3(256), 0, bp->stream, n, 1.0 / std::sqrt(nrm), nullptr, v, v);
+  double lam = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    if ((rc = bp_apply(bp, bp->lambda, v, w, nullptr))) return rc;
+    hipLaunchKernelGGL(k_mg_scale, mg_grid(n), dim3(256), 0, bp->stream, n, 1.0, bp->invDiagA, w, w);
+    if ((rc = norm2(w))) return rc;
+    lam = std::sqrt(nrm);
+    hipLaunchKernelGGL(k_mg_scale, mg_grid(n), dim3(256), 0, bp->stream, n, 1.0 / lam, nullptr, w, v);
+  }
+  NBX_HIP_LAUNCH();
+  *out = lam;
+  return NBX_SUCCESS;
+}
+
+int coarse_solve(nbx_mg* mg, nbx_mg_level& L, const int* skip)
+{
+  nbx_bp* bp = L.bp;
+  hipLaunchKernelGGL(k_mg_fill0, mg_grid(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, L.x, skip);
+  hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(64), 0, bp->stream, bp->S, mg->coarse_template, skip);
+  NBX_HIP_LAUNCH();
+  for (int it = 0; it <= mg->coarse_iters; ++it) {
+    int rc = bp_enqueue_iteration(bp, bp->lambda, L.r, L.x);
+    if (rc) return rc;
+  }
+  return NBX_SUCCESS;
+}
+
+int vcycle(nbx_mg* mg, int l, const double* r, double* x, const int* skip)
+{
+  nbx_mg_level& L = mg->lv[l];
+  if (l + 1 == (int)mg->lv.size()) return coarse_solve(mg, L, skip);
+  nbx_mg_level& C = mg->lv[l + 1];
+  nbx_bp* bp = L.bp;
+  const int32_t n = bp->Nlocal;
+  const dim3 g = mg_grid(n), b(256);
+  const int k = mg->degree;
+  int rc;
+  // pre-smoothing from x = 0
+  hipLaunchKernelGGL(k_cheb<0>, g, b, 0, bp->stream, n, 0.0, L.c2[0], bp->invDiagA, r, L.res, nullptr, L.d, x, skip);
+  for (int s = 1; s < k; ++s) {
+    if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
+    hipLaunchKernelGGL(k_cheb<1>, g, b, 0, bp->stream, n, L.c1[s], L.c2[s], bp->invDiagA, nullptr, L.res, L.Ad,
+                       L.d, x, skip);
+  }
+  if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
+  // restriction of the smoothed residual
+  hipLaunchKernelGGL(k_mg_restrict, dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, L.J, nbx_ogs_inv_degree(bp->ogs),
+                     L.res, L.Ad, C.r, skip);
+  NBX_HIP_LAUNCH();
+  nbx_bp* bc = C.bp;
+  if ((rc = gs_add_mask_launch(bc->ogs, C.r, 1, bc->fieldOffset, bc->Nmasked, bc->maskIds, skip, bc->stream)))
+    return rc;
+  if ((rc = vcycle(mg, l + 1, C.r, C.x, skip))) return rc;
+  // coarse correction
+  hipLaunchKernelGGL(k_mg_prolong, dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, L.J, C.x, L.d, x, skip);
+  NBX_HIP_LAUNCH();
+  if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
+  // post-smoothing
+  hipLaunchKernelGGL(k_cheb<2>, g, b, 0, bp->stream, n, 0.0, L.c2[0], bp->invDiagA, nullptr, L.res, L.Ad, L.d, x,
+                     skip);
+  for (int s = 1; s < k; ++s) {
+    if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
+    hipLaunchKernelGGL(k_cheb<1>, g, b, 0, bp->stream, n, L.c1[s], L.c2[s], bp->invDiagA, nullptr, L.res, L.Ad,
+                       L.d, x, skip);
+  }
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}
+
+}  // namespace
+
+namespace nbx {
+
+void mg_release(nbx_mg* mg)
+{
+  if (!mg) return;
+  (void)hipDeviceSynchronize();
+  for (auto& L : mg->lv)
+    if (L.owned && L.bp) nbx_bp_free(L.bp);
+  for (void* p : mg->owned) (void)hipFree(p);
+  if (mg->coarse_template) (void)hipFree(mg->coarse_template);
+  delete mg;
+}
+
+int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip)
+{
+  if (!bp->mg) return fail(NBX_ERR_INVALID, "multigrid preconditioner: not set up (nbx_bp_mg_setup)");
+  return vcycle(bp->mg, 0, r, z, skip);
+}
+
+}  // namespace nbx
+
+extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters,
+                               double lo, double hi)
+{
+  if (!bp || nlevels < 2 || nlevels > 8 || !orders || degree < 1 || degree > 16 || coarse_iters < 1 ||
+      !(lo > 0) || !(hi > lo))
+    return fail(NBX_ERR_INVALID, "bp_mg_setup: bad arguments");
+  if (bp->Nf != 1) return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: one field");
+  if (orders[0] != bp->N) return fail(NBX_ERR_INVALID, "bp_mg_setup: orders[0] must be the handle's N");
+  for (int l = 1; l < nlevels; ++l)
+    if (orders[l] < 1 || orders[l] >= orders[l - 1])
+      return fail(NBX_ERR_INVALID, "bp_mg_setup: orders must decrease to >= 1");
+  if (orders[0] > 11) return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: N <= 11");
+  if (bp->mg) {
+    mg_release(bp->mg);
+    bp->mg = nullptr;
+  }
+  nbx_mg* mg = new nbx_mg();
+  mg->degree = degree;
+  mg->coarse_iters = coarse_iters;
+  mg->lo = lo;
+  mg->hi = hi;
+  mg->lv.resize(nlevels);
+  int rc = NBX_SUCCESS;
+  for (int l = 0; l < nlevels && rc == NBX_SUCCESS; ++l) {
+    nbx_mg_level& L = mg->lv[l];
+    L.N = orders[l];
+    L.Nq = orders[l] + 1;
+    if (l == 0) {
+      L.bp = bp;
+    } else {
+      nbx_bp_options o = bp->opt;
+      o.box.N = orders[l];
+      o.precon = NBX_PRECON_JACOBI;
+      o.flexible = 0;
+      o.fused = 1;
+      o.verbose = 0;
+      L.bp = nbx_bp_setup(&o, bp->stream);
+      if (!L.bp) {
+        rc = NBX_ERR_INVALID;
+        break;
+      }
+      L.owned = true;
+    }
+    nbx_bp* b = L.bp;
+    if ((rc = bp_update_jacobi(b, b->lambda))) break;
+    if ((rc = mg_alloc(mg, &L.res, b->Nall)) || (rc = mg_alloc(mg, &L.d, b->Nall)) ||
+        (rc = mg_alloc(mg, &L.Ad, b->Nall)))
+      break;
+    if (l > 0 && ((rc = mg_alloc(mg, &L.r, b->Nall)) || (rc = mg_alloc(mg, &L.x, b->Nall)))) break;
+    if (l + 1 < nlevels) {
+      const std::vector<double> J = interp_matrix(orders[l], orders[l + 1]);
+      if ((rc = mg_alloc(mg, &L.J, (int64_t)J.size()))) break;
+      if (hipMemcpy(L.J, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(NBX_ERR_HIP, "bp_mg_setup: copy of J");
+        break;
+      }
+      if ((rc = power_lmax(b, bp->opt.box.seed, 17 + (uint64_t)l, 20, L.d, L.Ad, &L.lmax))) break;
+      cheb_coeffs(L.lmax, lo, hi, degree, L.c1, L.c2);
+    } else {
+      // coarse solve: fixed-count Jacobi PCG with device-resident scalars
+      if (b->hist_cap < coarse_iters + 2) {
+        if (b->hist_rdotr) (void)hipFree(b->hist_rdotr);
+        if (b->hist_alpha) (void)hipFree(b->hist_alpha);
+        b->hist_rdotr = b->hist_alpha = nullptr;
+        b->hist_cap = coarse_iters + 2;
+        if (hipMalloc(&b->hist_rdotr, sizeof(double) * b->hist_cap) != hipSuccess ||
+            hipMalloc(&b->hist_alpha, sizeof(double) * b->hist_cap) != hipSuccess) {
+          rc = fail(NBX_ERR_ALLOC, "bp_mg_setup: coarse history");
+          break;
+        }
+      }
+      PcgScalars t{};
+      t.rdotz1 = 1.0;
+      t.rdotz2 = 0.0;
+      t.tol2 = 0.0;
+      t.abs_floor = 0.0;
+      t.maxit = coarse_iters;
+      t.fixed = 1;
+      t.flexible = 0;
+      if (hipMalloc(&mg->coarse_template, sizeof(PcgScalars)) != hipSuccess ||
+          hipMemcpy(mg->coarse_template, &t, sizeof(PcgScalars), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(NBX_ERR_ALLOC, "bp_mg_setup: coarse scalars");
+        break;
+      }
+    }
+  }
+  if (rc == NBX_SUCCESS && hipStreamSynchronize(bp->stream) != hipSuccess)
+    rc = fail(NBX_ERR_HIP, "bp_mg_setup: stream");
+  if (rc != NBX_SUCCESS) {
+    mg_release(mg);
+    return rc;
+  }
+  bp->mg = mg;
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_mg_info(const nbx_bp* bp, int level, int* N, double* lmax)
+{
+  if (!bp || !bp->mg || level < 0 || level >= (int)bp->mg->lv.size())
+    return fail(NBX_ERR_INVALID, "bp_mg_info: no such level");
+  if (N) *N = bp->mg->lv[level].N;
+  if (lmax) *lmax = bp->mg->lv[level].lmax;
+  return (int)bp->mg->lv.size();
+}
+
+// one V-cycle z = M r (device vectors of the handle's size), synchronous
+extern "C" int nbx_bp_mg_apply(nbx_bp* bp, const double* r, double* z)
+{
+  if (!bp || !r || !z) return fail(NBX_ERR_INVALID, "bp_mg_apply: bad arguments");
+  int rc = mg_apply(bp, r, z, nullptr);
+  if (rc) return rc;
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  return NBX_SUCCESS;
+}

@@ -139,6 +139,24 @@ int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, doubl
 
 void nbo_bp_zero_mean(const nbo_bp* bp, double* q);
 
+/* ---- p-multigrid preconditioner (nbo_pcg.c) ------------------------------ */
+typedef struct nbo_mg {
+  int nlev;
+  const nbo_bp* bps;           /* levels, finest first */
+  const double* const* invD;   /* assembled inverse Jacobi diagonal per level */
+  const double* const* J;      /* level l < nlev-1: [N_l+1][N_{l+1}+1] interpolation */
+  int degree;                  /* Chebyshev degree */
+  const double* c1;            /* [nlev][degree] Chebyshev coefficients */
+  const double* c2;
+  int coarse_iters;            /* Jacobi-PCG iterations on the last level */
+} nbo_mg;
+void nbo_mg_interp(int Nf, int Nc, double* J);
+void nbo_mg_cheb(double lmax, double lo, double hi, int degree, double* c1, double* c2);
+double nbo_mg_power(const nbo_bp* bp, const double* invD, uint64_t seed, uint64_t stream, int iters);
+void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x);
+int nbo_bp_pcg_mg(const nbo_mg* mg, double* r, double* x, double tol, int maxit, double* hist_rdotr,
+                  double* hist_alpha, nbo_pcg_stats* st);
+
 /* ---- learned element preconditioner (nbo_net.c) -------------------------- */
 /* zl = w invDiag (r + s f(r)) per element (w NULL: weight 1) */
 void nbo_net_apply(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,

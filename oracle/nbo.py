@@ -331,3 +331,62 @@ def bf16_round(x):
     u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
     u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
     return u.view(np.float32)
+
+
+class MGStruct(C.Structure):
+    _fields_ = [("nlev", C.c_int), ("bps", C.c_void_p), ("invD", C.c_void_p), ("J", C.c_void_p),
+                ("degree", C.c_int), ("c1", C.c_void_p), ("c2", C.c_void_p), ("coarse_iters", C.c_int)]
+
+
+class OracleMG:
+    """The oracle's p-multigrid preconditioner, built as nbx_bp_mg_setup
+    builds it: one OracleBP per order, Jacobi diagonals, interpolation
+    matrices, power-iteration lambda_max and Chebyshev coefficients."""
+
+    def __init__(self, box: Box, orders, degree=2, coarse_iters=20, lo=0.1, hi=1.1, bpmode=True, lambda1=0.0,
+                 dirichlet=True, power_iters=20):
+        assert orders[0] == box.N
+        self.levels = []
+        for N in orders:
+            b = Box.make(N, box.NX, box.NY, box.NZ, box.periodic, box.perturb, box.zaspect, box.seed,
+                         (box.x0, box.x1, box.y0, box.y1, box.z0, box.z1))
+            self.levels.append(OracleBP(b, bpmode=bpmode, lambda1=lambda1, dirichlet=dirichlet))
+        n = len(orders)
+        self.fine = self.levels[0]
+        self.bps = (BPStruct * n)(*[lv.struct for lv in self.levels])
+        self.invD = [lv.invdiag() for lv in self.levels]
+        self.invD_ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in self.invD])
+        self.J = []
+        for l in range(n - 1):
+            Jm = np.zeros((orders[l] + 1) * (orders[l + 1] + 1))
+            lib().nbo_mg_interp(orders[l], orders[l + 1], p(Jm))
+            self.J.append(Jm)
+        self.J_ptrs = (C.c_void_p * n)(*([a.ctypes.data for a in self.J] + [None]))
+        power = lib().nbo_mg_power
+        power.restype = C.c_double
+        self.lmax = []
+        self.c1 = np.zeros(n * degree)
+        self.c2 = np.zeros(n * degree)
+        for l in range(n - 1):
+            lam = power(C.byref(self.bps, l * C.sizeof(BPStruct)), p(self.invD[l]), C.c_uint64(box.seed),
+                        C.c_uint64(17 + l), power_iters)
+            self.lmax.append(lam)
+            c1 = np.zeros(degree)
+            c2 = np.zeros(degree)
+            lib().nbo_mg_cheb(C.c_double(lam), C.c_double(lo), C.c_double(hi), degree, p(c1), p(c2))
+            self.c1[l * degree:(l + 1) * degree] = c1
+            self.c2[l * degree:(l + 1) * degree] = c2
+        self.struct = MGStruct(n, C.addressof(self.bps), C.addressof(self.invD_ptrs), C.addressof(self.J_ptrs),
+                               degree, self.c1.ctypes.data, self.c2.ctypes.data, coarse_iters)
+
+    def vcycle(self, r):
+        x = np.zeros_like(r)
+        lib().nbo_mg_vcycle(C.byref(self.struct), 0, p(np.ascontiguousarray(r)), p(x))
+        return x
+
+    def pcg(self, r, x, tol, maxit):
+        hr = np.zeros(maxit + 2)
+        ha = np.zeros(maxit + 2)
+        st = PCGStats()
+        lib().nbo_bp_pcg_mg(C.byref(self.struct), p(r), p(x), C.c_double(tol), maxit, p(hr), p(ha), C.byref(st))
+        return st, hr, ha

@@ -155,6 +155,7 @@ typedef struct {
   const float* feat;   /* normalised geometric factors [E][7][512] */
   int window;          /* stall test: rdotr(it) > ratio * rdotr(it - window) -> Jacobi */
   double ratio;
+  const nbo_mg* mg;    /* non-NULL: the preconditioner is one multigrid V-cycle instead */
 } learned_ctx;
 
 /* PCG core; lo == NULL: FP64 operator.  initial: r -= A x first.
@@ -189,7 +190,10 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
     /* z = M r; (r.z)_w, (r.r)_w [, (z.Ap)_w]  (solve.cpp:77-83, 94) */
     double d[3];
     const double* zsrc = jacobi ? z : r;
-    if (ln) {  /* z = gs(w invDiag (r + s f(r))), Jacobi after a stall */
+    if (ln && ln->mg) {  /* z = one V-cycle applied to r */
+      nbo_mg_vcycle(ln->mg, 0, r, z);
+      zsrc = z;
+    } else if (ln) {  /* z = gs(w invDiag (r + s f(r))), Jacobi after a stall */
       if (fallback) {
         for (int64_t n = 0; n < N; ++n) z[n] = bp->w[n] * (invDiag[n] * r[n]);
       } else {
@@ -226,7 +230,8 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
     if (it == 1) { rdotr0 = rdotr; TOL = fmax(tol * tol * rdotr, abs_floor); }  /* solve.cpp:85 */
     if (!isfinite(rdotr)) { breakdown = 1; iterations = it - 1; break; }
     if (rdotr <= TOL && !fixed) { converged = 1; iterations = it - 1; break; }  /* :90 */
-    if (ln && !fallback && it > ln->window && !(rdotr <= ln->ratio * hist_rdotr[it - ln->window])) {
+    if (ln && !ln->mg && !fallback && it > ln->window &&
+        !(rdotr <= ln->ratio * hist_rdotr[it - ln->window])) {
       fallback = 1;
       fallback_iter = it;
     }
@@ -341,7 +346,7 @@ int nbo_bp_pcg_learned(const nbo_bp* bp, const float* W, const float* feat, int 
   const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
   double* invDiag = (double*)malloc(sizeof(double) * N);
   local_jacobi(bp, invDiag);
-  const learned_ctx ln = {W, feat, window, ratio};
+  const learned_ctx ln = {W, feat, window, ratio, NULL};
   const int its = pcg_core_ext(bp, 1, 1, 1, NULL, invDiag, &ln, r, x, tol, tol * tol, maxit, 1, hist_rdotr,
                                hist_alpha, st);
   free(invDiag);
@@ -361,4 +366,205 @@ void nbo_bp_zero_mean(const nbo_bp* bp, double* q)
   });
   const double mean = d[0] / ((double)bp->E * bp->Nf * Np);
   for (int64_t i = 0; i < (int64_t)bp->Nf * bp->offset; ++i) q[i] += -mean;
+}
+
+
+/* ---- p-multigrid preconditioner (library: nbx_mg.hip) ------------------- */
+
+/* J[i][a] = prod_{m != a} (zf[i] - zc[m]) / (zc[a] - zc[m]), ascending m */
+void nbo_mg_interp(int Nf, int Nc, double* J)
+{
+  double zf[32], wf[32], zc[32], wc[32];
+  nbo_gll(Nf, zf, wf);
+  nbo_gll(Nc, zc, wc);
+  for (int i = 0; i <= Nf; ++i)
+    for (int a = 0; a <= Nc; ++a) {
+      double l = 1.0;
+      for (int m = 0; m <= Nc; ++m)
+        if (m != a) l = l * ((zf[i] - zc[m]) / (zc[a] - zc[m]));
+      J[i * (Nc + 1) + a] = l;
+    }
+}
+
+void nbo_mg_cheb(double lmax, double lo, double hi, int degree, double* c1, double* c2)
+{
+  const double a = lo * lmax, b = hi * lmax;
+  const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  for (int s = 0; s < degree; ++s) c1[s] = c2[s] = 0.0;
+  c2[0] = 1.0 / theta;
+  for (int s = 1; s < degree; ++s) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    c1[s] = rn * rho;
+    c2[s] = 2.0 * rn / delta;
+    rho = rn;
+  }
+}
+
+/* lambda_max(D^-1 A): v0 = gs(mask(U[-1,1))) normalised in the weighted
+ * norm, then v <- D^-1 A v / ||D^-1 A v||_w; the last norm */
+double nbo_mg_power(const nbo_bp* bp, const double* invD, uint64_t seed, uint64_t stream, int iters)
+{
+  const int64_t n = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  double* v = (double*)calloc(bp->offset, sizeof(double));
+  double* w = (double*)calloc(bp->offset, sizeof(double));
+  for (int64_t i = 0; i < n; ++i) v[i] = nbo_urand(seed, stream, (uint64_t)i);
+  for (int32_t m = 0; m < bp->Nmasked; ++m) v[bp->maskIds[m]] = 0.0;
+  nbo_gs_many(bp->ogs, v, 1, bp->offset, NBO_DOUBLE, NBO_ADD);
+  double nrm = nbo_bp_wdot(bp, v, v);
+  double sc = 1.0 / sqrt(nrm);
+  for (int64_t i = 0; i < n; ++i) v[i] = sc * v[i];
+  double lam = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    nbo_bp_apply(bp, v, w);
+    for (int64_t i = 0; i < n; ++i) w[i] = 1.0 * (invD[i] * w[i]);
+    nrm = nbo_bp_wdot(bp, w, w);
+    lam = sqrt(nrm);
+    sc = 1.0 / lam;
+    for (int64_t i = 0; i < n; ++i) v[i] = sc * w[i];
+  }
+  free(v);
+  free(w);
+  return lam;
+}
+
+/* V-cycle at level l: x = M_l r (x overwritten) */
+void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
+{
+  const nbo_bp* bp = &mg->bps[l];
+  const int Nq = bp->N + 1, Np = Nq * Nq * Nq;
+  const int64_t n = (int64_t)bp->E * Np;
+  const double* invD = mg->invD[l];
+  if (l == mg->nlev - 1) {  /* coarse: fixed-count Jacobi PCG from x = 0 (r is copied) */
+    double* rr = (double*)malloc(sizeof(double) * bp->offset);
+    memcpy(rr, r, sizeof(double) * bp->offset);
+    memset(x, 0, sizeof(double) * bp->offset);
+    nbo_pcg_stats st;
+    pcg_core(bp, 1, 0, 1, NULL, invD, rr, x, 0.0, 0.0, mg->coarse_iters, 0, NULL, NULL, &st);
+    free(rr);
+    return;
+  }
+  const nbo_bp* bc = &mg->bps[l + 1];
+  const int Nqc = bc->N + 1, Npc = Nqc * Nqc * Nqc;
+  const double* J = mg->J[l];
+  const double* c1 = mg->c1 + (size_t)l * mg->degree;
+  const double* c2 = mg->c2 + (size_t)l * mg->degree;
+  const int k = mg->degree;
+  double* res = (double*)calloc(bp->offset, sizeof(double));
+  double* d = (double*)calloc(bp->offset, sizeof(double));
+  double* Ad = (double*)calloc(bp->offset, sizeof(double));
+  double* rc = (double*)calloc(bc->offset, sizeof(double));
+  double* xc = (double*)calloc(bc->offset, sizeof(double));
+  /* pre-smoothing from x = 0 */
+  for (int64_t i = 0; i < n; ++i) {
+    const double ri = r[i];
+    res[i] = ri;
+    const double t = c2[0] * (invD[i] * ri);
+    d[i] = t;
+    x[i] = t;
+  }
+  for (int s = 1; s < k; ++s) {
+    nbo_bp_apply(bp, d, Ad);
+    for (int64_t i = 0; i < n; ++i) {
+      const double ri = res[i] - Ad[i];
+      res[i] = ri;
+      const double t = c2[s] * (invD[i] * ri);
+      d[i] = fma(c1[s], d[i], t);
+      x[i] = x[i] + d[i];
+    }
+  }
+  nbo_bp_apply(bp, d, Ad);
+  /* restriction: res -= Ad; rc = J^T (w res) per element, x then y then z */
+#pragma omp parallel
+  {
+    double* u = (double*)malloc(sizeof(double) * Np);
+    double* t1 = (double*)malloc(sizeof(double) * Np);
+    double* t2 = (double*)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+    for (int32_t e = 0; e < bp->E; ++e) {
+      for (int q = 0; q < Np; ++q) {
+        const int64_t id = (int64_t)e * Np + q;
+        const double rq = res[id] - Ad[id];
+        res[id] = rq;
+        u[q] = bp->w[id] * rq;
+      }
+      for (int q = 0; q < Nqc * Nq * Nq; ++q) {
+        const int a = q % Nqc, jk = q / Nqc;
+        double acc = 0.0;
+        for (int i = 0; i < Nq; ++i) acc = fma(J[i * Nqc + a], u[i + Nq * jk], acc);
+        t1[q] = acc;
+      }
+      for (int q = 0; q < Nqc * Nqc * Nq; ++q) {
+        const int a = q % Nqc, b = (q / Nqc) % Nqc, kk = q / (Nqc * Nqc);
+        double acc = 0.0;
+        for (int j = 0; j < Nq; ++j) acc = fma(J[j * Nqc + b], t1[a + Nqc * (j + Nq * kk)], acc);
+        t2[q] = acc;
+      }
+      for (int q = 0; q < Npc; ++q) {
+        const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
+        double acc = 0.0;
+        for (int kk = 0; kk < Nq; ++kk) acc = fma(J[kk * Nqc + c], t2[ab + Nqc * Nqc * kk], acc);
+        rc[(int64_t)e * Npc + q] = acc;
+      }
+    }
+    free(u); free(t1); free(t2);
+  }
+  nbo_gs_many(bc->ogs, rc, 1, bc->offset, NBO_DOUBLE, NBO_ADD);
+  for (int32_t m = 0; m < bc->Nmasked; ++m) rc[bc->maskIds[m]] = 0.0;
+  nbo_mg_vcycle(mg, l + 1, rc, xc);
+  /* prolongation: d = J xc; x += d */
+#pragma omp parallel
+  {
+    double* t1 = (double*)malloc(sizeof(double) * Np);
+    double* t2 = (double*)malloc(sizeof(double) * Np);
+#pragma omp for schedule(static)
+    for (int32_t e = 0; e < bp->E; ++e) {
+      const double* uc = xc + (int64_t)e * Npc;
+      for (int q = 0; q < Nq * Nqc * Nqc; ++q) {
+        const int i = q % Nq, bc_ = q / Nq;
+        double acc = 0.0;
+        for (int a = 0; a < Nqc; ++a) acc = fma(J[i * Nqc + a], uc[a + Nqc * bc_], acc);
+        t1[q] = acc;
+      }
+      for (int q = 0; q < Nq * Nq * Nqc; ++q) {
+        const int i = q % Nq, j = (q / Nq) % Nq, c = q / (Nq * Nq);
+        double acc = 0.0;
+        for (int b = 0; b < Nqc; ++b) acc = fma(J[j * Nqc + b], t1[i + Nq * (b + Nqc * c)], acc);
+        t2[q] = acc;
+      }
+      for (int q = 0; q < Np; ++q) {
+        const int ij = q % (Nq * Nq), kk = q / (Nq * Nq);
+        double acc = 0.0;
+        for (int c = 0; c < Nqc; ++c) acc = fma(J[kk * Nqc + c], t2[ij + Nq * Nq * c], acc);
+        const int64_t id = (int64_t)e * Np + q;
+        d[id] = acc;
+        x[id] = x[id] + acc;
+      }
+    }
+    free(t1); free(t2);
+  }
+  nbo_bp_apply(bp, d, Ad);
+  /* post-smoothing */
+  for (int s = 0; s < k; ++s) {
+    if (s > 0) nbo_bp_apply(bp, d, Ad);
+    for (int64_t i = 0; i < n; ++i) {
+      const double ri = res[i] - Ad[i];
+      res[i] = ri;
+      const double t = c2[s] * (invD[i] * ri);
+      const double di = s == 0 ? t : fma(c1[s], d[i], t);
+      d[i] = di;
+      x[i] = x[i] + di;
+    }
+  }
+  free(res); free(d); free(Ad); free(rc); free(xc);
+}
+
+/* BPPCG with the V-cycle as preconditioner (library: nbx_bp_pcg, precon MG):
+ * flexible beta, the stopping rule of BPPCG */
+int nbo_bp_pcg_mg(const nbo_mg* mg, double* r, double* x, double tol, int maxit, double* hist_rdotr,
+                  double* hist_alpha, nbo_pcg_stats* st)
+{
+  const learned_ctx ln = {NULL, NULL, 0, 0.0, mg};
+  return pcg_core_ext(&mg->bps[0], 1, 1, 1, NULL, mg->invD[0], &ln, r, x, tol, tol * tol, maxit, 1, hist_rdotr,
+                      hist_alpha, st);
 }

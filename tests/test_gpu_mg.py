@@ -1,0 +1,91 @@
+"""p-multigrid preconditioner on the GPU against the oracle (OracleMG):
+power-iteration lambda_max per level, one V-cycle, and the MG-preconditioned
+flexible PCG iterate for iterate, all bit for bit; argument checks."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def H(t):
+    return t.cpu().numpy()
+
+
+CASES = [  # (N, dims, orders, degree, coarse_iters, bpmode, lambda1)
+    (7, (3, 3, 2), [7, 3, 1], 2, 10, True, 0.0),
+    (5, (2, 3, 2), [5, 2, 1], 3, 6, False, 0.7),
+    (3, (3, 2, 2), [3, 1], 1, 12, True, 0.0),
+]
+
+
+def make(nbx, nbo, case):
+    N, dims, orders, deg, ci, bpmode, l1 = case
+    kw = dict(perturb=0.1, seed=50 + N)
+    bp = nbx.BP(nbx.Box.make(N, *dims, **kw), bpmode=bpmode, lambda1=l1, precon=nbx.PRECON_MG)
+    bp.mg_setup(orders, deg, ci)
+    mg = nbo.OracleMG(nbo.Box.make(N, *dims, **kw), orders, degree=deg, coarse_iters=ci, bpmode=bpmode,
+                      lambda1=l1)
+    return bp, mg
+
+
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3"])
+def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case):
+    import torch
+    bp, mg = make(nbx, nbo, case)
+    orders = case[2]
+    for l in range(len(orders) - 1):
+        N, lam = bp.mg_info(l)
+        assert N == orders[l]
+        assert lam == mg.lmax[l]
+    n = bp.fieldOffset
+    r = mg.fine.rhs.copy()
+    z = torch.zeros(n, dtype=torch.float64, device=dev)
+    bp.mg_apply(torch.tensor(r, device=dev), z)
+    assert np.array_equal(H(z), mg.vcycle(r))
+
+
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3"])
+def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case):
+    import torch
+    bp, mg = make(nbx, nbo, case)
+    n = bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-10, 200)
+    xr = np.zeros(n)
+    ost, _, _ = mg.pcg(mg.fine.rhs.copy(), xr, 1e-10, 200)
+    assert st.iterations == ost.iterations and st.converged == ost.converged == 1
+    assert st.rdotr == ost.rdotr
+    assert np.array_equal(H(x), xr)
+
+
+def test_mg_cuts_iterations(nbx, dev):
+    import torch
+    box = nbx.Box.make(7, 6, 6, 6, perturb=0.1, seed=3)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
+    bp.mg_setup([7, 3, 1], 2, 20)
+    bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, fused=True)
+    n = bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-8, 500)
+    rj = bj.device_array("rhs", n).clone()
+    xj = torch.zeros_like(rj)
+    sj = bj.pcg(rj, xj, 1e-8, 2000)
+    print("mg", st.iterations, st.elapsed_s, "jacobi", sj.iterations, sj.elapsed_s)
+    assert st.converged and sj.converged and st.iterations * 10 < sj.iterations
+    assert (x - xj).abs().max().item() <= 1e-6 * xj.abs().max().item()
+
+
+def test_mg_argument_checks(nbx, dev):
+    import torch
+    bp = nbx.BP(nbx.Box.make(5, 2, 2, 2), bpmode=True, precon=nbx.PRECON_MG)
+    r = bp.device_array("rhs", bp.fieldOffset).clone()
+    with pytest.raises(nbx.NbxError):
+        bp.pcg(r, torch.zeros_like(r), 1e-8, 10)  # not set up
+    for orders in ([4, 2, 1], [5, 5, 1], [5, 2, 0], [5]):
+        with pytest.raises(nbx.NbxError):
+            bp.mg_setup(orders)
+    b3 = nbx.BP(nbx.Box.make(5, 2, 2, 2), Nfields=3, precon=nbx.PRECON_MG)
+    with pytest.raises(nbx.NbxError):
+        b3.mg_setup([5, 2, 1])

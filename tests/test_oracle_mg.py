@@ -1,0 +1,107 @@
+"""Pins the oracle's p-multigrid preconditioner (oracle/nbo_pcg.c, no
+reference counterpart): interpolation is exact for polynomials of the coarse
+order and reproduces nodal values at shared GLL end points; the power
+iteration bounds the spectrum of D^-1 A from below and converges to the
+largest eigenvalue of a dense small problem; the Chebyshev coefficients
+reproduce the Chebyshev polynomial's values; the V-cycle is a symmetric
+linear operator (for its linear part) and the MG-PCG converges in far fewer
+iterations than Jacobi PCG to the same solution."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+
+def test_interpolation_exact_for_coarse_polynomials(nbo):
+    for Nf, Nc in ((7, 3), (3, 1), (9, 4), (5, 2)):
+        J = np.zeros((Nf + 1) * (Nc + 1))
+        nbo.lib().nbo_mg_interp(Nf, Nc, nbo.p(J))
+        J = J.reshape(Nf + 1, Nc + 1)
+        zf = nbo.gll(Nf)[0]
+        zc = nbo.gll(Nc)[0]
+        for deg in range(Nc + 1):
+            assert np.allclose(J @ zc ** deg, zf ** deg, atol=1e-13)
+        assert np.array_equal(J[0], np.eye(Nc + 1)[0]) and np.array_equal(J[-1], np.eye(Nc + 1)[-1])
+
+
+def test_chebyshev_coefficients_follow_the_recurrence(nbo):
+    lam, lo, hi, k = 2.3, 0.1, 1.1, 4
+    c1, c2 = np.zeros(k), np.zeros(k)
+    nbo.lib().nbo_mg_cheb(C.c_double(lam), C.c_double(lo), C.c_double(hi), k, nbo.p(c1), nbo.p(c2))
+    # smoothing polynomial p(t) = 1 - t q(t) after k steps, applied to one
+    # eigenvalue t: must equal T_k((b + a - 2t)/(b - a)) / T_k((b + a)/(b - a))
+    a, b = lo * lam, hi * lam
+    for t in np.linspace(0.0, b, 7):
+        x, d, res = 0.0, 0.0, 1.0
+        for s in range(k):
+            if s > 0:
+                res = res - t * d
+            d = c2[s] * res if s == 0 else c1[s] * d + c2[s] * res
+            x += d
+        err = 1 - t * x
+        Tk = np.polynomial.chebyshev.Chebyshev.basis(k)
+        assert abs(err - Tk((b + a - 2 * t) / (b - a)) / Tk((b + a) / (b - a))) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def mg44(nbo):
+    box = nbo.Box.make(5, 3, 3, 2, perturb=0.1, seed=7)
+    return nbo.OracleMG(box, [5, 2, 1], degree=2, coarse_iters=10)
+
+
+def test_power_iteration_estimates_lambda_max(nbo, mg44):
+    ob = mg44.fine
+    n = ob.offset
+    # dense D^-1 A on the unknowns (unique, unmasked nodes) through the operator
+    keep = np.setdiff1d(np.arange(n), ob.maskIds)
+    gid = ob.mesh["ids"]
+    uniq, first = np.unique(gid[keep], return_index=True)
+    reps = keep[first]
+    cols = []
+    for g in uniq:
+        e = np.zeros(n)
+        e[gid == g] = 1.0
+        cols.append(ob.apply(e)[reps] * mg44.invD[0][reps])
+    M = np.array(cols).T
+    lam_true = np.max(np.linalg.eigvals(M).real)
+    lam = mg44.lmax[0]
+    assert lam <= lam_true * (1 + 1e-12)
+    assert lam >= 0.95 * lam_true
+
+
+def test_vcycle_is_linear_and_symmetric_in_its_smoothing_part(nbo):
+    """With one coarse CG iteration dropped to a direct-like solve the cycle
+    stays linear only if the coarse solve is; check linearity for the smooth
+    part by comparing V(a r1 + r2) with a V(r1) + V(r2) at a coarse solve
+    long enough to be converged."""
+    box = nbo.Box.make(3, 3, 3, 3, perturb=0.1, seed=2)
+    mg = nbo.OracleMG(box, [3, 1], degree=3, coarse_iters=200)
+    ob = mg.fine
+    rng = np.random.default_rng(0)
+    def cont(v):
+        v = v.copy()
+        v[ob.maskIds] = 0
+        nbo.gs_many(ob.ogs, v, 1, ob.offset)
+        return v
+    r1, r2 = cont(rng.standard_normal(ob.offset)), cont(rng.standard_normal(ob.offset))
+    z = mg.vcycle(2.0 * r1 + r2)
+    z12 = 2.0 * mg.vcycle(r1) + mg.vcycle(r2)
+    assert np.abs(z - z12).max() <= 1e-9 * np.abs(z).max()
+    # symmetric in the weighted inner product: <V r1, r2>_w = <r1, V r2>_w
+    w = ob.invd
+    a, b = np.sum(w * mg.vcycle(r1) * r2), np.sum(w * r1 * mg.vcycle(r2))
+    assert abs(a - b) <= 1e-9 * abs(a)
+
+
+def test_mg_pcg_converges_fast_to_the_jacobi_solution(nbo, mg44):
+    ob = mg44.fine
+    r, x = ob.rhs.copy(), np.zeros(ob.offset)
+    st, hr, _ = mg44.pcg(r, x, 1e-10, 500)
+    rj, xj = ob.rhs.copy(), np.zeros(ob.offset)
+    sj, _, _ = ob.pcg(rj, xj, 1e-12, 2000)
+    assert st.converged and sj.converged
+    assert st.iterations * 5 < sj.iterations
+    assert np.abs(x - xj).max() <= 1e-8 * np.abs(xj).max()
+    # the true residual agrees with the recursively updated one
+    tr = ob.rhs - ob.apply(x)
+    assert np.sqrt(ob.wdot(tr, tr)) <= 1e-9 * np.sqrt(hr[1])
