@@ -45,6 +45,14 @@ __device__ __forceinline__ float ldg<float, uint16_t>(const uint16_t* p)
   return __uint_as_float((uint32_t)ldnt(p) << 16);
 }
 
+// Elements per workgroup: one element (Nq*Nq lanes) from Nq = 8 up; small
+// orders pack several elements (threadIdx.y = slot) so that a workgroup
+// holds at least 128 lanes.
+template <int NQ>
+struct Epb {
+  static constexpr int value = NQ * NQ >= 64 ? 1 : 128 / (NQ * NQ);
+};
+
 // Per-workgroup state of the operator: LDS slabs and the lane's D vectors.
 template <typename T, int NQ>
 struct AxLane {
@@ -128,18 +136,27 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
 }
 
 template <typename T, int NQ, int NF, bool BK, bool PARTIAL>
-__global__ __launch_bounds__(NQ* NQ) void k_axhelm(const int32_t Nelements, const int32_t offset,
-                                                   const int32_t* __restrict__ elementList,
-                                                   const T* __restrict__ ggeo, const T* __restrict__ D,
-                                                   const T* __restrict__ lambda, const T* __restrict__ q,
-                                                   T* __restrict__ Aq, const int* __restrict__ skip)
+__global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm(const int32_t Nelements, const int32_t offset,
+                                                                  const int32_t* __restrict__ elementList,
+                                                                  const T* __restrict__ ggeo,
+                                                                  const T* __restrict__ D,
+                                                                  const T* __restrict__ lambda,
+                                                                  const T* __restrict__ q, T* __restrict__ Aq,
+                                                                  const int* __restrict__ skip)
 {
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
+  constexpr int EPB = Epb<NQ>::value;
   if (skip && *skip) return;
-  __shared__ T s_D[NQ2], s_q[NQ2], s_Gr[NQ2], s_Gs[NQ2];
-  const int64_t e = PARTIAL ? (int64_t)elementList[blockIdx.x] : (int64_t)blockIdx.x;
-  s_D[threadIdx.x] = D[threadIdx.x];
+  __shared__ T s_D[NQ2], s_q[EPB][NQ2], s_Gr[EPB][NQ2], s_Gs[EPB][NQ2];
+  const int slot = EPB > 1 ? (int)threadIdx.y : 0;
+  const int32_t idx = (int32_t)blockIdx.x * EPB + slot;
+  const bool active = EPB == 1 || idx < Nelements;
+  // an idle slot works on the last element without storing: every lane
+  // reaches the workgroup barriers
+  const int32_t ie = EPB == 1 ? (int32_t)blockIdx.x : (active ? idx : Nelements - 1);
+  const int64_t e = PARTIAL ? (int64_t)elementList[ie] : (int64_t)ie;
+  if (slot == 0) s_D[threadIdx.x] = D[threadIdx.x];
   __syncthreads();
   AxLane<T, NQ> L;
   L.init(s_D);
@@ -152,9 +169,11 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm(const int32_t Nelements, cons
     T rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) rq[k] = ldnt(&q[base + k * NQ2]);
-    axhelm_column<T, NQ, BK>(L, rq, out, s_D, s_q, s_Gr, s_Gs, g, l0, l1);
+    axhelm_column<T, NQ, BK>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g, l0, l1);
+    if (active) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) stnt(out[k], &Aq[base + k * NQ2]);
+      for (int k = 0; k < NQ; ++k) stnt(out[k], &Aq[base + k * NQ2]);
+    }
   }
 }
 
@@ -167,22 +186,30 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm(const int32_t Nelements, cons
 // summed by a halving tree over the next power of two (zero padded); the
 // element partials go to partials[e].
 template <int NQ, int NF, bool BK, typename T, typename G>
-__global__ __launch_bounds__(NQ* NQ) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
-                                                       const G* __restrict__ ggeo, const T* __restrict__ D,
-                                                       const G* __restrict__ lambda,
-                                                       const double* __restrict__ z, double* __restrict__ p,
-                                                       double* __restrict__ Ap, double* __restrict__ partials,
-                                                       const PcgFlags* __restrict__ S)
+__global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
+                                                                      const G* __restrict__ ggeo,
+                                                                      const T* __restrict__ D,
+                                                                      const G* __restrict__ lambda,
+                                                                      const double* __restrict__ z,
+                                                                      double* __restrict__ p,
+                                                                      double* __restrict__ Ap,
+                                                                      double* __restrict__ partials,
+                                                                      const PcgFlags* __restrict__ S)
 {
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
+  constexpr int EPB = Epb<NQ>::value;
   constexpr int P2 = NQ2 <= 1 ? 1 : (NQ2 <= 2 ? 2 : (NQ2 <= 4 ? 4 : (NQ2 <= 8 ? 8 : (NQ2 <= 16 ? 16 : (NQ2 <= 32 ? 32 : (NQ2 <= 64 ? 64 : (NQ2 <= 128 ? 128 : 256)))))));
   if (S->done) return;
   const double beta = S->beta;
-  __shared__ T s_D[NQ2], s_q[NQ2], s_Gr[NQ2], s_Gs[NQ2];
-  __shared__ double s_red[P2];
-  const int64_t e = blockIdx.x;
-  s_D[threadIdx.x] = D[threadIdx.x];
+  __shared__ T s_D[NQ2], s_q[EPB][NQ2], s_Gr[EPB][NQ2], s_Gs[EPB][NQ2];
+  __shared__ double s_red[EPB][P2];
+  const int slot = EPB > 1 ? (int)threadIdx.y : 0;
+  const int32_t idx = (int32_t)blockIdx.x * EPB + slot;
+  const bool active = EPB == 1 || idx < Nelements;
+  // idle slots compute on the last element and store nothing
+  const int64_t e = EPB == 1 ? (int64_t)blockIdx.x : (active ? idx : Nelements - 1);
+  if (slot == 0) s_D[threadIdx.x] = D[threadIdx.x];
   __syncthreads();
   AxLane<T, NQ> L;
   L.init(s_D);
@@ -198,14 +225,18 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm_pcg(const int32_t Nelements, 
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       pk[k] = __builtin_fma(beta, p[base + k * NQ2], ldnt(&z[base + k * NQ2]));
-      p[base + k * NQ2] = pk[k];
       rq[k] = (T)pk[k];
     }
-    axhelm_column<T, NQ, BK, G>(L, rq, out, s_D, s_q, s_Gr, s_Gs, g, l0, l1);
+    if (EPB > 1) __syncthreads();  // all slots read p before any writes it (idle slots alias the last element)
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) p[base + k * NQ2] = pk[k];
+    }
+    axhelm_column<T, NQ, BK, G>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g, l0, l1);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const double o = (double)out[k];
-      stnt(o, &Ap[base + k * NQ2]);
+      if (active) stnt(o, &Ap[base + k * NQ2]);
       acc = __builtin_fma(pk[k], o, acc);
     }
   }
@@ -213,15 +244,17 @@ __global__ __launch_bounds__(NQ* NQ) void k_axhelm_pcg(const int32_t Nelements, 
   if (NQ2 == 64) {
     sum = wave_sum(acc);
   } else {
-    s_red[L.t] = acc;
-    if (L.t + NQ2 < P2) s_red[L.t + NQ2] = 0.0;
+    double* red = s_red[slot];
+    red[L.t] = acc;
+    if (L.t + NQ2 < P2) red[L.t + NQ2] = 0.0;
     for (int h = P2 / 2; h >= 1; h >>= 1) {
       __syncthreads();
-      if (L.t < h) s_red[L.t] = s_red[L.t] + s_red[L.t + h];
+      if (L.t < h) red[L.t] = red[L.t] + red[L.t + h];
     }
-    sum = s_red[0];
+    __syncthreads();
+    sum = red[0];
   }
-  if (L.t == 0) partials[e] = sum;
+  if (L.t == 0 && active) partials[e] = sum;
 }
 
 // Local Jacobi diagonal, nekBone/kernel/updateJacobi.okl:27-108, same
@@ -270,7 +303,8 @@ static int launch_axhelm_t(int32_t Nelements, int32_t offset, const int32_t* ele
                            const void* ggeo, const void* D, const void* lambda, const void* q, void* Aq,
                            const int* skip, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL>), dim3(Nelements), dim3(NQ * NQ), 0, s,
+  constexpr int EPB = Epb<NQ>::value;
+  hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, EPB), 0, s,
                      Nelements, offset, elementList, (const T*)ggeo, (const T*)D, (const T*)lambda,
                      (const T*)q, (T*)Aq, skip);
   NBX_HIP_LAUNCH();
@@ -352,20 +386,21 @@ static int axhelm_pcg_launch_t(int N, int Nf, int bk, int32_t Nelements, int32_t
 {
 #define NBX_AXP(NV)                                                                                          \
   case NV:                                                                                                   \
+    if (Nelements < 1) break;                                                                                \
     if (Nf == 1 && bk)                                                                                       \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, true, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, true, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), Epb<NV + 1>::value), 0, \
                          s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,       \
                          partials, S);                                                                       \
     else if (Nf == 1)                                                                                        \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, false, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)),   \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 1, false, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), Epb<NV + 1>::value),   \
                          0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
                          partials, S);                                                                       \
     else if (bk)                                                                                             \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)), 0, \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), Epb<NV + 1>::value), 0, \
                          s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,       \
                          partials, S);                                                                       \
     else                                                                                                     \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3(Nelements), dim3((NV + 1) * (NV + 1)),   \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), Epb<NV + 1>::value),   \
                          0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
                          partials, S);                                                                       \
     break;
