@@ -57,7 +57,7 @@ def cpu_baseline(N, NX, NY, NZ, iters):
     }
 
 
-def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None):
+def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None, mg_cfg=None):
     """Time to a 1e-8 relative FP64 residual on the same problem: FP64 Jacobi
     PCG without the iteration cap, and the mixed-precision refinement solvers.
     Each solver runs once untimed, then once timed; the error of each
@@ -91,9 +91,11 @@ def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None):
     run("mixed_bf16_jacobi", lambda: bp.pcg_mixed(r, x, tol, 20000, prec=nbx.NBX_BF16, inner_tol=2e-2,
                                                   max_cycles=40))
     if bp_mg is not None:
-        run("fp64_mg", lambda: bp_mg.pcg(r, x, tol, 500))
-        run("mixed_f32_mg", lambda: bp_mg.pcg_mixed(r, x, tol, 500, prec=nbx.NBX_F32, inner_tol=1e-6,
-                                                    max_cycles=10))
+        orders, degree, ci = mg_cfg
+        for name, prec in (("fp64_mg", nbx.NBX_F64), ("mixed_mg_f32cycle", nbx.NBX_F32),
+                           ("mixed_mg_bf16cycle", nbx.NBX_BF16)):
+            bp_mg.mg_setup(orders, degree=degree, coarse_iters=ci, prec=prec)
+            run(name, lambda: bp_mg.pcg(r, x, tol, 500))
     base = out["fp64_jacobi"]["time_s"]
     for k, v in out.items():
         v["speedup_vs_fp64_jacobi"] = round(base / v["time_s"], 3)
@@ -196,10 +198,11 @@ def main():
     compare = None
     if not args.no_compare:
         t1 = time.perf_counter()
+        mg_cfg = ([7, 3, 1], 3, 20)  # orders, Chebyshev degree, coarse PCG iterations
         bp_mg = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
-        bp_mg.mg_setup([7, 3, 1], degree=2, coarse_iters=20)
+        bp_mg.mg_setup(mg_cfg[0], degree=mg_cfg[1], coarse_iters=mg_cfg[2])
         log(f"[bench] multigrid setup {time.perf_counter() - t1:.1f} s (orders 7,3,1)")
-        compare = solver_comparison(nbx, bp, b, r, x, log, bp_mg=bp_mg)
+        compare = solver_comparison(nbx, bp, b, r, x, log, bp_mg=bp_mg, mg_cfg=mg_cfg)
         bp_mg.free()
 
     traffic = None

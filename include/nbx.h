@@ -64,7 +64,9 @@ int nbx_device_synchronize(void);
  * ignored when bk = 1 (may be NULL then).  q and Aq must not overlap; Aq is
  * overwritten.  elementList == NULL processes elements 0..Nelements-1.
  * prec selects the arithmetic and the element type of ggeo, D, lambda, q, Aq:
- * NBX_F64 (double), NBX_F32 (float).  N in [1, 15]. */
+ * NBX_F64 (double), NBX_F32 (float), NBX_BF16 (float D, q, Aq and FP32
+ * arithmetic; ggeo and lambda as bf16, i.e. the upper 16 bits of the float,
+ * uint16_t; one field, no elementList).  N in [1, 15]. */
 int nbx_axhelm(int N, int Nfields, int bk, nbx_prec prec,
                nbx_dlong Nelements, nbx_dlong offset, const nbx_dlong* elementList,
                const void* ggeo, const void* D, const void* lambda,
@@ -227,8 +229,12 @@ int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, dou
  * element-local GLL interpolation between levels, and coarse_iters Jacobi-PCG
  * iterations on the last level.  Used inside flexible PCG by nbx_bp_pcg and
  * nbx_bp_pcg_mixed when the handle was set up with precon = NBX_PRECON_MG.
- * One field, N <= 11.  Uses the handle's lambda. */
-int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters, double lo, double hi);
+ * prec: NBX_F64 (all FP64), NBX_F32 or NBX_BF16 (smoothing and transfers of
+ * every level but the last in FP32, operator geometry in FP32 or bf16; the
+ * cycle reads the FP64 residual and returns an FP64 correction; the coarse
+ * PCG is FP64).  One field, N <= 11.  Uses the handle's lambda. */
+int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters, double lo, double hi,
+                    nbx_prec prec);
 /* order and lambda_max estimate of a level; returns the number of levels */
 int nbx_bp_mg_info(const nbx_bp* bp, int level, int* N, double* lmax);
 /* z = M r, one V-cycle (device vectors), synchronous */

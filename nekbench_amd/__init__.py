@@ -312,9 +312,10 @@ class BP:
             raise NbxError(f"nbx_bp_pcg_mixed failed ({rc}): {last_error()}")
         return st
 
-    def mg_setup(self, orders=(7, 3, 1), degree=2, coarse_iters=20, lo=0.1, hi=1.1):
+    def mg_setup(self, orders=(7, 3, 1), degree=2, coarse_iters=20, lo=0.1, hi=1.1, prec=NBX_F64):
         o = np.ascontiguousarray(orders, dtype=np.int32)
-        check(lib().nbx_bp_mg_setup(self.h, o.size, ptr(o), degree, coarse_iters, lo, hi), "nbx_bp_mg_setup")
+        check(lib().nbx_bp_mg_setup(self.h, o.size, ptr(o), degree, coarse_iters, lo, hi, prec),
+              "nbx_bp_mg_setup")
 
     def mg_info(self, level):
         N = C.c_int(0)
@@ -383,7 +384,7 @@ def _wrap_device(addr: int, n: int, dtype):
 _SIGS.update({
     "nbx_net_num_params": (_I, []),
     "nbx_bp_zero_mean": (_I, [_P, _P]),
-    "nbx_bp_mg_setup": (_I, [_P, _I, _P, _I, _I, _D, _D]),
+    "nbx_bp_mg_setup": (_I, [_P, _I, _P, _I, _I, _D, _D, _I]),
     "nbx_bp_mg_info": (_I, [_P, _I, _P, _P]),
     "nbx_bp_mg_apply": (_I, [_P, _P, _P]),
     "nbx_bp_all_neumann": (_I, [_P]),

@@ -20,6 +20,7 @@
 #include "nbx_reduce.h"
 #include <cstring>
 #include <cstdio>
+#include <type_traits>
 #include <cstdlib>
 
 namespace nbx {
@@ -135,12 +136,12 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
   }
 }
 
-template <typename T, int NQ, int NF, bool BK, bool PARTIAL>
+template <typename T, int NQ, int NF, bool BK, bool PARTIAL, typename G = T>
 __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm(const int32_t Nelements, const int32_t offset,
                                                                   const int32_t* __restrict__ elementList,
-                                                                  const T* __restrict__ ggeo,
+                                                                  const G* __restrict__ ggeo,
                                                                   const T* __restrict__ D,
-                                                                  const T* __restrict__ lambda,
+                                                                  const G* __restrict__ lambda,
                                                                   const T* __restrict__ q, T* __restrict__ Aq,
                                                                   const int* __restrict__ skip)
 {
@@ -160,16 +161,16 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm(const int32_t
   __syncthreads();
   AxLane<T, NQ> L;
   L.init(s_D);
-  const T* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
-  const T* l0 = lambda + e * NP + L.t;
-  const T* l1 = lambda + e * NP + offset + L.t;
+  const G* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
+  const G* l0 = lambda + e * NP + L.t;
+  const G* l1 = lambda + e * NP + offset + L.t;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const int64_t base = e * NP + (int64_t)f * offset + L.t;
     T rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) rq[k] = ldnt(&q[base + k * NQ2]);
-    axhelm_column<T, NQ, BK>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g, l0, l1);
+    axhelm_column<T, NQ, BK, G>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g, l0, l1);
     if (active) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) stnt(out[k], &Aq[base + k * NQ2]);
@@ -298,24 +299,29 @@ __global__ __launch_bounds__(NQ* NQ) void k_jacobi(const int32_t Nelements, cons
   }
 }
 
-template <typename T, int NQ, int NF, bool BK, bool PARTIAL>
+template <typename T, int NQ, int NF, bool BK, bool PARTIAL, typename G = T>
 static int launch_axhelm_t(int32_t Nelements, int32_t offset, const int32_t* elementList,
                            const void* ggeo, const void* D, const void* lambda, const void* q, void* Aq,
                            const int* skip, hipStream_t s)
 {
   constexpr int EPB = Epb<NQ>::value;
-  hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, EPB), 0, s,
-                     Nelements, offset, elementList, (const T*)ggeo, (const T*)D, (const T*)lambda,
+  hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL, G>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, EPB), 0,
+                     s, Nelements, offset, elementList, (const G*)ggeo, (const T*)D, (const G*)lambda,
                      (const T*)q, (T*)Aq, skip);
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
 }
 
-template <typename T, int NQ>
+template <typename T, int NQ, typename G = T>
 static int dispatch_flags(int Nf, int bk, bool partial, int32_t Nelements, int32_t offset,
                           const int32_t* elementList, const void* ggeo, const void* D,
                           const void* lambda, const void* q, void* Aq, const int* skip, hipStream_t s)
 {
+  if (!std::is_same<T, G>::value) {  // bf16 geometry: one field, whole mesh
+    if (Nf != 1 || partial) return fail(NBX_ERR_UNSUPPORTED, "axhelm bf16: one field, no element list");
+    if (bk) return launch_axhelm_t<T, NQ, 1, true, false, G>(Nelements, offset, nullptr, ggeo, D, lambda, q, Aq, skip, s);
+    return launch_axhelm_t<T, NQ, 1, false, false, G>(Nelements, offset, nullptr, ggeo, D, lambda, q, Aq, skip, s);
+  } else {
 #define NBX_AX_CASE(NFV, BKV, PV)                                                               \
   if (Nf == NFV && (bk != 0) == BKV && partial == PV)                                           \
     return launch_axhelm_t<T, NQ, NFV, BKV, PV>(Nelements, offset, elementList, ggeo, D, lambda, \
@@ -329,10 +335,11 @@ static int dispatch_flags(int Nf, int bk, bool partial, int32_t Nelements, int32
   NBX_AX_CASE(3, false, true)
   NBX_AX_CASE(3, true, true)
 #undef NBX_AX_CASE
+  }
   return fail(NBX_ERR_UNSUPPORTED, "axhelm: Nfields=%d not supported (1 or 3)", Nf);
 }
 
-template <typename T>
+template <typename T, typename G = T>
 static int dispatch_n(int N, int Nf, int bk, bool partial, int32_t Nelements, int32_t offset,
                       const int32_t* elementList, const void* ggeo, const void* D, const void* lambda,
                       const void* q, void* Aq, const int* skip, hipStream_t s)
@@ -340,7 +347,7 @@ static int dispatch_n(int N, int Nf, int bk, bool partial, int32_t Nelements, in
   switch (N) {
 #define NBX_N_CASE(NV) \
   case NV:             \
-    return dispatch_flags<T, NV + 1>(Nf, bk, partial, Nelements, offset, elementList, ggeo, D, lambda, q, Aq, skip, s);
+    return dispatch_flags<T, NV + 1, G>(Nf, bk, partial, Nelements, offset, elementList, ggeo, D, lambda, q, Aq, skip, s);
     NBX_N_CASE(1)
     NBX_N_CASE(2)
     NBX_N_CASE(3)
@@ -374,6 +381,9 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
     case NBX_F32:
       return dispatch_n<float>(N, Nf, bk, partial, Nelements, offset, elementList, ggeo, D, lambda, q, Aq, skip,
                                s);
+    case NBX_BF16:  // FP32 vectors and arithmetic, bf16 geometric factors and lambda
+      return dispatch_n<float, uint16_t>(N, Nf, bk, partial, Nelements, offset, elementList, ggeo, D, lambda, q, Aq,
+                                         skip, s);
     default:
       return fail(NBX_ERR_UNSUPPORTED, "axhelm: precision %d not supported by nbx_axhelm", (int)prec);
   }

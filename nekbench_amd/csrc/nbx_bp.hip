@@ -691,7 +691,7 @@ extern "C" int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x
   return S.iterations;
 }
 
-static int bp_prepare_precision(nbx_bp* bp, nbx_prec prec, const double* lambda)
+int bp_prepare_precision(nbx_bp* bp, nbx_prec prec, const double* lambda)
 {
   const int64_t ng = (int64_t)bp->E * 7 * bp->Np, nl = 2 * (int64_t)bp->fieldOffset;
   if (prec == NBX_F32 || prec == NBX_BF16) {

@@ -79,6 +79,8 @@ int bp_solve_plain(nbx_bp* bp, nbx_precon precon, double* r, double* x, double t
 int bp_update_jacobi(nbx_bp* bp, const double* lambda);
 // enqueue one PCG iteration body (device-resident scalars; no host wait)
 int bp_enqueue_iteration(nbx_bp* bp, const double* lambda, double* r, double* x);
+// FP32 / bf16 copies of D, ggeo and lambda for the low-precision operator
+int bp_prepare_precision(nbx_bp* bp, nbx_prec prec, const double* lambda);
 // Aq = mask(gs(axhelm(q))) with the handle's FP64 operator (no dot)
 int bp_apply(nbx_bp* bp, const double* lambda, const double* q, double* Aq, const int* skip);
 // seeded right-hand side of the handle's problem (U[-1,1) per local node,

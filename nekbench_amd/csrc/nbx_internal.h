@@ -50,6 +50,8 @@ int axhelm_pcg_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, i
 // double add gather-scatter of every field plus the Dirichlet mask, one launch
 int gs_add_mask_launch(const nbx_ogs* o, double* v, int k, int32_t stride, int32_t Nmasked,
                        const int32_t* maskIds, const int* skip, hipStream_t s);
+int gs_add_mask_launch(const nbx_ogs* o, float* v, int k, int32_t stride, int32_t Nmasked,
+                       const int32_t* maskIds, const int* skip, hipStream_t s);
 // group size per local node as bytes (null when a group exceeds 255); the
 // weight 1.0/(double)deg equals invDegree bit for bit
 const uint8_t* ogs_degree(const nbx_ogs* o);

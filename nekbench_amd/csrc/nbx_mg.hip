@@ -22,6 +22,12 @@
 // bit.  All work is enqueued on the handle's stream; every kernel honours the
 // outer solver's done flag so the iterations enqueued past convergence cost
 // nothing.
+//
+// Precision: NBX_F64 runs the whole cycle in FP64.  NBX_F32 and NBX_BF16 run
+// the smoothers and transfers of every level but the last in FP32 (vectors,
+// Jacobi diagonal, interpolation and arithmetic), with the operator's ggeo and
+// lambda in FP32 or bf16; the cycle reads the FP64 residual and returns an
+// FP64 correction, and the coarse PCG stays FP64.
 #include "nbx_bp_internal.h"
 #include "nbx_blas.h"
 #include <cmath>
@@ -33,14 +39,19 @@ struct nbx_mg_level {
   int N = 0, Nq = 0;
   double lmax = 0.0;
   std::vector<double> c1, c2;  // Chebyshev step coefficients
-  double *r = nullptr, *x = nullptr, *res = nullptr, *d = nullptr, *Ad = nullptr;
+  // vectors of the cycle's precision (double or float); the last level's r
+  // and x are double (its PCG); level 0 reads r and writes x of the solver
+  void *r = nullptr, *x = nullptr, *res = nullptr, *d = nullptr, *Ad = nullptr;
   double* J = nullptr;  // [Nq][Nq of level l+1]: interpolation from the next coarser level
+  float* J32 = nullptr;
+  float* invD32 = nullptr;  // FP32 Jacobi diagonal (FP32 / bf16 cycles)
 };
 
 struct nbx_mg {
   std::vector<nbx_mg_level> lv;
   int degree = 2, coarse_iters = 20;
   double lo = 0.1, hi = 1.1;
+  nbx_prec prec = NBX_F64;
   nbx::PcgScalars* coarse_template = nullptr;  // device copy of the coarse solve's initial scalars
   std::vector<void*> owned;
 };
@@ -49,116 +60,133 @@ namespace nbx {
 
 inline dim3 mg_grid(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
-// Chebyshev-Jacobi step (n local nodes of one field):
-//   MODE 0 (first pre-smoothing step, x = 0):  res = r; d = c2 D^-1 res; x = d
+template <typename V>
+__device__ __forceinline__ V fmaV(V a, V b, V c);
+template <>
+__device__ __forceinline__ double fmaV<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <>
+__device__ __forceinline__ float fmaV<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Chebyshev-Jacobi step (n local nodes of one field), arithmetic in V; the
+// level's right-hand side is R, its solution X (level 0: the solver's FP64
+// residual and correction):
+//   MODE 0 (first pre-smoothing step, x = 0):  res = (V)r; d = c2 D^-1 res; x = d
 //   MODE 1 (general step):  res -= Ad; d = c1 d + c2 D^-1 res; x += d
 //   MODE 2 (first post-smoothing step):  res -= Ad; d = c2 D^-1 res; x += d
-template <int MODE>
-__global__ __launch_bounds__(256) void k_cheb(const int32_t n, const double c1, const double c2,
-                                              const double* __restrict__ invD, const double* __restrict__ r,
-                                              double* __restrict__ res, const double* __restrict__ Ad,
-                                              double* __restrict__ d, double* __restrict__ x,
+template <int MODE, typename V, typename R, typename X>
+__global__ __launch_bounds__(256) void k_cheb(const int32_t n, const V c1, const V c2, const V* __restrict__ invD,
+                                              const R* __restrict__ r, V* __restrict__ res,
+                                              const V* __restrict__ Ad, V* __restrict__ d, X* __restrict__ x,
                                               const int* __restrict__ skip)
 {
   if (skip && *skip) return;
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double ri;
-  if (MODE == 0) ri = r[i];
+  V ri;
+  if (MODE == 0) ri = (V)r[i];
   else ri = res[i] - Ad[i];
   res[i] = ri;
-  const double t = c2 * (invD[i] * ri);
-  const double di = (MODE == 1) ? __builtin_fma(c1, d[i], t) : t;
+  const V t = c2 * (invD[i] * ri);
+  const V di = (MODE == 1) ? fmaV<V>(c1, d[i], t) : t;
   d[i] = di;
-  x[i] = (MODE == 0) ? di : x[i] + di;
+  x[i] = (MODE == 0) ? (X)di : x[i] + (X)di;
 }
 
 // Restriction of one element: res -= Ad (stored), u = w res, then
 // rc[a,b,c] = sum_k J[k][c] sum_j J[j][b] sum_i J[i][a] u[i,j,k]
 // (node index i + j Nq + k Nq^2; fma chains in ascending fine index).
-__global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nqc, const double* __restrict__ J,
-                                                     const double* __restrict__ w, double* __restrict__ res,
-                                                     const double* __restrict__ Ad, double* __restrict__ rc,
+template <typename V, typename O>
+__global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nqc, const V* __restrict__ J,
+                                                     const double* __restrict__ w, V* __restrict__ res,
+                                                     const V* __restrict__ Ad, O* __restrict__ rc,
                                                      const int* __restrict__ skip)
 {
   if (skip && *skip) return;
-  __shared__ double s_u[12 * 12 * 12];
-  __shared__ double s_t[12 * 12 * 12];
-  __shared__ double s_J[12 * 12];
+  __shared__ V s_u[12 * 12 * 12];
+  __shared__ V s_t[12 * 12 * 12];
+  __shared__ V s_J[12 * 12];
   const int64_t e = blockIdx.x;
   const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
   for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
   for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
     const int64_t id = e * Npf + q;
-    const double rq = res[id] - Ad[id];
+    const V rq = res[id] - Ad[id];
     res[id] = rq;
-    s_u[q] = w[id] * rq;
+    s_u[q] = (V)w[id] * rq;
   }
   __syncthreads();
   // x: t1[a + Nqc (j + Nqf k)]
   for (int q = threadIdx.x; q < Nqc * Nqf * Nqf; q += blockDim.x) {
     const int a = q % Nqc, jk = q / Nqc;
-    double acc = 0.0;
-    for (int i = 0; i < Nqf; ++i) acc = __builtin_fma(s_J[i * Nqc + a], s_u[i + Nqf * jk], acc);
+    V acc = V(0);
+    for (int i = 0; i < Nqf; ++i) acc = fmaV<V>(s_J[i * Nqc + a], s_u[i + Nqf * jk], acc);
     s_t[q] = acc;
   }
   __syncthreads();
   // y: t2[a + Nqc (b + Nqc k)] into s_u
   for (int q = threadIdx.x; q < Nqc * Nqc * Nqf; q += blockDim.x) {
     const int a = q % Nqc, b = (q / Nqc) % Nqc, k = q / (Nqc * Nqc);
-    double acc = 0.0;
-    for (int j = 0; j < Nqf; ++j) acc = __builtin_fma(s_J[j * Nqc + b], s_t[a + Nqc * (j + Nqf * k)], acc);
+    V acc = V(0);
+    for (int j = 0; j < Nqf; ++j) acc = fmaV<V>(s_J[j * Nqc + b], s_t[a + Nqc * (j + Nqf * k)], acc);
     s_u[q] = acc;
   }
   __syncthreads();
   // z: rc[a + Nqc (b + Nqc c)]
   for (int q = threadIdx.x; q < Npc; q += blockDim.x) {
     const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
-    double acc = 0.0;
-    for (int k = 0; k < Nqf; ++k) acc = __builtin_fma(s_J[k * Nqc + c], s_u[ab + Nqc * Nqc * k], acc);
-    rc[e * Npc + q] = acc;
+    V acc = V(0);
+    for (int k = 0; k < Nqf; ++k) acc = fmaV<V>(s_J[k * Nqc + c], s_u[ab + Nqc * Nqc * k], acc);
+    rc[e * Npc + q] = (O)acc;
   }
 }
 
 // Prolongation of one element: v[i,j,k] = sum_c J[k][c] sum_b J[j][b] sum_a J[i][a] xc[a,b,c];
 // d = v, x += v.
-__global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc, const double* __restrict__ J,
-                                                    const double* __restrict__ xc, double* __restrict__ d,
-                                                    double* __restrict__ x, const int* __restrict__ skip)
+template <typename V, typename I, typename X>
+__global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc, const V* __restrict__ J,
+                                                    const I* __restrict__ xc, V* __restrict__ d,
+                                                    X* __restrict__ x, const int* __restrict__ skip)
 {
   if (skip && *skip) return;
-  __shared__ double s_u[12 * 12 * 12];
-  __shared__ double s_t[12 * 12 * 12];
-  __shared__ double s_J[12 * 12];
+  __shared__ V s_u[12 * 12 * 12];
+  __shared__ V s_t[12 * 12 * 12];
+  __shared__ V s_J[12 * 12];
   const int64_t e = blockIdx.x;
   const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
   for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
-  for (int q = threadIdx.x; q < Npc; q += blockDim.x) s_u[q] = xc[e * Npc + q];
+  for (int q = threadIdx.x; q < Npc; q += blockDim.x) s_u[q] = (V)xc[e * Npc + q];
   __syncthreads();
   // x: t1[i + Nqf (b + Nqc c)]
   for (int q = threadIdx.x; q < Nqf * Nqc * Nqc; q += blockDim.x) {
     const int i = q % Nqf, bc = q / Nqf;
-    double acc = 0.0;
-    for (int a = 0; a < Nqc; ++a) acc = __builtin_fma(s_J[i * Nqc + a], s_u[a + Nqc * bc], acc);
+    V acc = V(0);
+    for (int a = 0; a < Nqc; ++a) acc = fmaV<V>(s_J[i * Nqc + a], s_u[a + Nqc * bc], acc);
     s_t[q] = acc;
   }
   __syncthreads();
   // y: t2[i + Nqf (j + Nqf c)] into s_u
   for (int q = threadIdx.x; q < Nqf * Nqf * Nqc; q += blockDim.x) {
     const int i = q % Nqf, j = (q / Nqf) % Nqf, c = q / (Nqf * Nqf);
-    double acc = 0.0;
-    for (int b = 0; b < Nqc; ++b) acc = __builtin_fma(s_J[j * Nqc + b], s_t[i + Nqf * (b + Nqc * c)], acc);
+    V acc = V(0);
+    for (int b = 0; b < Nqc; ++b) acc = fmaV<V>(s_J[j * Nqc + b], s_t[i + Nqf * (b + Nqc * c)], acc);
     s_u[q] = acc;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
     const int ij = q % (Nqf * Nqf), k = q / (Nqf * Nqf);
-    double acc = 0.0;
-    for (int c = 0; c < Nqc; ++c) acc = __builtin_fma(s_J[k * Nqc + c], s_u[ij + Nqf * Nqf * c], acc);
+    V acc = V(0);
+    for (int c = 0; c < Nqc; ++c) acc = fmaV<V>(s_J[k * Nqc + c], s_u[ij + Nqf * Nqf * c], acc);
     const int64_t id = e * Npf + q;
     d[id] = acc;
-    x[id] = x[id] + acc;
+    x[id] = x[id] + (X)acc;
   }
+}
+
+__global__ __launch_bounds__(256) void k_mg_to_f32(const int64_t n, const double* __restrict__ a,
+                                                   float* __restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)a[i];
 }
 
 __global__ __launch_bounds__(256) void k_mg_fill0(const int64_t n, double* __restrict__ v,
@@ -226,10 +254,11 @@ void cheb_coeffs(double lmax, double lo, double hi, int degree, std::vector<doub
   }
 }
 
-int mg_alloc(nbx_mg* mg, double** p, int64_t n)
+template <typename P>
+int mg_alloc(nbx_mg* mg, P** p, int64_t n, size_t elem = sizeof(P))
 {
-  NBX_HIP(hipMalloc((void**)p, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
-  NBX_HIP(hipMemset(*p, 0, sizeof(double) * (size_t)std::max<int64_t>(n, 1)));
+  NBX_HIP(hipMalloc((void**)p, elem * (size_t)std::max<int64_t>(n, 1)));
+  NBX_HIP(hipMemset(*p, 0, elem * (size_t)std::max<int64_t>(n, 1)));
   mg->owned.push_back(*p);
   return NBX_SUCCESS;
 }
@@ -277,56 +306,117 @@ int power_lmax(nbx_bp* bp, uint64_t seed, uint64_t stream_id, int iters, double*
 int coarse_solve(nbx_mg* mg, nbx_mg_level& L, const int* skip)
 {
   nbx_bp* bp = L.bp;
-  hipLaunchKernelGGL(k_mg_fill0, mg_grid(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, L.x, skip);
+  double* x = static_cast<double*>(L.x);
+  hipLaunchKernelGGL(k_mg_fill0, mg_grid(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, x, skip);
   hipLaunchKernelGGL(k_mg_reset, dim3(1), dim3(64), 0, bp->stream, bp->S, mg->coarse_template, skip);
   NBX_HIP_LAUNCH();
   for (int it = 0; it <= mg->coarse_iters; ++it) {
-    int rc = bp_enqueue_iteration(bp, bp->lambda, L.r, L.x);
+    int rc = bp_enqueue_iteration(bp, bp->lambda, static_cast<double*>(L.r), x);
     if (rc) return rc;
   }
   return NBX_SUCCESS;
 }
 
-int vcycle(nbx_mg* mg, int l, const double* r, double* x, const int* skip)
+// level operator Aq = mask(gs(A q)) in the cycle's precision
+int level_apply(nbx_mg* mg, nbx_mg_level& L, const double* q, double* Aq, const int* skip)
+{
+  return bp_apply(L.bp, L.bp->lambda, q, Aq, skip);
+}
+
+int level_apply(nbx_mg* mg, nbx_mg_level& L, const float* q, float* Aq, const int* skip)
+{
+  nbx_bp* bp = L.bp;
+  const bool bf = mg->prec == NBX_BF16;
+  int rc = axhelm_launch(bp->N, 1, bp->opt.bpmode ? 1 : 0, mg->prec, bp->E, bp->fieldOffset, nullptr,
+                         bf ? (const void*)bp->ggeo16 : (const void*)bp->ggeo32, bp->D32,
+                         bf ? (const void*)bp->lambda16 : (const void*)bp->lambda32, q, Aq, skip, bp->stream);
+  if (rc) return rc;
+  return gs_add_mask_launch(bp->ogs, Aq, 1, bp->fieldOffset, bp->Nmasked, bp->maskIds, skip, bp->stream);
+}
+
+template <typename V>
+const V* level_J(const nbx_mg_level& L);
+template <>
+const double* level_J<double>(const nbx_mg_level& L) { return L.J; }
+template <>
+const float* level_J<float>(const nbx_mg_level& L) { return L.J32; }
+
+template <typename V>
+const V* level_invD(const nbx_mg_level& L);
+template <>
+const double* level_invD<double>(const nbx_mg_level& L) { return L.bp->invDiagA; }
+template <>
+const float* level_invD<float>(const nbx_mg_level& L) { return L.invD32; }
+
+template <typename V>
+int vcycle(nbx_mg* mg, int l, const void* r, void* x, const int* skip);
+
+// one level of the cycle: right-hand side R, solution X, coarse vectors C
+template <typename V, typename R, typename X, typename CV>
+int vcycle_level(nbx_mg* mg, int l, const R* r, X* x, const int* skip)
 {
   nbx_mg_level& L = mg->lv[l];
-  if (l + 1 == (int)mg->lv.size()) return coarse_solve(mg, L, skip);
   nbx_mg_level& C = mg->lv[l + 1];
   nbx_bp* bp = L.bp;
   const int32_t n = bp->Nlocal;
   const dim3 g = mg_grid(n), b(256);
   const int k = mg->degree;
+  V* res = static_cast<V*>(L.res);
+  V* d = static_cast<V*>(L.d);
+  V* Ad = static_cast<V*>(L.Ad);
+  const V* invD = level_invD<V>(L);
   int rc;
   // pre-smoothing from x = 0
-  hipLaunchKernelGGL(k_cheb<0>, g, b, 0, bp->stream, n, 0.0, L.c2[0], bp->invDiagA, r, L.res, nullptr, L.d, x, skip);
+  hipLaunchKernelGGL((k_cheb<0, V, R, X>), g, b, 0, bp->stream, n, V(0), (V)L.c2[0], invD, r, res,
+                     (const V*)nullptr, d, x, skip);
   for (int s = 1; s < k; ++s) {
-    if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
-    hipLaunchKernelGGL(k_cheb<1>, g, b, 0, bp->stream, n, L.c1[s], L.c2[s], bp->invDiagA, nullptr, L.res, L.Ad,
-                       L.d, x, skip);
+    if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
+    hipLaunchKernelGGL((k_cheb<1, V, R, X>), g, b, 0, bp->stream, n, (V)L.c1[s], (V)L.c2[s], invD,
+                       (const R*)nullptr, res, Ad, d, x, skip);
   }
-  if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
+  if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
   // restriction of the smoothed residual
-  hipLaunchKernelGGL(k_mg_restrict, dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, L.J, nbx_ogs_inv_degree(bp->ogs),
-                     L.res, L.Ad, C.r, skip);
+  CV* rcv = static_cast<CV*>(C.r);
+  hipLaunchKernelGGL((k_mg_restrict<V, CV>), dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, level_J<V>(L),
+                     nbx_ogs_inv_degree(bp->ogs), res, Ad, rcv, skip);
   NBX_HIP_LAUNCH();
   nbx_bp* bc = C.bp;
-  if ((rc = gs_add_mask_launch(bc->ogs, C.r, 1, bc->fieldOffset, bc->Nmasked, bc->maskIds, skip, bc->stream)))
+  if ((rc = gs_add_mask_launch(bc->ogs, rcv, 1, bc->fieldOffset, bc->Nmasked, bc->maskIds, skip, bc->stream)))
     return rc;
-  if ((rc = vcycle(mg, l + 1, C.r, C.x, skip))) return rc;
+  if ((rc = vcycle<V>(mg, l + 1, C.r, C.x, skip))) return rc;
   // coarse correction
-  hipLaunchKernelGGL(k_mg_prolong, dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, L.J, C.x, L.d, x, skip);
+  hipLaunchKernelGGL((k_mg_prolong<V, CV, X>), dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, level_J<V>(L),
+                     static_cast<const CV*>(C.x), d, x, skip);
   NBX_HIP_LAUNCH();
-  if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
+  if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
   // post-smoothing
-  hipLaunchKernelGGL(k_cheb<2>, g, b, 0, bp->stream, n, 0.0, L.c2[0], bp->invDiagA, nullptr, L.res, L.Ad, L.d, x,
-                     skip);
+  hipLaunchKernelGGL((k_cheb<2, V, R, X>), g, b, 0, bp->stream, n, V(0), (V)L.c2[0], invD, (const R*)nullptr, res,
+                     Ad, d, x, skip);
   for (int s = 1; s < k; ++s) {
-    if ((rc = bp_apply(bp, bp->lambda, L.d, L.Ad, skip))) return rc;
-    hipLaunchKernelGGL(k_cheb<1>, g, b, 0, bp->stream, n, L.c1[s], L.c2[s], bp->invDiagA, nullptr, L.res, L.Ad,
-                       L.d, x, skip);
+    if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
+    hipLaunchKernelGGL((k_cheb<1, V, R, X>), g, b, 0, bp->stream, n, (V)L.c1[s], (V)L.c2[s], invD,
+                       (const R*)nullptr, res, Ad, d, x, skip);
   }
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
+}
+
+template <typename V>
+int vcycle(nbx_mg* mg, int l, const void* r, void* x, const int* skip)
+{
+  const int L = (int)mg->lv.size();
+  if (l + 1 == L) return coarse_solve(mg, mg->lv[l], skip);
+  const bool coarse_next = l + 2 == L;  // the next level is the FP64 coarse PCG
+  if (l == 0) {
+    const double* rr = static_cast<const double*>(r);
+    double* xx = static_cast<double*>(x);
+    return coarse_next ? vcycle_level<V, double, double, double>(mg, l, rr, xx, skip)
+                       : vcycle_level<V, double, double, V>(mg, l, rr, xx, skip);
+  }
+  const V* rr = static_cast<const V*>(r);
+  V* xx = static_cast<V*>(x);
+  return coarse_next ? vcycle_level<V, V, V, double>(mg, l, rr, xx, skip)
+                     : vcycle_level<V, V, V, V>(mg, l, rr, xx, skip);
 }
 
 }  // namespace
@@ -347,17 +437,20 @@ void mg_release(nbx_mg* mg)
 int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip)
 {
   if (!bp->mg) return fail(NBX_ERR_INVALID, "multigrid preconditioner: not set up (nbx_bp_mg_setup)");
-  return vcycle(bp->mg, 0, r, z, skip);
+  if (bp->mg->prec == NBX_F64) return vcycle<double>(bp->mg, 0, r, z, skip);
+  return vcycle<float>(bp->mg, 0, r, z, skip);
 }
 
 }  // namespace nbx
 
 extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters,
-                               double lo, double hi)
+                               double lo, double hi, nbx_prec prec)
 {
   if (!bp || nlevels < 2 || nlevels > 8 || !orders || degree < 1 || degree > 16 || coarse_iters < 1 ||
       !(lo > 0) || !(hi > lo))
     return fail(NBX_ERR_INVALID, "bp_mg_setup: bad arguments");
+  if (prec != NBX_F64 && prec != NBX_F32 && prec != NBX_BF16)
+    return fail(NBX_ERR_UNSUPPORTED, "bp_mg_setup: precision %d (F64, F32 or BF16)", (int)prec);
   if (bp->Nf != 1) return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: one field");
   if (orders[0] != bp->N) return fail(NBX_ERR_INVALID, "bp_mg_setup: orders[0] must be the handle's N");
   for (int l = 1; l < nlevels; ++l)
@@ -373,7 +466,9 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
   mg->coarse_iters = coarse_iters;
   mg->lo = lo;
   mg->hi = hi;
+  mg->prec = prec;
   mg->lv.resize(nlevels);
+  const size_t es = prec == NBX_F64 ? sizeof(double) : sizeof(float);
   int rc = NBX_SUCCESS;
   for (int l = 0; l < nlevels && rc == NBX_SUCCESS; ++l) {
     nbx_mg_level& L = mg->lv[l];
@@ -397,19 +492,35 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
     }
     nbx_bp* b = L.bp;
     if ((rc = bp_update_jacobi(b, b->lambda))) break;
-    if ((rc = mg_alloc(mg, &L.res, b->Nall)) || (rc = mg_alloc(mg, &L.d, b->Nall)) ||
-        (rc = mg_alloc(mg, &L.Ad, b->Nall)))
+    const bool last = l + 1 == nlevels;
+    if (!last && ((rc = mg_alloc(mg, &L.res, b->Nall, es)) || (rc = mg_alloc(mg, &L.d, b->Nall, es)) ||
+                  (rc = mg_alloc(mg, &L.Ad, b->Nall, es))))
       break;
-    if (l > 0 && ((rc = mg_alloc(mg, &L.r, b->Nall)) || (rc = mg_alloc(mg, &L.x, b->Nall)))) break;
-    if (l + 1 < nlevels) {
+    if (l > 0 && ((rc = mg_alloc(mg, &L.r, b->Nall, last ? sizeof(double) : es)) ||
+                  (rc = mg_alloc(mg, &L.x, b->Nall, last ? sizeof(double) : es))))
+      break;
+    if (!last) {
       const std::vector<double> J = interp_matrix(orders[l], orders[l + 1]);
       if ((rc = mg_alloc(mg, &L.J, (int64_t)J.size()))) break;
       if (hipMemcpy(L.J, J.data(), sizeof(double) * J.size(), hipMemcpyHostToDevice) != hipSuccess) {
         rc = fail(NBX_ERR_HIP, "bp_mg_setup: copy of J");
         break;
       }
-      if ((rc = power_lmax(b, bp->opt.box.seed, 17 + (uint64_t)l, 20, L.d, L.Ad, &L.lmax))) break;
+      // lambda_max of the FP64 operator (the handle's z and Ax as scratch)
+      if ((rc = power_lmax(b, bp->opt.box.seed, 17 + (uint64_t)l, 20, b->z, b->Ax, &L.lmax))) break;
       cheb_coeffs(L.lmax, lo, hi, degree, L.c1, L.c2);
+      if (prec != NBX_F64) {
+        if ((rc = bp_prepare_precision(b, prec, b->lambda))) break;
+        std::vector<float> J32(J.size());
+        for (size_t q = 0; q < J.size(); ++q) J32[q] = (float)J[q];
+        if ((rc = mg_alloc(mg, &L.J32, (int64_t)J.size())) || (rc = mg_alloc(mg, &L.invD32, b->Nlocal))) break;
+        if (hipMemcpy(L.J32, J32.data(), sizeof(float) * J32.size(), hipMemcpyHostToDevice) != hipSuccess) {
+          rc = fail(NBX_ERR_HIP, "bp_mg_setup: copy of J");
+          break;
+        }
+        hipLaunchKernelGGL(k_mg_to_f32, mg_grid(b->Nlocal), dim3(256), 0, b->stream, (int64_t)b->Nlocal,
+                           b->invDiagA, L.invD32);
+      }
     } else {
       // coarse solve: fixed-count Jacobi PCG with device-resident scalars
       if (b->hist_cap < coarse_iters + 2) {

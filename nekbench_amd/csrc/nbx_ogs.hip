@@ -143,8 +143,9 @@ struct GsClasses {
   const int32_t* off;    // CSR offsets
 };
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const int k, const int32_t stride,
-                                                     double* __restrict__ q, const int* __restrict__ skip)
+                                                     T* __restrict__ q, const int* __restrict__ skip)
 {
   if (skip && *skip) return;
   const int32_t xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
@@ -155,15 +156,15 @@ __global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const in
   if (item >= min(C.n[c], (xcd + 1) * C.per[c])) return;
   if (c == 4) {
     const int32_t id = C.ids[4][item];
-    for (int f = 0; f < k; ++f) q[id + (int64_t)f * stride] = 0.0;
+    for (int f = 0; f < k; ++f) q[id + (int64_t)f * stride] = T(0);
     return;
   }
   if (c == 3) {
     const int32_t s0 = C.off[item], e0 = C.off[item + 1];
     const int32_t* idg = C.ids[3];
     for (int f = 0; f < k; ++f) {
-      double* qf = q + (int64_t)f * stride;
-      double acc = 0.0;
+      T* qf = q + (int64_t)f * stride;
+      T acc = T(0);
       for (int32_t n = s0; n < e0; ++n) acc = acc + qf[idg[n]];
       for (int32_t n = s0; n < e0; ++n) qf[idg[n]] = acc;
     }
@@ -185,8 +186,8 @@ __global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const in
     }
   }
   for (int f = 0; f < k; ++f) {
-    double* qf = q + (int64_t)f * stride;
-    double acc = 0.0;
+    T* qf = q + (int64_t)f * stride;
+    T acc = T(0);
     for (int n = 0; n < deg; ++n) acc = acc + qf[m[n]];
     for (int n = 0; n < deg; ++n) qf[m[n]] = acc;
   }
@@ -247,8 +248,9 @@ int gs_launch(const nbx_ogs* o, void* v, int k, int32_t stride, nbx_type type, n
   }
 }
 
-int gs_add_mask_launch(const nbx_ogs* o, double* v, int k, int32_t stride, int32_t Nmasked,
-                       const int32_t* maskIds, const int* skip, hipStream_t s)
+template <typename T>
+static int gs_add_mask_launch_t(const nbx_ogs* o, T* v, int k, int32_t stride, int32_t Nmasked,
+                                const int32_t* maskIds, const int* skip, hipStream_t s)
 {
   GsClasses C{};
   const int32_t n[5] = {o->Nfix[0], o->Nfix[1], o->Nfix[2], o->Ngen, Nmasked};
@@ -262,9 +264,21 @@ int gs_add_mask_launch(const nbx_ogs* o, double* v, int k, int32_t stride, int32
   }
   C.off = o->d_offsets;
   if (C.bstart[5] == 0) return NBX_SUCCESS;
-  hipLaunchKernelGGL(k_gs_add_mask, dim3((unsigned)(8 * C.bstart[5])), dim3(256), 0, s, C, k, stride, v, skip);
+  hipLaunchKernelGGL(k_gs_add_mask<T>, dim3((unsigned)(8 * C.bstart[5])), dim3(256), 0, s, C, k, stride, v, skip);
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
+}
+
+int gs_add_mask_launch(const nbx_ogs* o, double* v, int k, int32_t stride, int32_t Nmasked,
+                       const int32_t* maskIds, const int* skip, hipStream_t s)
+{
+  return gs_add_mask_launch_t<double>(o, v, k, stride, Nmasked, maskIds, skip, s);
+}
+
+int gs_add_mask_launch(const nbx_ogs* o, float* v, int k, int32_t stride, int32_t Nmasked,
+                       const int32_t* maskIds, const int* skip, hipStream_t s)
+{
+  return gs_add_mask_launch_t<float>(o, v, k, stride, Nmasked, maskIds, skip, s);
 }
 
 const uint8_t* ogs_degree(const nbx_ogs* o) { return o ? o->d_degree : nullptr; }

@@ -149,6 +149,12 @@ typedef struct nbo_mg {
   const double* c1;            /* [nlev][degree] Chebyshev coefficients */
   const double* c2;
   int coarse_iters;            /* Jacobi-PCG iterations on the last level */
+  int prec;                    /* 0: FP64 cycle; 1 FP32, 2 bf16 geometry: FP32 cycle but the last level */
+  const float* const* ggeo32;  /* per level: ggeo as FP32 (or bf16-rounded) values */
+  const float* const* lam32;   /* per level: lambda (2 offset) likewise */
+  const float* const* D32;
+  const float* const* invD32;
+  const float* const* J32;
 } nbo_mg;
 void nbo_mg_interp(int Nf, int Nc, double* J);
 void nbo_mg_cheb(double lmax, double lo, double hi, int degree, double* c1, double* c2);

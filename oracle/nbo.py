@@ -335,7 +335,9 @@ def bf16_round(x):
 
 class MGStruct(C.Structure):
     _fields_ = [("nlev", C.c_int), ("bps", C.c_void_p), ("invD", C.c_void_p), ("J", C.c_void_p),
-                ("degree", C.c_int), ("c1", C.c_void_p), ("c2", C.c_void_p), ("coarse_iters", C.c_int)]
+                ("degree", C.c_int), ("c1", C.c_void_p), ("c2", C.c_void_p), ("coarse_iters", C.c_int),
+                ("prec", C.c_int), ("ggeo32", C.c_void_p), ("lam32", C.c_void_p), ("D32", C.c_void_p),
+                ("invD32", C.c_void_p), ("J32", C.c_void_p)]
 
 
 class OracleMG:
@@ -344,7 +346,7 @@ class OracleMG:
     matrices, power-iteration lambda_max and Chebyshev coefficients."""
 
     def __init__(self, box: Box, orders, degree=2, coarse_iters=20, lo=0.1, hi=1.1, bpmode=True, lambda1=0.0,
-                 dirichlet=True, power_iters=20):
+                 dirichlet=True, power_iters=20, prec=0):
         assert orders[0] == box.N
         self.levels = []
         for N in orders:
@@ -376,8 +378,20 @@ class OracleMG:
             lib().nbo_mg_cheb(C.c_double(lam), C.c_double(lo), C.c_double(hi), degree, p(c1), p(c2))
             self.c1[l * degree:(l + 1) * degree] = c1
             self.c2[l * degree:(l + 1) * degree] = c2
+        # FP32 / bf16-geometry cycle: float copies (double -> float, bf16 by RNE of the float)
+        def lo_vals(a):
+            f = np.ascontiguousarray(a, dtype=np.float32)
+            return bf16_round(f) if prec == 2 else f
+        self.f32 = []
+        for l, lv in enumerate(self.levels):
+            self.f32.append((lo_vals(lv.mesh["ggeo"]), lo_vals(lv.lam), np.ascontiguousarray(lv.mesh["D"], np.float32),
+                             self.invD[l].astype(np.float32),
+                             self.J[l].astype(np.float32) if l < n - 1 else np.zeros(1, np.float32)))
+        arr = lambda i: (C.c_void_p * n)(*[t[i].ctypes.data for t in self.f32])
+        self.f32_ptrs = [arr(i) for i in range(5)]
         self.struct = MGStruct(n, C.addressof(self.bps), C.addressof(self.invD_ptrs), C.addressof(self.J_ptrs),
-                               degree, self.c1.ctypes.data, self.c2.ctypes.data, coarse_iters)
+                               degree, self.c1.ctypes.data, self.c2.ctypes.data, coarse_iters, prec,
+                               *[C.addressof(a) for a in self.f32_ptrs])
 
     def vcycle(self, r):
         x = np.zeros_like(r)

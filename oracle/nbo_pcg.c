@@ -428,9 +428,15 @@ double nbo_mg_power(const nbo_bp* bp, const double* invD, uint64_t seed, uint64_
   return lam;
 }
 
+static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x);
+
 /* V-cycle at level l: x = M_l r (x overwritten) */
 void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
 {
+  if (mg->prec != 0 && l < mg->nlev - 1) {
+    mg_vcycle_f(mg, l, r, x);
+    return;
+  }
   const nbo_bp* bp = &mg->bps[l];
   const int Nq = bp->N + 1, Np = Nq * Nq * Nq;
   const int64_t n = (int64_t)bp->E * Np;
@@ -567,4 +573,157 @@ int nbo_bp_pcg_mg(const nbo_mg* mg, double* r, double* x, double tol, int maxit,
   const learned_ctx ln = {NULL, NULL, 0, 0.0, mg};
   return pcg_core_ext(&mg->bps[0], 1, 1, 1, NULL, mg->invD[0], &ln, r, x, tol, tol * tol, maxit, 1, hist_rdotr,
                       hist_alpha, st);
+}
+
+/* ---- FP32 cycle (library: vcycle<float> in nbx_mg.hip) -------------------
+ * Every level but the last runs in float: Jacobi diagonal, interpolation,
+ * Chebyshev coefficients rounded to float, FP32 operator (nbo_axhelm_f32 with
+ * FP32 or bf16-valued geometry), float gather-scatter.  Level 0 reads the
+ * double residual and accumulates the double correction; the transfers to
+ * and from the last level (FP64 PCG) convert. */
+static void mg_apply_f(const nbo_mg* mg, int l, const float* q, float* Aq)
+{
+  const nbo_bp* bp = &mg->bps[l];
+  nbo_axhelm_f32(bp->N, 1, bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
+  nbo_gs_many(bp->ogs, Aq, 1, bp->offset, NBO_FLOAT, NBO_ADD);
+  for (int32_t m = 0; m < bp->Nmasked; ++m) Aq[bp->maskIds[m]] = 0.f;
+}
+
+static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
+{
+  const nbo_bp* bp = &mg->bps[l];
+  const nbo_bp* bc = &mg->bps[l + 1];
+  const int Nq = bp->N + 1, Np = Nq * Nq * Nq, Nqc = bc->N + 1, Npc = Nqc * Nqc * Nqc;
+  const int64_t n = (int64_t)bp->E * Np;
+  const int rdbl = l == 0, cnext = l + 2 == mg->nlev;
+  const float* invD = mg->invD32[l];
+  const float* J = mg->J32[l];
+  const int k = mg->degree;
+  const double* c1 = mg->c1 + (size_t)l * k;
+  const double* c2 = mg->c2 + (size_t)l * k;
+  const double* rD = (const double*)r;
+  const float* rF = (const float*)r;
+  double* xD = (double*)x;
+  float* xF = (float*)x;
+  float* res = (float*)calloc(bp->offset, sizeof(float));
+  float* d = (float*)calloc(bp->offset, sizeof(float));
+  float* Ad = (float*)calloc(bp->offset, sizeof(float));
+  const size_t ces = cnext ? sizeof(double) : sizeof(float);
+  void* rc = calloc(bc->offset, ces);
+  void* xc = calloc(bc->offset, ces);
+#define XADD(i, v) do { if (rdbl) xD[i] = xD[i] + (double)(v); else xF[i] = xF[i] + (v); } while (0)
+  {
+    const float c2f = (float)c2[0];
+    for (int64_t i = 0; i < n; ++i) {
+      const float ri = rdbl ? (float)rD[i] : rF[i];
+      res[i] = ri;
+      const float t = c2f * (invD[i] * ri);
+      d[i] = t;
+      if (rdbl) xD[i] = (double)t; else xF[i] = t;
+    }
+  }
+  for (int s = 1; s < k; ++s) {
+    mg_apply_f(mg, l, d, Ad);
+    const float c1f = (float)c1[s], c2f = (float)c2[s];
+    for (int64_t i = 0; i < n; ++i) {
+      const float ri = res[i] - Ad[i];
+      res[i] = ri;
+      const float t = c2f * (invD[i] * ri);
+      const float di = fmaf(c1f, d[i], t);
+      d[i] = di;
+      XADD(i, di);
+    }
+  }
+  mg_apply_f(mg, l, d, Ad);
+#pragma omp parallel
+  {
+    float* u = (float*)malloc(sizeof(float) * Np);
+    float* t1 = (float*)malloc(sizeof(float) * Np);
+    float* t2 = (float*)malloc(sizeof(float) * Np);
+#pragma omp for schedule(static)
+    for (int32_t e = 0; e < bp->E; ++e) {
+      for (int q = 0; q < Np; ++q) {
+        const int64_t id = (int64_t)e * Np + q;
+        const float rq = res[id] - Ad[id];
+        res[id] = rq;
+        u[q] = (float)bp->w[id] * rq;
+      }
+      for (int q = 0; q < Nqc * Nq * Nq; ++q) {
+        const int a = q % Nqc, jk = q / Nqc;
+        float acc = 0.f;
+        for (int i = 0; i < Nq; ++i) acc = fmaf(J[i * Nqc + a], u[i + Nq * jk], acc);
+        t1[q] = acc;
+      }
+      for (int q = 0; q < Nqc * Nqc * Nq; ++q) {
+        const int a = q % Nqc, b = (q / Nqc) % Nqc, kk = q / (Nqc * Nqc);
+        float acc = 0.f;
+        for (int j = 0; j < Nq; ++j) acc = fmaf(J[j * Nqc + b], t1[a + Nqc * (j + Nq * kk)], acc);
+        t2[q] = acc;
+      }
+      for (int q = 0; q < Npc; ++q) {
+        const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
+        float acc = 0.f;
+        for (int kk = 0; kk < Nq; ++kk) acc = fmaf(J[kk * Nqc + c], t2[ab + Nqc * Nqc * kk], acc);
+        if (cnext) ((double*)rc)[(int64_t)e * Npc + q] = (double)acc;
+        else ((float*)rc)[(int64_t)e * Npc + q] = acc;
+      }
+    }
+    free(u); free(t1); free(t2);
+  }
+  if (cnext) {
+    nbo_gs_many(bc->ogs, rc, 1, bc->offset, NBO_DOUBLE, NBO_ADD);
+    for (int32_t m = 0; m < bc->Nmasked; ++m) ((double*)rc)[bc->maskIds[m]] = 0.0;
+    nbo_mg_vcycle(mg, l + 1, (const double*)rc, (double*)xc);
+  } else {
+    nbo_gs_many(bc->ogs, rc, 1, bc->offset, NBO_FLOAT, NBO_ADD);
+    for (int32_t m = 0; m < bc->Nmasked; ++m) ((float*)rc)[bc->maskIds[m]] = 0.f;
+    mg_vcycle_f(mg, l + 1, rc, xc);
+  }
+#pragma omp parallel
+  {
+    float* u = (float*)malloc(sizeof(float) * Npc);
+    float* t1 = (float*)malloc(sizeof(float) * Np);
+    float* t2 = (float*)malloc(sizeof(float) * Np);
+#pragma omp for schedule(static)
+    for (int32_t e = 0; e < bp->E; ++e) {
+      for (int q = 0; q < Npc; ++q)
+        u[q] = cnext ? (float)((double*)xc)[(int64_t)e * Npc + q] : ((float*)xc)[(int64_t)e * Npc + q];
+      for (int q = 0; q < Nq * Nqc * Nqc; ++q) {
+        const int i = q % Nq, bc_ = q / Nq;
+        float acc = 0.f;
+        for (int a = 0; a < Nqc; ++a) acc = fmaf(J[i * Nqc + a], u[a + Nqc * bc_], acc);
+        t1[q] = acc;
+      }
+      for (int q = 0; q < Nq * Nq * Nqc; ++q) {
+        const int i = q % Nq, j = (q / Nq) % Nq, c = q / (Nq * Nq);
+        float acc = 0.f;
+        for (int b = 0; b < Nqc; ++b) acc = fmaf(J[j * Nqc + b], t1[i + Nq * (b + Nqc * c)], acc);
+        t2[q] = acc;
+      }
+      for (int q = 0; q < Np; ++q) {
+        const int ij = q % (Nq * Nq), kk = q / (Nq * Nq);
+        float acc = 0.f;
+        for (int c = 0; c < Nqc; ++c) acc = fmaf(J[kk * Nqc + c], t2[ij + Nq * Nq * c], acc);
+        const int64_t id = (int64_t)e * Np + q;
+        d[id] = acc;
+        XADD(id, acc);
+      }
+    }
+    free(u); free(t1); free(t2);
+  }
+  mg_apply_f(mg, l, d, Ad);
+  for (int s = 0; s < k; ++s) {
+    if (s > 0) mg_apply_f(mg, l, d, Ad);
+    const float c1f = (float)c1[s], c2f = (float)c2[s];
+    for (int64_t i = 0; i < n; ++i) {
+      const float ri = res[i] - Ad[i];
+      res[i] = ri;
+      const float t = c2f * (invD[i] * ri);
+      const float di = s == 0 ? t : fmaf(c1f, d[i], t);
+      d[i] = di;
+      XADD(i, di);
+    }
+  }
+#undef XADD
+  free(res); free(d); free(Ad); free(rc); free(xc);
 }

@@ -1,6 +1,7 @@
 """p-multigrid preconditioner on the GPU against the oracle (OracleMG):
 power-iteration lambda_max per level, one V-cycle, and the MG-preconditioned
-flexible PCG iterate for iterate, all bit for bit; argument checks."""
+flexible PCG iterate for iterate, all bit for bit, for the FP64 cycle and the
+FP32 / bf16-geometry cycles; argument checks."""
 import numpy as np
 import pytest
 
@@ -18,20 +19,24 @@ CASES = [  # (N, dims, orders, degree, coarse_iters, bpmode, lambda1)
 ]
 
 
-def make(nbx, nbo, case):
+def make(nbx, nbo, case, prec=0):
     N, dims, orders, deg, ci, bpmode, l1 = case
     kw = dict(perturb=0.1, seed=50 + N)
     bp = nbx.BP(nbx.Box.make(N, *dims, **kw), bpmode=bpmode, lambda1=l1, precon=nbx.PRECON_MG)
-    bp.mg_setup(orders, deg, ci)
+    bp.mg_setup(orders, deg, ci, prec=prec)
     mg = nbo.OracleMG(nbo.Box.make(N, *dims, **kw), orders, degree=deg, coarse_iters=ci, bpmode=bpmode,
-                      lambda1=l1)
+                      lambda1=l1, prec=prec)
     return bp, mg
 
 
+PRECS = {"f64": 0, "f32": 1, "bf16": 2}
+
+
+@pytest.mark.parametrize("prec", list(PRECS))
 @pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3"])
-def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case):
+def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
     import torch
-    bp, mg = make(nbx, nbo, case)
+    bp, mg = make(nbx, nbo, case, PRECS[prec])
     orders = case[2]
     for l in range(len(orders) - 1):
         N, lam = bp.mg_info(l)
@@ -44,10 +49,11 @@ def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case):
     assert np.array_equal(H(z), mg.vcycle(r))
 
 
+@pytest.mark.parametrize("prec", list(PRECS))
 @pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3"])
-def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case):
+def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case, prec):
     import torch
-    bp, mg = make(nbx, nbo, case)
+    bp, mg = make(nbx, nbo, case, PRECS[prec])
     n = bp.fieldOffset
     r = bp.device_array("rhs", n).clone()
     x = torch.zeros_like(r)
@@ -86,6 +92,8 @@ def test_mg_argument_checks(nbx, dev):
     for orders in ([4, 2, 1], [5, 5, 1], [5, 2, 0], [5]):
         with pytest.raises(nbx.NbxError):
             bp.mg_setup(orders)
+    with pytest.raises(nbx.NbxError):
+        bp.mg_setup([5, 2, 1], prec=nbx.NBX_F16)
     b3 = nbx.BP(nbx.Box.make(5, 2, 2, 2), Nfields=3, precon=nbx.PRECON_MG)
     with pytest.raises(nbx.NbxError):
         b3.mg_setup([5, 2, 1])
