@@ -102,6 +102,196 @@ def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None, mg_cfg=None):
     return out
 
 
+def field_errors(x, x64, Nf, offset, nlocal):
+    xs, rs = x.view(-1)[:Nf * offset].view(Nf, offset)[:, :nlocal], x64.view(-1)[:Nf * offset].view(Nf, offset)[:, :nlocal]
+    return [float((xs[f] - rs[f]).abs().max() / rs[f].abs().max()) for f in range(Nf)]
+
+
+def timed(fn, sync):
+    sync()
+    t0 = time.perf_counter()
+    st = fn()
+    sync()
+    return st, time.perf_counter() - t0
+
+
+def run_config3(nbx, torch, args):
+    """BP6 vector Helmholtz, 3 fields, lambda0 = 1, lambda1 = 0.1 (axhelm_n3),
+    N = 9, 24^3 elements, perturbed Dirichlet box; FP64 flexible Jacobi PCG and
+    the FP32 / bf16-geometry iterative-refinement solvers to 1e-10; error of
+    each field against the FP64 solution."""
+    box = nbx.Box.make(9, 24, 24, 24, perturb=0.1, seed=1)
+    t0 = time.perf_counter()
+    bp = nbx.BP(box, Nfields=3, bpmode=False, lambda1=0.1, precon=nbx.PRECON_JACOBI, flexible=True, fused=True)
+    log(f"[bench3] setup {time.perf_counter() - t0:.1f} s E={bp.Nelements} Np={bp.Np}")
+    n = 3 * bp.fieldOffset
+    b = bp.device_array("rhs", n).clone()
+    r, x = torch.empty_like(b), torch.empty_like(b)
+    sync = torch.cuda.synchronize
+    out = {}
+    x64 = None
+    runs = [("fp64_jacobi", lambda: bp.pcg(r, x, 1e-10, 20000)),
+            ("mixed_f32_jacobi", lambda: bp.pcg_mixed(r, x, 1e-10, 20000, prec=nbx.NBX_F32, inner_tol=1e-5,
+                                                      max_cycles=30)),
+            ("mixed_bf16_jacobi", lambda: bp.pcg_mixed(r, x, 1e-10, 20000, prec=nbx.NBX_BF16, inner_tol=2e-2,
+                                                       max_cycles=80))]
+    for name, fn in runs:
+        for _ in range(2):  # the second run is timed
+            r.copy_(b)
+            x.zero_()
+            st, t = timed(fn, sync)
+        rec = {"time_s": round(t, 5), "iterations": st.iterations, "cycles": st.cycles,
+               "converged": bool(st.converged), "rel_residual": (st.rdotr / st.rdotr0) ** 0.5}
+        if x64 is None:
+            x64 = x.clone()
+        else:
+            rec["err_vs_fp64_per_field"] = field_errors(x, x64, 3, bp.fieldOffset, bp.Nlocal)
+        out[name] = rec
+        log(f"[bench3] {name}: {rec}")
+    for v in out.values():
+        v["speedup_vs_fp64_jacobi"] = round(out["fp64_jacobi"]["time_s"] / v["time_s"], 3)
+    return {"config": "BP6 vector Helmholtz N=9, 24^3 elements, 3 fields, lambda1=0.1, tol 1e-10",
+            "elements": bp.Nelements, "solvers": out}
+
+
+def train_net(nbx, torch, bp, nseeds, snapshots, steps, lr, log, seed0=1000, mode=1, select_seed=None,
+              chunk=250):
+    """Pairs from seeded FP64 Jacobi PCG solves, then energy-norm training in
+    chunks; with select_seed, the parameters after the chunk whose learned
+    PCG needs the fewest iterations on that (held-out) right-hand side are
+    kept (model selection on a validation solve)."""
+    net = nbx.Net(seed=1)
+    net.attach(bp)
+    t0 = time.perf_counter()
+    k = net.generate_pairs(bp, nseeds=nseeds, seed0=seed0, snapshots=snapshots, tol_exact=1e-11)
+    t_pairs = time.perf_counter() - t0
+    hist, sel = [], []
+    best = None
+    if select_seed is not None:
+        n = bp.fieldOffset
+        vb = torch.empty(n, dtype=torch.float64, device="cuda")
+        bp.make_rhs(select_seed, vb)
+        vr, vx = torch.empty_like(vb), torch.empty_like(vb)
+        bp.set_fallback(10 ** 6, 0.9)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        m = min(chunk, steps - done)
+        hist.extend(net.train_energy(bp, steps=m, lr=lr, seed=7 + done, mode=mode).tolist())
+        done += m
+        if select_seed is not None:
+            vr.copy_(vb)
+            vx.zero_()
+            its = bp.pcg(vr, vx, 1e-8, 3000).iterations
+            sel.append((done, its))
+            if best is None or its < best[1]:
+                best = (done, its, net.params())
+    if best is not None:
+        net.set_params(best[2])
+        bp.set_fallback(20, 0.9)
+    t_train = time.perf_counter() - t0
+    import numpy as np
+    hist = np.array(hist)
+    log(f"[bench] learned: {k} pairs in {t_pairs:.1f} s, {steps} energy steps (mode {mode}) in {t_train:.1f} s, "
+        f"loss {hist[:20].mean():.4f} -> {hist[-100:].mean():.4f}; selection {sel}")
+    return net, {"pairs": k, "pair_seeds": [seed0, seed0 + nseeds - 1], "snapshots": snapshots,
+                 "pairs_time_s": round(t_pairs, 2), "train_steps": steps, "train_time_s": round(t_train, 2),
+                 "loss": "energy" if mode == 0 else "scale-invariant energy",
+                 "loss_first20": float(hist[:20].mean()), "loss_last100": float(hist[-100:].mean()),
+                 "selection_seed": select_seed, "selection_iterations": sel,
+                 "selected_after_steps": best[0] if best else steps}
+
+
+SNAPS_2000 = [0, 1, 2, 4, 6, 9, 13, 18, 25, 35, 50, 70, 95, 130, 175, 230, 300, 380, 470, 560]
+
+
+def run_config4(nbx, torch, args):
+    """Learned element preconditioner on the 16^3, N=7 perturbed box: 2000
+    (residual, error) pairs from 100 seeded FP64 Jacobi PCG solves (seeds
+    1000..1099, 20 snapshots each), energy-norm training with model selection
+    on the solve of seed 8999, then test on right-hand sides of unseen seeds
+    (9000..9004): iterations and time of the learned flexible PCG (with and
+    without the stall fallback) against FP64 Jacobi PCG to 1e-8."""
+    box = nbx.Box.make(7, 16, 16, 16, perturb=0.1, seed=1)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
+    bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, flexible=True, fused=True)
+    net, info = train_net(nbx, torch, bp, 100, SNAPS_2000, args.train_steps, 1e-3, log, mode=args.learn_mode,
+                          select_seed=8999)
+    n = bp.fieldOffset
+    b, r, x = (torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3))
+    sync = torch.cuda.synchronize
+    rows = []
+    for seed in range(9000, 9005):
+        bp.make_rhs(seed, b)
+        rec = {"seed": seed}
+        for name, h, fb in (("jacobi", bj, None), ("learned", bp, (20, 0.9)), ("learned_nofallback", bp, (10 ** 6, 0.9))):
+            if fb:
+                h.set_fallback(*fb)
+            for _ in range(2):
+                r.copy_(b)
+                x.zero_()
+                st, t = timed(lambda: h.pcg(r, x, 1e-8, 5000), sync)
+            rec[name] = {"iterations": st.iterations, "time_s": round(t, 5), "converged": bool(st.converged),
+                         "fallback_iteration": st.fallback_iteration}
+        rows.append(rec)
+        log(f"[bench4] {rec}")
+    def avg(k, f):
+        return float(sum(row[k][f] for row in rows) / len(rows))
+    summary = {k: {"iterations": avg(k, "iterations"), "time_s": avg(k, "time_s")}
+               for k in ("jacobi", "learned", "learned_nofallback")}
+    return {"config": "learned element preconditioner, 16^3 elements N=7, 2000 pairs, validation seeds 9000-9004",
+            "training": info, "validation": rows, "mean": summary}
+
+
+def run_config5(nbx, torch, args):
+    """Stress case: 64x64x32 elements N=7 (~45M unknowns), z graded to
+    element aspect 1:10 at the bottom, +-0.1 perturbation, tol 1e-10: FP64
+    Jacobi PCG, the multigrid PCG with bf16-geometry V-cycle, and the learned
+    preconditioner together with the FP32 operator (iterative refinement; the
+    network trained on a 16x16x8 copy of the graded mesh)."""
+    box = nbx.Box.make(7, 64, 64, 32, perturb=0.1, zaspect=10.0, seed=1)
+    sync = torch.cuda.synchronize
+    t0 = time.perf_counter()
+    bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, fused=True)
+    log(f"[bench5] setup {time.perf_counter() - t0:.1f} s E={bj.Nelements}")
+    n = bj.fieldOffset
+    b = bj.device_array("rhs", n).clone()
+    r, x = torch.empty_like(b), torch.empty_like(b)
+    out = {}
+
+    def run(name, fn):
+        r.copy_(b)
+        x.zero_()
+        st, t = timed(fn, sync)
+        rec = {"time_s": round(t, 4), "iterations": st.iterations, "cycles": st.cycles,
+               "converged": bool(st.converged), "rel_residual": (st.rdotr / st.rdotr0) ** 0.5}
+        if "fp64_jacobi" in out:
+            rec["err_vs_fp64"] = float((x - x64[0]).abs().max() / x64[0].abs().max())
+        out[name] = rec
+        log(f"[bench5] {name}: {rec}")
+
+    x64 = []
+    run("fp64_jacobi", lambda: bj.pcg(r, x, 1e-10, args.maxit5))
+    x64.append(x.clone())
+    bj.free()
+    bm = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
+    bm.mg_setup([7, 3, 1], degree=3, coarse_iters=30, prec=nbx.NBX_BF16)
+    run("mixed_mg_bf16cycle", lambda: bm.pcg(r, x, 1e-10, 500))
+    bm.free()
+    small = nbx.BP(nbx.Box.make(7, 16, 16, 8, perturb=0.1, zaspect=10.0, seed=2), bpmode=True,
+                   precon=nbx.PRECON_LEARNED)
+    net, info = train_net(nbx, torch, small, 30, SNAPS_2000[:14], args.train_steps // 2, 1e-3, log,
+                          mode=args.learn_mode, select_seed=8999)
+    bl = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
+    net.attach(bl)
+    run("learned_mixed_f32", lambda: bl.pcg_mixed(r, x, 1e-10, args.maxit5, prec=nbx.NBX_F32, inner_tol=1e-5,
+                                                  max_cycles=20))
+    for v in out.values():
+        v["speedup_vs_fp64_jacobi"] = round(out["fp64_jacobi"]["time_s"] / v["time_s"], 3)
+    return {"config": "64x64x32 elements N=7, z-graded aspect 1:10, perturbed, tol 1e-10",
+            "elements": box.E, "learned_training": info, "solvers": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +305,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the time-to-1e-8 comparison of the FP64, mixed and learned solvers")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+                    help="2: the PCG benchmark line (default); 3: BP6 N=9 mixed precision; 4: learned "
+                         "preconditioner on 16^3; 5: 64x64x32 graded stress case (3-5 print one JSON line each)")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--learn-mode", type=int, default=1, choices=(0, 1),
+                    help="energy-norm loss: 0 fixed scale, 1 scale-invariant")
+    ap.add_argument("--maxit5", type=int, default=30000)
     ap.add_argument("--profile-pass", action="store_true",
                     help="short fixed sequence for rocprofv3 counter runs: a 1 GiB device copy (calibration), "
                          "5 axhelm launches, one 10-iteration PCG solve; prints no JSON")
@@ -126,6 +323,12 @@ def main():
 
     import torch
     import nekbench_amd as nbx
+
+    if args.config != 2:
+        torch.cuda.set_device(0)
+        res = {3: run_config3, 4: run_config4, 5: run_config5}[args.config](nbx, torch, args)
+        print(json.dumps(res), flush=True)
+        return
 
     N, NX = 7, args.nx
     dev = torch.device("cuda:0")

@@ -146,7 +146,7 @@ int nbx_box_mesh(const nbx_box* box, int mask_boundary, double* x, double* y, do
  * (result_dev) or host (result_host) pointer; results are deterministic: a
  * fixed tree independent of timing (tiles of 256 lanes x 8 values, lane
  * fma chain, per-wave halving tree, (S0+S2)+(S1+S3), partials combined the
- * same way in a last-block pass). */
+ * same way by a one-workgroup second pass). */
 int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
                                const double* x, const double* y, double* result_host, void* stream);
 int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream);
@@ -158,6 +158,10 @@ int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, doub
  * diagonal and the PCG workspace; r and x are the caller's. */
 typedef struct nbx_bp nbx_bp;
 
+/* COPY and JACOBI as nekBone's PRECONDITIONER option (solve.cpp:167-179);
+ * JACOBI also for Nfields = 3 (one diagonal for every field; nekBone stops
+ * there, setup.cpp:158-162).  LEARNED: the element network below; MG: the
+ * p-multigrid V-cycle below.  LEARNED and MG force the flexible beta. */
 typedef enum { NBX_PRECON_COPY = 0, NBX_PRECON_JACOBI = 1, NBX_PRECON_LEARNED = 2, NBX_PRECON_MG = 3 } nbx_precon;
 
 typedef struct {
@@ -204,6 +208,10 @@ int nbx_bp_ax_operator(nbx_bp* bp, const double* lambda, const double* q, double
  * is 0 (setup.cpp:210-214) and no Dirichlet mask is set (the reference never
  * masks, so it projects every lambda1 = 0 problem). */
 int nbx_bp_zero_mean(nbx_bp* bp, double* q);
+/* Right-hand side built as the setup builds "rhs" (U[-1,1) per local node
+ * from splitmix64(seed, stream 3 + field), masked, assembled; nekBone/setup.cpp
+ * :77-99) but with another seed; out is a device vector of Nfields*offset. */
+int nbx_bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out);
 int nbx_bp_all_neumann(const nbx_bp* bp);
 
 /* BPPCG: solves A x = r in place (r becomes the residual), tol relative,
@@ -278,9 +286,13 @@ int nbx_net_gradient(nbx_net* net, nbx_bp* bp, const int32_t* samples, int nsamp
                      float* grad, double* loss);
 /* Adam on the L2 error loss above, batch random (pair, element) samples a step */
 int nbx_net_train(nbx_net* net, nbx_bp* bp, int steps, int batch, float lr, uint64_t seed, double* loss_hist);
-/* Adam on the energy-norm loss: one random pair a step, all its elements,
- * L = d^T A d / e^T A e with d = mask(gs(w invDiag (r + s f(r)))) - e */
-int nbx_net_train_energy(nbx_net* net, nbx_bp* bp, int steps, float lr, uint64_t seed, double* loss_hist);
+/* Adam on an energy-norm loss, one random pair a step, all its elements;
+ * z = mask(gs(w invDiag (r + s f(r)))):
+ *   mode 0: L = (z - e)^T A (z - e) / e^T A e
+ *   mode 1: L = 1 - <r,z>^2 / (e^T A e <z,Az>)  (scale-invariant: the error
+ *           left by the best multiple of z) */
+int nbx_net_train_energy(nbx_net* net, nbx_bp* bp, int steps, float lr, uint64_t seed, int mode,
+                         double* loss_hist);
 
 /* per-iteration history of the last solve (host outputs, length >= iterations+1):
  * rdotr and alpha per iteration; returns the count written */

@@ -328,6 +328,9 @@ class BP:
     def mg_apply(self, r, z):
         check(lib().nbx_bp_mg_apply(self.h, ptr(r), ptr(z)), "nbx_bp_mg_apply")
 
+    def make_rhs(self, seed, out):
+        check(lib().nbx_bp_make_rhs(self.h, seed, ptr(out)), "nbx_bp_make_rhs")
+
     def zero_mean(self, q):
         check(lib().nbx_bp_zero_mean(self.h, ptr(q)), "nbx_bp_zero_mean")
 
@@ -384,6 +387,7 @@ def _wrap_device(addr: int, n: int, dtype):
 _SIGS.update({
     "nbx_net_num_params": (_I, []),
     "nbx_bp_zero_mean": (_I, [_P, _P]),
+    "nbx_bp_make_rhs": (_I, [_P, C.c_uint64, _P]),
     "nbx_bp_mg_setup": (_I, [_P, _I, _P, _I, _I, _D, _D, _I]),
     "nbx_bp_mg_info": (_I, [_P, _I, _P, _P]),
     "nbx_bp_mg_apply": (_I, [_P, _P, _P]),
@@ -402,7 +406,7 @@ _SIGS.update({
     "nbx_net_set_pairs": (_I, [_P, _P, _I, _P, _P]),
     "nbx_net_gradient": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "nbx_net_train": (_I, [_P, _P, _I, _I, C.c_float, C.c_uint64, _P]),
-    "nbx_net_train_energy": (_I, [_P, _P, _I, C.c_float, C.c_uint64, _P]),
+    "nbx_net_train_energy": (_I, [_P, _P, _I, C.c_float, C.c_uint64, _I, _P]),
 })
 
 
@@ -473,9 +477,9 @@ class Net:
         check(lib().nbx_net_train(self.h, bp.h, steps, batch, C.c_float(lr), seed, ptr(hist)), "nbx_net_train")
         return hist[:steps]
 
-    def train_energy(self, bp: "BP", steps, lr=1e-3, seed=1):
+    def train_energy(self, bp: "BP", steps, lr=1e-3, seed=1, mode=0):
         hist = np.zeros(max(steps, 1))
-        check(lib().nbx_net_train_energy(self.h, bp.h, steps, C.c_float(lr), seed, ptr(hist)),
+        check(lib().nbx_net_train_energy(self.h, bp.h, steps, C.c_float(lr), seed, mode, ptr(hist)),
               "nbx_net_train_energy")
         return hist[:steps]
 

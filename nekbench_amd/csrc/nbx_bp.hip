@@ -310,8 +310,9 @@ static int bp_setup_impl(nbx_bp* bp)
   bp->Np = bp->Nq * bp->Nq * bp->Nq;
   bp->Nf = o.Nfields;
   if (bp->Nf != 1 && bp->Nf != 3) return fail(NBX_ERR_INVALID, "bp: Nfields must be 1 or 3");
-  if (o.precon == NBX_PRECON_JACOBI && bp->Nf > 1)  // nekBone/setup.cpp:158-162
-    return fail(NBX_ERR_UNSUPPORTED, "bp: JACOBI preconditioner not supported for Nfields>1");
+  // JACOBI with Nfields = 3: nekBone stops here (setup.cpp:158-162); the
+  // vector operator applies the same ggeo and lambda to every field, so the
+  // one diagonal serves all three and the solve proceeds with it.
   const int64_t E = (int64_t)b.NX * b.NY * b.NZ;
   if (E * bp->Np * bp->Nf > INT32_MAX) return fail(NBX_ERR_INVALID, "bp: problem exceeds dlong range");
   bp->E = (int32_t)E;
@@ -806,6 +807,15 @@ int bp_solve_plain(nbx_bp* bp, nbx_precon precon, double* r, double* x, double t
   bp->opt.flexible = keepFlex;
   if (iterations) *iterations = S.iterations;
   return rc;
+}
+
+extern "C" int nbx_bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out)
+{
+  if (!bp || !out) return fail(NBX_ERR_INVALID, "bp_make_rhs: bad arguments");
+  int rc = bp_make_rhs(bp, seed, out);
+  if (rc) return rc;
+  NBX_HIP(hipStreamSynchronize(bp->stream));
+  return NBX_SUCCESS;
 }
 
 int bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out)

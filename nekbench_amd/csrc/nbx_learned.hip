@@ -155,6 +155,19 @@ __global__ __launch_bounds__(256) void k_energy_grad(const int64_t n, const doub
   if (i < n) Ad[i] = 2.0 * wn[0] * (w[i] * Ad[i]);
 }
 
+// scale-invariant loss L = 1 - a^2 gE / b (a = <r,z>_w, b = <z,Az>_w, gE =
+// 1 / e^T A e) and G = w dL/dz = w (2 a gE / b) ((a / b) Az - r), into Az
+__global__ __launch_bounds__(256) void k_angle_grad(const int64_t n, const double* __restrict__ ab,
+                                                    const double* __restrict__ gE, const double* __restrict__ w,
+                                                    const double* __restrict__ r, double* __restrict__ Az,
+                                                    double* __restrict__ loss)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = ab[0], b = ab[2], g = gE[0];
+  if (i == 0) loss[0] = 1.0 - a * a * g / b;
+  if (i < n) Az[i] = w[i] * ((2.0 * a * g / b) * ((a / b) * Az[i] - r[i]));
+}
+
 __global__ void k_scalar_mul(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = a[0] * b[0];
@@ -520,10 +533,14 @@ extern "C" int nbx_net_train(nbx_net* n, nbx_bp* bp, int steps, int batch, float
 // relative to the error before it.  dL/dz_local = 2 w A d / e^T A e is
 // back-propagated through every element.  loss_hist (host, may be null)
 // receives each step's L.
-extern "C" int nbx_net_train_energy(nbx_net* n, nbx_bp* bp, int steps, float lr, uint64_t seed, double* loss_hist)
+// mode 1: the scale-invariant form L = 1 - <r,z>^2 / (e^T A e <z,Az>), the
+// A-norm error left by the best multiple of z (CG does not see the scale of
+// the preconditioner).
+extern "C" int nbx_net_train_energy(nbx_net* n, nbx_bp* bp, int steps, float lr, uint64_t seed, int mode,
+                                    double* loss_hist)
 {
   if (!n || !bp || steps < 0 || !bp->feat || n->npairs < 1 || !(lr > 0) || bp->Nf != 1 ||
-      n->pairStride != bp->Nlocal)
+      n->pairStride != bp->Nlocal || (mode != 0 && mode != 1))
     return fail(NBX_ERR_INVALID, "net_train_energy: bad arguments (network attached, pairs of this mesh?)");
   const int64_t Nall = bp->Nall, Nl = bp->Nlocal;
   if (n->work_n < Nall) {
@@ -566,15 +583,28 @@ extern "C" int nbx_net_train_energy(nbx_net* n, nbx_bp* bp, int steps, float lr,
       return rc;
     if ((rc = gs_add_mask_launch(bp->ogs, zb, 1, bp->fieldOffset, bp->Nmasked, bp->maskIds, nullptr, bp->stream)))
       return rc;
-    hipLaunchKernelGGL(k_sub_f, g, b, 0, bp->stream, Nl, zb, Ep);
-    if ((rc = bp_apply(bp, bp->lambda, zb, Ad, nullptr))) return rc;
-    if ((rc = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, w, zb, Ad, false, bp->red, bp->red.result, bp->stream)))
-      return rc;
-    hipLaunchKernelGGL(k_energy_grad, g, b, 0, bp->stream, Nl, n->invNormE + p, w, Ad);
+    if (mode == 0) {
+      hipLaunchKernelGGL(k_sub_f, g, b, 0, bp->stream, Nl, zb, Ep);
+      if ((rc = bp_apply(bp, bp->lambda, zb, Ad, nullptr))) return rc;
+      if ((rc = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, w, zb, Ad, false, bp->red, bp->red.result, bp->stream)))
+        return rc;
+      hipLaunchKernelGGL(k_energy_grad, g, b, 0, bp->stream, Nl, n->invNormE + p, w, Ad);
+    } else {
+      if ((rc = bp_apply(bp, bp->lambda, zb, Ad, nullptr))) return rc;
+      if ((rc = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, w, rd, zb, false, bp->red, bp->red.result, bp->stream)) ||
+          (rc = launch_wdot(bp->Nlocal, bp->fieldOffset, 1, w, zb, Ad, false, bp->red, bp->red.result + 2,
+                            bp->stream)))
+        return rc;
+      hipLaunchKernelGGL(k_angle_grad, g, b, 0, bp->stream, Nl, bp->red.result, n->invNormE + p, w, rd, Ad,
+                         bp->red.result + 4);
+    }
     hipLaunchKernelGGL(k_pair_samples, dim3((bp->E + 255) / 256), b, 0, bp->stream, p, bp->E, n->samples);
     NBX_HIP_LAUNCH();
     if ((rc = run_gradient(n, bp, bp->E, chunk, Ad))) return rc;
-    hipLaunchKernelGGL(k_scalar_mul, dim3(1), dim3(64), 0, bp->stream, bp->red.result, n->invNormE + p, n->loss);
+    if (mode == 0)
+      hipLaunchKernelGGL(k_scalar_mul, dim3(1), dim3(64), 0, bp->stream, bp->red.result, n->invNormE + p, n->loss);
+    else
+      NBX_HIP(hipMemcpyAsync(n->loss, bp->red.result + 4, sizeof(double), hipMemcpyDeviceToDevice, bp->stream));
     n->steps += 1;
     if ((rc = net_adam_launch(n->master, n->m, n->v, n->grad, n->wbf, lr, n->steps, bp->stream))) return rc;
     if (loss_hist) {

@@ -173,3 +173,21 @@ def test_all_neumann_zero_mean_matches_oracle(nbx, nbo, dev, Nf):
     assert np.array_equal(H(x), xr)
     assert np.array_equal(H(r), rr)
     assert abs(xr.sum()) <= 1e-12 * np.abs(xr).sum()
+
+
+def test_bp6_jacobi_matches_oracle(nbx, nbo, dev):
+    """JACOBI with three fields (one diagonal shared by the fields of the
+    vector operator): iterates equal the oracle's."""
+    import torch
+    kw = dict(perturb=0.1, seed=8)
+    bp = nbx.BP(nbx.Box.make(4, 3, 2, 2, **kw), Nfields=3, lambda1=0.3, precon=nbx.PRECON_JACOBI, flexible=True,
+                fused=True)
+    ob = nbo.OracleBP(nbo.Box.make(4, 3, 2, 2, **kw), Nf=3, lambda1=0.3)
+    n = 3 * bp.fieldOffset
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg(r, x, 1e-10, 400)
+    xr = np.zeros(n)
+    ost, _, _ = ob.pcg(ob.rhs.copy(), xr, 1e-10, 400, jacobi=True, flexible=True)
+    assert st.converged and st.iterations == ost.iterations
+    assert np.array_equal(H(x), xr)

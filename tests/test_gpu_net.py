@@ -109,11 +109,52 @@ def test_energy_training_lowers_energy_loss(nbx, dev):
     net = nbx.Net(seed=1)
     net.attach(bp)
     net.generate_pairs(bp, nseeds=4, seed0=1000, snapshots=[0, 3, 10, 30], tol_exact=1e-10)
-    hist = net.train_energy(bp, steps=200, lr=2e-3, seed=3)
+    hist = net.train_energy(bp, steps=200, lr=2e-3, seed=3, mode=0)
     print("energy loss", hist[:3], hist[-3:])
     # at zero output layer the loss is that of one Jacobi step, below 1
     assert 0 < hist[0] < 1.0
     assert hist[-20:].mean() < 0.9 * hist[:3].mean()
+
+
+def test_trained_network_cuts_iterations_on_unseen_rhs(nbx, dev):
+    """What the trained network gains over the method without it (Jacobi
+    PCG): pairs from seeded FP64 Jacobi solves (seeds 1000..1019), the
+    scale-invariant energy loss, parameters selected on the solve of seed
+    8999; the learned PCG then needs fewer iterations than Jacobi PCG on the
+    right-hand sides of seeds 9000 and 9001, and reaches the same solution."""
+    import torch
+    box = nbx.Box.make(7, 8, 8, 8, perturb=0.1, seed=4)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
+    bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, flexible=True, fused=True)
+    net = nbx.Net(seed=1)
+    net.attach(bp)
+    net.generate_pairs(bp, nseeds=20, seed0=1000, snapshots=[0, 2, 6, 15, 35, 70, 120], tol_exact=1e-11)
+    n = bp.fieldOffset
+    b, r, x = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3))
+    bp.set_fallback(10 ** 6, 0.9)
+
+    def iters(h, seed):
+        h.make_rhs(seed, b)
+        r.copy_(b)
+        x.zero_()
+        st = h.pcg(r, x, 1e-8, 4000)
+        assert st.converged
+        return st.iterations
+
+    best = (iters(bp, 8999), net.params())  # zero output layer: Jacobi
+    for c in range(8):
+        net.train_energy(bp, steps=100, lr=1e-3, seed=7 + c, mode=1)
+        its = iters(bp, 8999)
+        if its < best[0]:
+            best = (its, net.params())
+    net.set_params(best[1])
+    for seed in (9000, 9001):
+        lj = iters(bj, seed)
+        xj = x.clone()
+        ll = iters(bp, seed)
+        print("seed", seed, "jacobi", lj, "learned", ll)
+        assert ll < lj
+        assert (x - xj).abs().max().item() <= 1e-6 * xj.abs().max().item()
 
 
 @pytest.mark.parametrize("scale3", [0.05, 0.5])
