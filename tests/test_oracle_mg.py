@@ -105,3 +105,23 @@ def test_mg_pcg_converges_fast_to_the_jacobi_solution(nbo, mg44):
     # the true residual agrees with the recursively updated one
     tr = ob.rhs - ob.apply(x)
     assert np.sqrt(ob.wdot(tr, tr)) <= 1e-9 * np.sqrt(hr[1])
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 2e-5), (2, 2e-2)])
+def test_low_precision_cycles_track_the_fp64_cycle(nbo, prec, tol):
+    """The FP32 cycle (prec 1) and the bf16-geometry cycle (prec 2) restate the
+    FP64 cycle in float: their output must agree with it to float rounding
+    (prec 1) or to the bf16 rounding of the geometry (prec 2), and an MG-PCG
+    with either reaches the FP64 solution."""
+    box = nbo.Box.make(5, 3, 3, 2, perturb=0.1, seed=7)
+    ref = nbo.OracleMG(box, [5, 2, 1], degree=2, coarse_iters=30, prec=0)
+    lo = nbo.OracleMG(box, [5, 2, 1], degree=2, coarse_iters=30, prec=prec)
+    r = ref.fine.rhs.copy()
+    z0, z1 = ref.vcycle(r), lo.vcycle(r)
+    assert np.abs(z1 - z0).max() <= tol * np.abs(z0).max()
+    assert not np.array_equal(z1, z0)
+    x0, x1 = np.zeros(ref.fine.offset), np.zeros(ref.fine.offset)
+    s0, _, _ = ref.pcg(ref.fine.rhs.copy(), x0, 1e-10, 300)
+    s1, _, _ = lo.pcg(lo.fine.rhs.copy(), x1, 1e-10, 300)
+    assert s0.converged and s1.converged and abs(s1.iterations - s0.iterations) <= 2
+    assert np.abs(x1 - x0).max() <= 1e-8 * np.abs(x0).max()
