@@ -303,6 +303,33 @@ def run_config5(nbx, torch, args):
             "elements": 64 * 64 * 32, "learned_training": info, "solvers": out}
 
 
+def profile_mg_pass(nbx, torch, NX):
+    import numpy as np
+    box = nbx.Box.make(7, NX, NX, NX, perturb=0.1, seed=1)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
+    bp.mg_setup([7, 3, 1], degree=3, coarse_iters=20, prec=nbx.NBX_BF16)
+    n = bp.fieldOffset
+    b = bp.device_array("rhs", n).clone()
+    r, x = torch.empty_like(b), torch.empty_like(b)
+    for _ in range(2):
+        r.copy_(b)
+        x.zero_()
+        st = bp.pcg(r, x, 1e-8, 500)
+    log(f"[bench] multigrid bf16 cycle: {st.iterations} iterations, {st.elapsed_s * 1e3:.1f} ms")
+    bl = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
+    net = nbx.Net(seed=3)
+    W = net.params()
+    W[2176:3040] = np.random.default_rng(0).uniform(-0.05, 0.05, 864).astype(np.float32)  # nonzero output layer
+    net.set_params(W)
+    net.attach(bl)
+    bl.update_jacobi()
+    zl = torch.empty_like(b)
+    for _ in range(5):
+        net.apply(bl, b, zl)
+    torch.cuda.synchronize()
+    log("[bench] profile pass done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +350,9 @@ def main():
     ap.add_argument("--learn-mode", type=int, default=1, choices=(0, 1),
                     help="energy-norm loss: 0 fixed scale, 1 scale-invariant")
     ap.add_argument("--maxit5", type=int, default=30000)
+    ap.add_argument("--profile-mg", action="store_true",
+                    help="short sequence for a rocprofv3 kernel trace: two multigrid PCG solves (bf16-geometry "
+                         "V-cycle, degree 3, 20 coarse iterations) to 1e-8 and five network applies; prints no JSON")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short fixed sequence for rocprofv3 counter runs: a 1 GiB device copy (calibration), "
                          "5 axhelm launches, one 10-iteration PCG solve; prints no JSON")
@@ -335,6 +365,9 @@ def main():
     import torch
     import nekbench_amd as nbx
 
+    if args.profile_mg:
+        profile_mg_pass(nbx, torch, args.nx)
+        return
     if args.config != 2:
         torch.cuda.set_device(0)
         res = {3: run_config3, 4: run_config4, 5: run_config5}[args.config](nbx, torch, args)

@@ -132,3 +132,23 @@ def test_zero_mean_projection(nbo):
         assert np.all(c == 0.0)
     assert not nbo.OracleBP(box, bpmode=False, lambda1=1.1, dirichlet=False).all_neumann
     assert not nbo.OracleBP(nbo.Box.make(3, 2, 2, 2), bpmode=True, dirichlet=True).all_neumann
+
+
+@pytest.mark.parametrize("prec,inner", [(0, 1e-4), (1, 1e-5), (2, 1e-2)])
+def test_mixed_refinement_reaches_the_fp64_solution(nbo, prec, inner):
+    """Iterative refinement (FP64 true residual, inner PCG with the FP64, FP32
+    or bf16-geometry operator): converges to the FP64 solution; with the FP64
+    operator and a tight inner tolerance one cycle equals plain PCG."""
+    box = nbo.Box.make(4, 3, 3, 2, perturb=0.1, seed=6)
+    ob = nbo.OracleBP(box, bpmode=True)
+    x64 = np.zeros(ob.offset)
+    ob.pcg(ob.rhs.copy(), x64, 1e-13, 3000)
+    x = np.zeros(ob.offset)
+    st = ob.pcg_mixed(ob.rhs.copy(), x, 1e-10, 3000, prec=prec, inner_tol=inner, max_cycles=40)
+    assert st.converged and (st.rdotr / st.rdotr0) ** 0.5 <= 1e-10
+    assert np.abs(x - x64).max() <= 1e-8 * np.abs(x64).max()
+    if prec == 0:
+        assert st.iter >= 2  # cycles: the inner tolerance is looser than the outer one
+    # the true residual of the result meets the tolerance
+    tr = ob.rhs - ob.apply(x)
+    assert ob.wdot(tr, tr) <= 1e-20 * ob.wdot(ob.rhs, ob.rhs) * 1.0001
