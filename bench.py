@@ -93,7 +93,7 @@ def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None, mg_cfg=None):
     if bp_mg is not None:
         orders, degree, ci = mg_cfg
         for name, prec in (("fp64_mg", nbx.NBX_F64), ("mixed_mg_f32cycle", nbx.NBX_F32),
-                           ("mixed_mg_bf16cycle", nbx.NBX_BF16)):
+                           ("mixed_mg_bf16cycle", nbx.NBX_BF16), ("mixed_mg_bf16mfma_cycle", nbx.NBX_BF16_MFMA)):
             bp_mg.mg_setup(orders, degree=degree, coarse_iters=ci, prec=prec)
             run(name, lambda: bp_mg.pcg(r, x, tol, 500))
     base = out["fp64_jacobi"]["time_s"]
@@ -442,6 +442,26 @@ def main():
     log(f"[bench] axhelm BK N=7 E={E}: {ax_s * 1e6:.1f} us/launch, {ax_gbs:.0f} GB/s, "
         f"{E * N ** 3 / ax_s / 1e9:.1f} GDOF/s")
 
+    # the other precisions of the same operator on the same mesh (FP32 vectors)
+    variants = {}
+    g32, D32, q32 = ggeo.float(), D.float(), x.float().contiguous()
+    g16 = ggeo.to(torch.bfloat16).view(torch.int16)
+    A32 = torch.empty_like(q32)
+    for name, prec, g, bpn in (("f32", nbx.NBX_F32, g32, 36), ("bf16_geometry", nbx.NBX_BF16, g16, 22),
+                               ("bf16_mfma", nbx.NBX_BF16_MFMA, g16, 22)):
+        for _ in range(3):
+            nbx.axhelm(N, 1, 1, E, n, g, D32, None, q32, A32, prec=prec, stream=s)
+        ev0.record(s)
+        for _ in range(args.ax_launches):
+            nbx.axhelm(N, 1, 1, E, n, g, D32, None, q32, A32, prec=prec, stream=s)
+        ev1.record(s)
+        ev1.synchronize()
+        t_v = ev0.elapsed_time(ev1) / 1e3 / args.ax_launches
+        variants[name] = {"us_per_launch": round(t_v * 1e6, 2), "bytes_per_node": bpn,
+                          "GB_s": round(E * bp.Np * bpn / t_v / 1e9, 1), "gdofs": round(E * N ** 3 / t_v / 1e9, 2)}
+    log(f"[bench] axhelm variants: {variants}")
+    del g32, g16, q32, A32
+
     compare = None
     if not args.no_compare:
         t1 = time.perf_counter()
@@ -464,7 +484,8 @@ def main():
                                f"tol {args.tol:g} or {args.maxit} iterations", "elements": E, "N": N,
                    "iterations": its, "converged": bool(st.converged),
                    "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
-                   "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "parallelism": "single GPU",
+                   "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "axhelm_variants": variants,
+                   "parallelism": "single GPU",
                    "time_to_1e-8": compare},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,

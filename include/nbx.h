@@ -40,7 +40,9 @@ typedef int64_t nbx_hlong;
 #define NBX_ERR_ALLOC (-4)       /* allocation failed */
 #define NBX_ERR_DIVERGED (-5)    /* PCG breakdown (non-finite or non-positive curvature) */
 
-typedef enum { NBX_F64 = 0, NBX_F32 = 1, NBX_BF16 = 2, NBX_F16 = 3 } nbx_prec;
+/* NBX_BF16_MFMA: bf16 geometry and bf16 MFMA tensor contractions with FP32
+ * accumulation, FP32 vectors (nbx_axhelm: N = 7, one field, no elementList). */
+typedef enum { NBX_F64 = 0, NBX_F32 = 1, NBX_BF16 = 2, NBX_F16 = 3, NBX_BF16_MFMA = 4 } nbx_prec;
 /* ogs types / ops (3rdParty/ogs/ogs.hpp:128-140: "double", "float", "int",
  * "long long int"; "add", "mul", "min", "max") */
 typedef enum { NBX_DOUBLE = 0, NBX_FLOAT = 1, NBX_INT = 2, NBX_LONG = 3 } nbx_type;
@@ -66,7 +68,9 @@ int nbx_device_synchronize(void);
  * prec selects the arithmetic and the element type of ggeo, D, lambda, q, Aq:
  * NBX_F64 (double), NBX_F32 (float), NBX_BF16 (float D, q, Aq and FP32
  * arithmetic; ggeo and lambda as bf16, i.e. the upper 16 bits of the float,
- * uint16_t; one field, no elementList).  N in [1, 15]. */
+ * uint16_t; one field, no elementList), NBX_BF16_MFMA (as NBX_BF16, with q,
+ * D and the G-scaled gradients rounded to bf16 for the matrix-core
+ * contractions, FP32 accumulation; N = 7).  N in [1, 15]. */
 int nbx_axhelm(int N, int Nfields, int bk, nbx_prec prec,
                nbx_dlong Nelements, nbx_dlong offset, const nbx_dlong* elementList,
                const void* ggeo, const void* D, const void* lambda,
@@ -237,8 +241,9 @@ int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, dou
  * element-local GLL interpolation between levels, and coarse_iters Jacobi-PCG
  * iterations on the last level.  Used inside flexible PCG by nbx_bp_pcg and
  * nbx_bp_pcg_mixed when the handle was set up with precon = NBX_PRECON_MG.
- * prec: NBX_F64 (all FP64), NBX_F32 or NBX_BF16 (smoothing and transfers of
- * every level but the last in FP32, operator geometry in FP32 or bf16; the
+ * prec: NBX_F64 (all FP64), NBX_F32, NBX_BF16 or NBX_BF16_MFMA (smoothing and
+ * transfers of every level but the last in FP32, operator geometry in FP32 or
+ * bf16, the N = 7 level's contractions on the matrix cores for MFMA; the
  * cycle reads the FP64 residual and returns an FP64 correction; the coarse
  * PCG is FP64).  One field, N <= 11.  Uses the handle's lambda. */
 int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters, double lo, double hi,

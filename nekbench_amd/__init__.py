@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libnbx.so"
 
-NBX_F64, NBX_F32, NBX_BF16, NBX_F16 = 0, 1, 2, 3
+NBX_F64, NBX_F32, NBX_BF16, NBX_F16, NBX_BF16_MFMA = 0, 1, 2, 3, 4
 NBX_DOUBLE, NBX_FLOAT, NBX_INT, NBX_LONG = 0, 1, 2, 3
 NBX_ADD, NBX_MUL, NBX_MIN, NBX_MAX = 0, 1, 2, 3
 PRECON_COPY, PRECON_JACOBI, PRECON_LEARNED, PRECON_MG = 0, 1, 2, 3

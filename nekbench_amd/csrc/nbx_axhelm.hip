@@ -384,6 +384,9 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
     case NBX_BF16:  // FP32 vectors and arithmetic, bf16 geometric factors and lambda
       return dispatch_n<float, uint16_t>(N, Nf, bk, partial, Nelements, offset, elementList, ggeo, D, lambda, q, Aq,
                                          skip, s);
+    case NBX_BF16_MFMA:  // as NBX_BF16 with the contractions on the matrix cores
+      if (partial) return fail(NBX_ERR_UNSUPPORTED, "axhelm bf16-MFMA: no element list");
+      return axhelm_mma_launch(N, Nf, bk, true, Nelements, offset, ggeo, D, lambda, q, Aq, skip, s);
     default:
       return fail(NBX_ERR_UNSUPPORTED, "axhelm: precision %d not supported by nbx_axhelm", (int)prec);
   }

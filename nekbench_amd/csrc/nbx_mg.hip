@@ -326,8 +326,10 @@ int level_apply(nbx_mg* mg, nbx_mg_level& L, const double* q, double* Aq, const 
 int level_apply(nbx_mg* mg, nbx_mg_level& L, const float* q, float* Aq, const int* skip)
 {
   nbx_bp* bp = L.bp;
-  const bool bf = mg->prec == NBX_BF16;
-  int rc = axhelm_launch(bp->N, 1, bp->opt.bpmode ? 1 : 0, mg->prec, bp->E, bp->fieldOffset, nullptr,
+  const bool bf = mg->prec == NBX_BF16 || mg->prec == NBX_BF16_MFMA;
+  // matrix-core contractions at N = 7; the other levels use the bf16-geometry FP32 kernel
+  const nbx_prec p = (mg->prec == NBX_BF16_MFMA && bp->N != 7) ? NBX_BF16 : mg->prec;
+  int rc = axhelm_launch(bp->N, 1, bp->opt.bpmode ? 1 : 0, p, bp->E, bp->fieldOffset, nullptr,
                          bf ? (const void*)bp->ggeo16 : (const void*)bp->ggeo32, bp->D32,
                          bf ? (const void*)bp->lambda16 : (const void*)bp->lambda32, q, Aq, skip, bp->stream);
   if (rc) return rc;
@@ -449,8 +451,8 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
   if (!bp || nlevels < 2 || nlevels > 8 || !orders || degree < 1 || degree > 16 || coarse_iters < 1 ||
       !(lo > 0) || !(hi > lo))
     return fail(NBX_ERR_INVALID, "bp_mg_setup: bad arguments");
-  if (prec != NBX_F64 && prec != NBX_F32 && prec != NBX_BF16)
-    return fail(NBX_ERR_UNSUPPORTED, "bp_mg_setup: precision %d (F64, F32 or BF16)", (int)prec);
+  if (prec != NBX_F64 && prec != NBX_F32 && prec != NBX_BF16 && prec != NBX_BF16_MFMA)
+    return fail(NBX_ERR_UNSUPPORTED, "bp_mg_setup: precision %d (F64, F32, BF16 or BF16_MFMA)", (int)prec);
   if (bp->Nf != 1) return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: one field");
   if (orders[0] != bp->N) return fail(NBX_ERR_INVALID, "bp_mg_setup: orders[0] must be the handle's N");
   for (int l = 1; l < nlevels; ++l)
@@ -510,7 +512,7 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
       if ((rc = power_lmax(b, bp->opt.box.seed, 17 + (uint64_t)l, 20, b->z, b->Ax, &L.lmax))) break;
       cheb_coeffs(L.lmax, lo, hi, degree, L.c1, L.c2);
       if (prec != NBX_F64) {
-        if ((rc = bp_prepare_precision(b, prec, b->lambda))) break;
+        if ((rc = bp_prepare_precision(b, prec == NBX_BF16_MFMA ? NBX_BF16 : prec, b->lambda))) break;
         std::vector<float> J32(J.size());
         for (size_t q = 0; q < J.size(); ++q) J32[q] = (float)J[q];
         if ((rc = mg_alloc(mg, &L.J32, (int64_t)J.size())) || (rc = mg_alloc(mg, &L.invD32, b->Nlocal))) break;

@@ -690,3 +690,67 @@ double nbo_tree_dot(int64_t n, const double* x, const double* y)
 
 #include <omp.h>
 int nbo_max_threads(void) { return omp_get_max_threads(); }
+
+/* axhelm with bf16 matrix-core contractions (library: k_axhelm_mma,
+ * nbx_axhelm_mma.hip), N = 7, one field: q, D and the G-scaled gradients are
+ * rounded to bf16 (nearest even); the 8-term contractions are accumulated in
+ * double and rounded to float (the products are exact in FP32; the matrix
+ * cores' internal summation order is not specified, so agreement is to FP32
+ * rounding, not bit for bit); geometric factors applied in FP32 with the
+ * FP32 kernel's fma order; direction results summed r, then s, then t. */
+static float bf16r(float x)
+{
+  union { float f; uint32_t u; } a;
+  a.f = x;
+  a.u = (a.u + 0x7FFFu + ((a.u >> 16) & 1u)) & 0xFFFF0000u;
+  return a.f;
+}
+
+void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
+                        const float* q, float* Aq)
+{
+  enum { Q = 8, P = 512 };
+  float Db[Q * Q];
+  for (int i = 0; i < Q * Q; ++i) Db[i] = bf16r(D[i]);
+#pragma omp parallel
+  {
+    float qb[P], qr[P], qs[P], qt[P], gr[P], gs[P], gt[P];
+#pragma omp for schedule(static)
+    for (int32_t e = 0; e < E; ++e) {
+      const float* qe = q + (int64_t)e * P;
+      const float* g = ggeo + (int64_t)e * 7 * P;
+      for (int n = 0; n < P; ++n) qb[n] = bf16r(qe[n]);
+      for (int n = 0; n < P; ++n) {
+        const int i = n % Q, j = (n / Q) % Q, k = n / (Q * Q);
+        double ar = 0, as = 0, at = 0;
+        for (int m = 0; m < Q; ++m) {
+          ar += (double)Db[i * Q + m] * qb[m + Q * j + Q * Q * k];
+          as += (double)Db[j * Q + m] * qb[i + Q * m + Q * Q * k];
+          at += (double)Db[k * Q + m] * qb[i + Q * j + Q * Q * m];
+        }
+        qr[n] = (float)ar; qs[n] = (float)as; qt[n] = (float)at;
+      }
+      for (int n = 0; n < P; ++n) {
+        const float G00 = g[1 * P + n], G01 = g[2 * P + n], G02 = g[3 * P + n];
+        const float G11 = g[4 * P + n], G12 = g[5 * P + n], G22 = g[6 * P + n];
+        float Gr = fmaf(G02, qt[n], fmaf(G01, qs[n], G00 * qr[n]));
+        float Gs = fmaf(G12, qt[n], fmaf(G11, qs[n], G01 * qr[n]));
+        float Gt = fmaf(G22, qt[n], fmaf(G12, qs[n], G02 * qr[n]));
+        if (!bk) { const float l0 = lambda[(int64_t)e * P + n]; Gr = l0 * Gr; Gs = l0 * Gs; Gt = l0 * Gt; }
+        gr[n] = bf16r(Gr); gs[n] = bf16r(Gs); gt[n] = bf16r(Gt);
+      }
+      for (int n = 0; n < P; ++n) {
+        const int i = n % Q, j = (n / Q) % Q, k = n / (Q * Q);
+        double ar = 0, as = 0, at = 0;
+        for (int m = 0; m < Q; ++m) {
+          ar += (double)Db[m * Q + i] * gr[m + Q * j + Q * Q * k];
+          as += (double)Db[m * Q + j] * gs[i + Q * m + Q * Q * k];
+          at += (double)Db[m * Q + k] * gt[i + Q * j + Q * Q * m];
+        }
+        float res = ((float)ar + (float)as) + (float)at;
+        if (!bk) res = fmaf(g[0 * P + n] * lambda[(int64_t)e * P + offset + n], qe[n], res);
+        Aq[(int64_t)e * P + n] = res;
+      }
+    }
+  }
+}

@@ -81,6 +81,8 @@ void nbo_axhelm(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const 
                 const double* ggeo, const double* D, const double* lambda,
                 const double* q, double* Aq);
 /* single-precision restatement (same loop nest, float arithmetic) */
+void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
+                        const float* q, float* Aq);
 void nbo_axhelm_f32(int N, int Nf, int bk, int32_t Nelements, int32_t offset,
                     const float* ggeo, const float* D, const float* lambda,
                     const float* q, float* Aq);
@@ -149,7 +151,8 @@ typedef struct nbo_mg {
   const double* c1;            /* [nlev][degree] Chebyshev coefficients */
   const double* c2;
   int coarse_iters;            /* Jacobi-PCG iterations on the last level */
-  int prec;                    /* 0: FP64 cycle; 1 FP32, 2 bf16 geometry: FP32 cycle but the last level */
+  int prec;                    /* 0: FP64 cycle; 1 FP32, 2 bf16 geometry, 3 bf16 geometry with bf16
+                                  matrix-core contractions at N = 7: FP32 cycle but the last level */
   const float* const* ggeo32;  /* per level: ggeo as FP32 (or bf16-rounded) values */
   const float* const* lam32;   /* per level: lambda (2 offset) likewise */
   const float* const* D32;

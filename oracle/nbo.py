@@ -295,6 +295,13 @@ class OracleBP:
             pass
 
 
+def axhelm_bf16mma(bk, E, offset, ggeo, D, lam, q):
+    """float32 arrays; ggeo and lam hold bf16 values"""
+    Aq = np.zeros_like(q)
+    lib().nbo_axhelm_bf16mma(int(bk), C.c_int32(E), C.c_int32(offset), p(ggeo), p(D), p(lam), p(q), p(Aq))
+    return Aq
+
+
 def sumsq_inv_f(a):
     f = lib().nbo_sumsq_inv_f
     f.restype = C.c_double
@@ -381,7 +388,7 @@ class OracleMG:
         # FP32 / bf16-geometry cycle: float copies (double -> float, bf16 by RNE of the float)
         def lo_vals(a):
             f = np.ascontiguousarray(a, dtype=np.float32)
-            return bf16_round(f) if prec == 2 else f
+            return bf16_round(f) if prec in (2, 3) else f
         self.f32 = []
         for l, lv in enumerate(self.levels):
             self.f32.append((lo_vals(lv.mesh["ggeo"]), lo_vals(lv.lam), np.ascontiguousarray(lv.mesh["D"], np.float32),

@@ -584,7 +584,10 @@ int nbo_bp_pcg_mg(const nbo_mg* mg, double* r, double* x, double tol, int maxit,
 static void mg_apply_f(const nbo_mg* mg, int l, const float* q, float* Aq)
 {
   const nbo_bp* bp = &mg->bps[l];
-  nbo_axhelm_f32(bp->N, 1, bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
+  if (mg->prec == 3 && bp->N == 7)  /* matrix-core contractions at N = 7 */
+    nbo_axhelm_bf16mma(bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
+  else
+    nbo_axhelm_f32(bp->N, 1, bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
   nbo_gs_many(bp->ogs, Aq, 1, bp->offset, NBO_FLOAT, NBO_ADD);
   for (int32_t m = 0; m < bp->Nmasked; ++m) Aq[bp->maskIds[m]] = 0.f;
 }

@@ -204,3 +204,33 @@ def test_scaled_add(nbx, dev):
     nbx.scaled_add(1001, -2.5, x, 0.5, y)
     torch.cuda.synchronize()
     assert torch.equal(y, ref)
+
+
+@pytest.mark.parametrize("E", [5, 8])
+@pytest.mark.parametrize("bk", [1, 0])
+def test_axhelm_bf16_mfma_matches_oracle(nbx, nbo, dev, E, bk):
+    """bf16 matrix-core axhelm (N=7): equal to the oracle to FP32 rounding of
+    the 8-term contractions (the matrix cores' summation order is not
+    specified), within bf16 rounding of the FP64 operator; an odd element
+    count exercises the idle second slot of the last workgroup."""
+    import torch
+    box = nbo.Box.make(7, E, 1, 1, perturb=0.1, seed=21)
+    m = nbo.box_mesh(box)
+    off = E * 512
+    rng = np.random.default_rng(E + bk)
+    q = rng.uniform(-1, 1, off)
+    lam = np.concatenate([rng.uniform(0.5, 1.5, off), rng.uniform(0.1, 0.5, off)])
+    g32 = nbo.bf16_round(m["ggeo"].astype(np.float32))
+    l32 = nbo.bf16_round(lam.astype(np.float32))
+    D32 = m["D"].astype(np.float32)
+    bits = lambda a: torch.from_numpy((a.view(np.uint32) >> 16).astype(np.uint16).view(np.int16).copy())
+    dg, dl = bits(g32).to(dev), bits(l32).to(dev)
+    dq = torch.tensor(q.astype(np.float32), device=dev)
+    dA = torch.zeros_like(dq)
+    nbx.axhelm(7, 1, bk, E, off, dg, torch.tensor(D32, device=dev), dl, dq, dA, prec=nbx.NBX_BF16_MFMA)
+    torch.cuda.synchronize()
+    out = dA.cpu().numpy()
+    exp = nbo.axhelm_bf16mma(bk, E, off, g32, D32, l32, q.astype(np.float32))
+    assert np.abs(out - exp).max() <= 1e-5 * np.abs(exp).max()
+    ref = nbo.axhelm(7, 1, bk, E, off, m["ggeo"], m["D"], lam, q)
+    assert np.abs(out - ref).max() <= 3e-2 * np.abs(ref).max()

@@ -65,6 +65,34 @@ def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case, prec):
     assert np.array_equal(H(x), xr)
 
 
+def test_mg_bf16_mfma_cycle(nbx, nbo, dev):
+    """V-cycle with the N=7 level on the matrix cores: close to the oracle's
+    restatement (the matrix cores' summation order is not specified, so not
+    bit for bit), and the MG-PCG reaches the FP64 solution in the oracle's
+    iteration count (+-1)."""
+    import torch
+    case = CASES[0]
+    N, dims, orders, deg, ci, bpmode, l1 = case
+    kw = dict(perturb=0.1, seed=50 + N)
+    bp = nbx.BP(nbx.Box.make(N, *dims, **kw), bpmode=bpmode, lambda1=l1, precon=nbx.PRECON_MG)
+    bp.mg_setup(orders, deg, ci, prec=nbx.NBX_BF16_MFMA)
+    mg = nbo.OracleMG(nbo.Box.make(N, *dims, **kw), orders, degree=deg, coarse_iters=ci, bpmode=bpmode,
+                      lambda1=l1, prec=3)
+    n = bp.fieldOffset
+    r = mg.fine.rhs.copy()
+    z = torch.zeros(n, dtype=torch.float64, device=dev)
+    bp.mg_apply(torch.tensor(r, device=dev), z)
+    zo = mg.vcycle(r)
+    assert np.abs(H(z) - zo).max() <= 1e-3 * np.abs(zo).max()
+    rr = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(rr)
+    st = bp.pcg(rr, x, 1e-10, 200)
+    xo = np.zeros(n)
+    ost, _, _ = mg.pcg(mg.fine.rhs.copy(), xo, 1e-10, 200)
+    assert st.converged and abs(st.iterations - ost.iterations) <= 1
+    assert np.abs(H(x) - xo).max() <= 1e-8 * np.abs(xo).max()
+
+
 def test_mg_cuts_iterations(nbx, dev):
     import torch
     box = nbx.Box.make(7, 6, 6, 6, perturb=0.1, seed=3)

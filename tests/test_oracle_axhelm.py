@@ -138,3 +138,27 @@ def test_f32_restatement_close(nbo):
     f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     got = nbo.axhelm_f32(N, 1, 0, E, off, f(d["ggeo"]), f(d["D"]), f(d["lam"]), f(d["q"][:off]))
     assert relerr(got.astype(np.float64), d["axhelm_v0"]) < 2e-5
+
+
+def test_bf16_mfma_oracle_tracks_fp64(nbo):
+    """Matrix-core form (bf16 q, D and G-scaled gradients, FP32 accumulation):
+    within bf16 rounding of the FP64 operator for BK and non-BK, and a
+    constant vector stays near the null space."""
+    box = nbo.Box.make(7, 2, 2, 1, perturb=0.1, seed=12)
+    m = nbo.box_mesh(box)
+    E, Np = box.E, 512
+    off = E * Np
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-1, 1, off)
+    lam = np.concatenate([rng.uniform(0.5, 1.5, off), rng.uniform(0.1, 0.5, off)])
+    g32 = nbo.bf16_round(m["ggeo"].astype(np.float32))
+    l32 = nbo.bf16_round(lam.astype(np.float32))
+    D32 = m["D"].astype(np.float32)
+    for bk in (1, 0):
+        ref = nbo.axhelm(7, 1, bk, E, off, m["ggeo"], m["D"], lam, q)
+        out = nbo.axhelm_bf16mma(bk, E, off, g32, D32, l32, q.astype(np.float32))
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        assert 1e-5 < err < 3e-2, (bk, err)
+    one = nbo.axhelm_bf16mma(1, E, off, g32, D32, l32, np.ones(off, np.float32))
+    ref1 = nbo.axhelm(7, 1, 1, E, off, m["ggeo"], m["D"], lam, q)
+    assert np.abs(one).max() < 3e-2 * np.abs(ref1).max()
