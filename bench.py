@@ -117,9 +117,10 @@ def timed(fn, sync):
 
 def run_config3(nbx, torch, args):
     """BP6 vector Helmholtz, 3 fields, lambda0 = 1, lambda1 = 0.1 (axhelm_n3),
-    N = 9, 24^3 elements, perturbed Dirichlet box; FP64 flexible Jacobi PCG and
-    the FP32 / bf16-geometry iterative-refinement solvers to 1e-10; error of
-    each field against the FP64 solution."""
+    N = 9, 24^3 elements, perturbed Dirichlet box; FP64 flexible Jacobi PCG,
+    the FP32 / bf16-geometry iterative-refinement solvers and the multigrid
+    PCG (orders 9, 4, 1; FP64 and FP32 cycles) to 1e-10; error of each field
+    against the FP64 solution."""
     box = nbx.Box.make(9, 24, 24, 24, perturb=0.1, seed=1)
     t0 = time.perf_counter()
     bp = nbx.BP(box, Nfields=3, bpmode=False, lambda1=0.1, precon=nbx.PRECON_JACOBI, flexible=True, fused=True)
@@ -135,7 +136,17 @@ def run_config3(nbx, torch, args):
                                                       max_cycles=30)),
             ("mixed_bf16_jacobi", lambda: bp.pcg_mixed(r, x, 1e-10, 20000, prec=nbx.NBX_BF16, inner_tol=2e-2,
                                                        max_cycles=80))]
+    bm = nbx.BP(box, Nfields=3, bpmode=False, lambda1=0.1, precon=nbx.PRECON_MG)
+
+    def mg(prec):
+        def fn():
+            return bm.pcg(r, x, 1e-10, 500)
+        return fn
+    runs += [("fp64_mg", mg(nbx.NBX_F64)), ("mixed_mg_f32cycle", mg(nbx.NBX_F32))]
+    mg_prec = {"fp64_mg": nbx.NBX_F64, "mixed_mg_f32cycle": nbx.NBX_F32}
     for name, fn in runs:
+        if name in mg_prec:
+            bm.mg_setup([9, 4, 1], degree=3, coarse_iters=20, prec=mg_prec[name])
         for _ in range(2):  # the second run is timed
             r.copy_(b)
             x.zero_()

@@ -67,36 +67,39 @@ __device__ __forceinline__ double fmaV<double>(double a, double b, double c) { r
 template <>
 __device__ __forceinline__ float fmaV<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// Chebyshev-Jacobi step (n local nodes of one field), arithmetic in V; the
+// Chebyshev-Jacobi step (n local nodes per field, fields at stride offset,
+// blockIdx.y = field, one Jacobi diagonal for all fields), arithmetic in V; the
 // level's right-hand side is R, its solution X (level 0: the solver's FP64
 // residual and correction):
 //   MODE 0 (first pre-smoothing step, x = 0):  res = (V)r; d = c2 D^-1 res; x = d
 //   MODE 1 (general step):  res -= Ad; d = c1 d + c2 D^-1 res; x += d
 //   MODE 2 (first post-smoothing step):  res -= Ad; d = c2 D^-1 res; x += d
 template <int MODE, typename V, typename R, typename X>
-__global__ __launch_bounds__(256) void k_cheb(const int32_t n, const V c1, const V c2, const V* __restrict__ invD,
-                                              const R* __restrict__ r, V* __restrict__ res,
-                                              const V* __restrict__ Ad, V* __restrict__ d, X* __restrict__ x,
-                                              const int* __restrict__ skip)
+__global__ __launch_bounds__(256) void k_cheb(const int32_t n, const int32_t offset, const V c1, const V c2,
+                                              const V* __restrict__ invD, const R* __restrict__ r,
+                                              V* __restrict__ res, const V* __restrict__ Ad, V* __restrict__ d,
+                                              X* __restrict__ x, const int* __restrict__ skip)
 {
   if (skip && *skip) return;
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // node; blockIdx.y = field
   if (i >= n) return;
+  const int64_t id = (int64_t)blockIdx.y * offset + i;
   V ri;
-  if (MODE == 0) ri = (V)r[i];
-  else ri = res[i] - Ad[i];
-  res[i] = ri;
+  if (MODE == 0) ri = (V)r[id];
+  else ri = res[id] - Ad[id];
+  res[id] = ri;
   const V t = c2 * (invD[i] * ri);
-  const V di = (MODE == 1) ? fmaV<V>(c1, d[i], t) : t;
-  d[i] = di;
-  x[i] = (MODE == 0) ? (X)di : x[i] + (X)di;
+  const V di = (MODE == 1) ? fmaV<V>(c1, d[id], t) : t;
+  d[id] = di;
+  x[id] = (MODE == 0) ? (X)di : x[id] + (X)di;
 }
 
 // Restriction of one element: res -= Ad (stored), u = w res, then
 // rc[a,b,c] = sum_k J[k][c] sum_j J[j][b] sum_i J[i][a] u[i,j,k]
 // (node index i + j Nq + k Nq^2; fma chains in ascending fine index).
 template <typename V, typename O>
-__global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nqc, const V* __restrict__ J,
+__global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nqc, const int32_t offf,
+                                                     const int32_t offc, const V* __restrict__ J,
                                                      const double* __restrict__ w, V* __restrict__ res,
                                                      const V* __restrict__ Ad, O* __restrict__ rc,
                                                      const int* __restrict__ skip)
@@ -108,10 +111,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
   const int64_t e = blockIdx.x;
   const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
   for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
+  const int64_t fo = (int64_t)blockIdx.y * offf, fc = (int64_t)blockIdx.y * offc;  // blockIdx.y = field
   for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
     const int64_t id = e * Npf + q;
-    const V rq = res[id] - Ad[id];
-    res[id] = rq;
+    const V rq = res[fo + id] - Ad[fo + id];
+    res[fo + id] = rq;
     s_u[q] = (V)w[id] * rq;
   }
   __syncthreads();
@@ -136,14 +140,15 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
     const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
     V acc = V(0);
     for (int k = 0; k < Nqf; ++k) acc = fmaV<V>(s_J[k * Nqc + c], s_u[ab + Nqc * Nqc * k], acc);
-    rc[e * Npc + q] = (O)acc;
+    rc[fc + e * Npc + q] = (O)acc;
   }
 }
 
 // Prolongation of one element: v[i,j,k] = sum_c J[k][c] sum_b J[j][b] sum_a J[i][a] xc[a,b,c];
 // d = v, x += v.
 template <typename V, typename I, typename X>
-__global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc, const V* __restrict__ J,
+__global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc, const int32_t offf,
+                                                    const int32_t offc, const V* __restrict__ J,
                                                     const I* __restrict__ xc, V* __restrict__ d,
                                                     X* __restrict__ x, const int* __restrict__ skip)
 {
@@ -154,7 +159,8 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc
   const int64_t e = blockIdx.x;
   const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
   for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
-  for (int q = threadIdx.x; q < Npc; q += blockDim.x) s_u[q] = (V)xc[e * Npc + q];
+  const int64_t fo = (int64_t)blockIdx.y * offf, fc = (int64_t)blockIdx.y * offc;  // blockIdx.y = field
+  for (int q = threadIdx.x; q < Npc; q += blockDim.x) s_u[q] = (V)xc[fc + e * Npc + q];
   __syncthreads();
   // x: t1[i + Nqf (b + Nqc c)]
   for (int q = threadIdx.x; q < Nqf * Nqc * Nqc; q += blockDim.x) {
@@ -176,7 +182,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc
     const int ij = q % (Nqf * Nqf), k = q / (Nqf * Nqf);
     V acc = V(0);
     for (int c = 0; c < Nqc; ++c) acc = fmaV<V>(s_J[k * Nqc + c], s_u[ij + Nqf * Nqf * c], acc);
-    const int64_t id = e * Npf + q;
+    const int64_t id = fo + e * Npf + q;
     d[id] = acc;
     x[id] = x[id] + (X)acc;
   }
@@ -263,7 +269,8 @@ int mg_alloc(nbx_mg* mg, P** p, int64_t n, size_t elem = sizeof(P))
   return NBX_SUCCESS;
 }
 
-// lambda_max(D^-1 A) by power iteration: v0 = gs(mask(U[-1,1) seeded)),
+// lambda_max(D^-1 A) by power iteration on field 0 (the vector operator is
+// block-diagonal with equal blocks): v0 = gs(mask(U[-1,1) seeded)),
 // normalised in the weighted norm; v <- D^-1 A v / ||D^-1 A v||_w; the
 // estimate is the last norm.  Norms are taken on the host (setup only).
 int power_lmax(nbx_bp* bp, uint64_t seed, uint64_t stream_id, int iters, double* v, double* w, double* out)
@@ -325,15 +332,15 @@ int level_apply(nbx_mg* mg, nbx_mg_level& L, const double* q, double* Aq, const 
 
 int level_apply(nbx_mg* mg, nbx_mg_level& L, const float* q, float* Aq, const int* skip)
 {
-  nbx_bp* bp = L.bp;
+  nbx_bp* bp = L.bp;  // FP32 operator: Nf fields; bf16 geometry: one field (checked at setup)
   const bool bf = mg->prec == NBX_BF16 || mg->prec == NBX_BF16_MFMA;
   // matrix-core contractions at N = 7; the other levels use the bf16-geometry FP32 kernel
   const nbx_prec p = (mg->prec == NBX_BF16_MFMA && bp->N != 7) ? NBX_BF16 : mg->prec;
-  int rc = axhelm_launch(bp->N, 1, bp->opt.bpmode ? 1 : 0, p, bp->E, bp->fieldOffset, nullptr,
+  int rc = axhelm_launch(bp->N, bp->Nf, bp->opt.bpmode ? 1 : 0, p, bp->E, bp->fieldOffset, nullptr,
                          bf ? (const void*)bp->ggeo16 : (const void*)bp->ggeo32, bp->D32,
                          bf ? (const void*)bp->lambda16 : (const void*)bp->lambda32, q, Aq, skip, bp->stream);
   if (rc) return rc;
-  return gs_add_mask_launch(bp->ogs, Aq, 1, bp->fieldOffset, bp->Nmasked, bp->maskIds, skip, bp->stream);
+  return gs_add_mask_launch(bp->ogs, Aq, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, skip, bp->stream);
 }
 
 template <typename V>
@@ -360,8 +367,9 @@ int vcycle_level(nbx_mg* mg, int l, const R* r, X* x, const int* skip)
   nbx_mg_level& L = mg->lv[l];
   nbx_mg_level& C = mg->lv[l + 1];
   nbx_bp* bp = L.bp;
-  const int32_t n = bp->Nlocal;
-  const dim3 g = mg_grid(n), b(256);
+  const int32_t n = bp->Nlocal, off = bp->fieldOffset;
+  const dim3 g((unsigned)((n + 255) / 256), (unsigned)bp->Nf), b(256);
+  const dim3 ge((unsigned)bp->E, (unsigned)bp->Nf);
   const int k = mg->degree;
   V* res = static_cast<V*>(L.res);
   V* d = static_cast<V*>(L.d);
@@ -369,34 +377,34 @@ int vcycle_level(nbx_mg* mg, int l, const R* r, X* x, const int* skip)
   const V* invD = level_invD<V>(L);
   int rc;
   // pre-smoothing from x = 0
-  hipLaunchKernelGGL((k_cheb<0, V, R, X>), g, b, 0, bp->stream, n, V(0), (V)L.c2[0], invD, r, res,
+  hipLaunchKernelGGL((k_cheb<0, V, R, X>), g, b, 0, bp->stream, n, off, V(0), (V)L.c2[0], invD, r, res,
                      (const V*)nullptr, d, x, skip);
   for (int s = 1; s < k; ++s) {
     if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
-    hipLaunchKernelGGL((k_cheb<1, V, R, X>), g, b, 0, bp->stream, n, (V)L.c1[s], (V)L.c2[s], invD,
+    hipLaunchKernelGGL((k_cheb<1, V, R, X>), g, b, 0, bp->stream, n, off, (V)L.c1[s], (V)L.c2[s], invD,
                        (const R*)nullptr, res, Ad, d, x, skip);
   }
   if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
   // restriction of the smoothed residual
   CV* rcv = static_cast<CV*>(C.r);
-  hipLaunchKernelGGL((k_mg_restrict<V, CV>), dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, level_J<V>(L),
+  hipLaunchKernelGGL((k_mg_restrict<V, CV>), ge, b, 0, bp->stream, L.Nq, C.Nq, off, C.bp->fieldOffset, level_J<V>(L),
                      nbx_ogs_inv_degree(bp->ogs), res, Ad, rcv, skip);
   NBX_HIP_LAUNCH();
   nbx_bp* bc = C.bp;
-  if ((rc = gs_add_mask_launch(bc->ogs, rcv, 1, bc->fieldOffset, bc->Nmasked, bc->maskIds, skip, bc->stream)))
+  if ((rc = gs_add_mask_launch(bc->ogs, rcv, bc->Nf, bc->fieldOffset, bc->Nmasked, bc->maskIds, skip, bc->stream)))
     return rc;
   if ((rc = vcycle<V>(mg, l + 1, C.r, C.x, skip))) return rc;
   // coarse correction
-  hipLaunchKernelGGL((k_mg_prolong<V, CV, X>), dim3(bp->E), b, 0, bp->stream, L.Nq, C.Nq, level_J<V>(L),
+  hipLaunchKernelGGL((k_mg_prolong<V, CV, X>), ge, b, 0, bp->stream, L.Nq, C.Nq, off, C.bp->fieldOffset, level_J<V>(L),
                      static_cast<const CV*>(C.x), d, x, skip);
   NBX_HIP_LAUNCH();
   if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
   // post-smoothing
-  hipLaunchKernelGGL((k_cheb<2, V, R, X>), g, b, 0, bp->stream, n, V(0), (V)L.c2[0], invD, (const R*)nullptr, res,
+  hipLaunchKernelGGL((k_cheb<2, V, R, X>), g, b, 0, bp->stream, n, off, V(0), (V)L.c2[0], invD, (const R*)nullptr, res,
                      Ad, d, x, skip);
   for (int s = 1; s < k; ++s) {
     if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
-    hipLaunchKernelGGL((k_cheb<1, V, R, X>), g, b, 0, bp->stream, n, (V)L.c1[s], (V)L.c2[s], invD,
+    hipLaunchKernelGGL((k_cheb<1, V, R, X>), g, b, 0, bp->stream, n, off, (V)L.c1[s], (V)L.c2[s], invD,
                        (const R*)nullptr, res, Ad, d, x, skip);
   }
   NBX_HIP_LAUNCH();
@@ -453,7 +461,8 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
     return fail(NBX_ERR_INVALID, "bp_mg_setup: bad arguments");
   if (prec != NBX_F64 && prec != NBX_F32 && prec != NBX_BF16 && prec != NBX_BF16_MFMA)
     return fail(NBX_ERR_UNSUPPORTED, "bp_mg_setup: precision %d (F64, F32, BF16 or BF16_MFMA)", (int)prec);
-  if (bp->Nf != 1) return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: one field");
+  if ((prec == NBX_BF16 || prec == NBX_BF16_MFMA) && bp->Nf != 1)
+    return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: bf16 geometry with one field only");
   if (orders[0] != bp->N) return fail(NBX_ERR_INVALID, "bp_mg_setup: orders[0] must be the handle's N");
   for (int l = 1; l < nlevels; ++l)
     if (orders[l] < 1 || orders[l] >= orders[l - 1])

@@ -353,13 +353,13 @@ class OracleMG:
     matrices, power-iteration lambda_max and Chebyshev coefficients."""
 
     def __init__(self, box: Box, orders, degree=2, coarse_iters=20, lo=0.1, hi=1.1, bpmode=True, lambda1=0.0,
-                 dirichlet=True, power_iters=20, prec=0):
+                 dirichlet=True, power_iters=20, prec=0, Nf=1):
         assert orders[0] == box.N
         self.levels = []
         for N in orders:
             b = Box.make(N, box.NX, box.NY, box.NZ, box.periodic, box.perturb, box.zaspect, box.seed,
                          (box.x0, box.x1, box.y0, box.y1, box.z0, box.z1))
-            self.levels.append(OracleBP(b, bpmode=bpmode, lambda1=lambda1, dirichlet=dirichlet))
+            self.levels.append(OracleBP(b, Nf=Nf, bpmode=bpmode, lambda1=lambda1, dirichlet=dirichlet))
         n = len(orders)
         self.fine = self.levels[0]
         self.bps = (BPStruct * n)(*[lv.struct for lv in self.levels])

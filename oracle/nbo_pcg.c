@@ -205,11 +205,14 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
     }
     if (ln) TILE_TREE(3, N, d, {
       const double wid = bp->w[id];
-      const double rf = r[id];
-      const double zf = z[id];
-      acc[0] = fma(wid * rf, zf, acc[0]);
-      acc[1] = fma(wid * rf, rf, acc[1]);
-      acc[2] = fma(wid * zf, Ap[id], acc[2]);
+      for (int f = 0; f < bp->Nf; ++f) {
+        const int64_t iid = id + (int64_t)f * bp->offset;
+        const double rf = r[iid];
+        const double zf = z[iid];
+        acc[0] = fma(wid * rf, zf, acc[0]);
+        acc[1] = fma(wid * rf, rf, acc[1]);
+        acc[2] = fma(wid * zf, Ap[iid], acc[2]);
+      }
     });
     else TILE_TREE(3, N, d, {
       const double wid = bp->w[id];
@@ -403,8 +406,12 @@ void nbo_mg_cheb(double lmax, double lo, double hi, int degree, double* c1, doub
 
 /* lambda_max(D^-1 A): v0 = gs(mask(U[-1,1))) normalised in the weighted
  * norm, then v <- D^-1 A v / ||D^-1 A v||_w; the last norm */
-double nbo_mg_power(const nbo_bp* bp, const double* invD, uint64_t seed, uint64_t stream, int iters)
+double nbo_mg_power(const nbo_bp* bp_in, const double* invD, uint64_t seed, uint64_t stream, int iters)
 {
+  /* field 0 only: the vector operator is block-diagonal with equal blocks */
+  nbo_bp one = *bp_in;
+  one.Nf = 1;
+  const nbo_bp* bp = &one;
   const int64_t n = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
   double* v = (double*)calloc(bp->offset, sizeof(double));
   double* w = (double*)calloc(bp->offset, sizeof(double));
@@ -440,11 +447,13 @@ void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
   const nbo_bp* bp = &mg->bps[l];
   const int Nq = bp->N + 1, Np = Nq * Nq * Nq;
   const int64_t n = (int64_t)bp->E * Np;
+  const int Nf = bp->Nf;
+  const int64_t Nall = (int64_t)Nf * bp->offset;
   const double* invD = mg->invD[l];
   if (l == mg->nlev - 1) {  /* coarse: fixed-count Jacobi PCG from x = 0 (r is copied) */
-    double* rr = (double*)malloc(sizeof(double) * bp->offset);
-    memcpy(rr, r, sizeof(double) * bp->offset);
-    memset(x, 0, sizeof(double) * bp->offset);
+    double* rr = (double*)malloc(sizeof(double) * Nall);
+    memcpy(rr, r, sizeof(double) * Nall);
+    memset(x, 0, sizeof(double) * Nall);
     nbo_pcg_stats st;
     pcg_core(bp, 1, 0, 1, NULL, invD, rr, x, 0.0, 0.0, mg->coarse_iters, 0, NULL, NULL, &st);
     free(rr);
@@ -456,111 +465,122 @@ void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
   const double* c1 = mg->c1 + (size_t)l * mg->degree;
   const double* c2 = mg->c2 + (size_t)l * mg->degree;
   const int k = mg->degree;
-  double* res = (double*)calloc(bp->offset, sizeof(double));
-  double* d = (double*)calloc(bp->offset, sizeof(double));
-  double* Ad = (double*)calloc(bp->offset, sizeof(double));
-  double* rc = (double*)calloc(bc->offset, sizeof(double));
-  double* xc = (double*)calloc(bc->offset, sizeof(double));
+  double* res = (double*)calloc(Nall, sizeof(double));
+  double* d = (double*)calloc(Nall, sizeof(double));
+  double* Ad = (double*)calloc(Nall, sizeof(double));
+  double* rc = (double*)calloc((size_t)Nf * bc->offset, sizeof(double));
+  double* xc = (double*)calloc((size_t)Nf * bc->offset, sizeof(double));
   /* pre-smoothing from x = 0 */
-  for (int64_t i = 0; i < n; ++i) {
-    const double ri = r[i];
-    res[i] = ri;
-    const double t = c2[0] * (invD[i] * ri);
-    d[i] = t;
-    x[i] = t;
-  }
+  for (int f = 0; f < Nf; ++f)
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t id = i + (int64_t)f * bp->offset;
+      const double ri = r[id];
+      res[id] = ri;
+      const double t = c2[0] * (invD[i] * ri);
+      d[id] = t;
+      x[id] = t;
+    }
   for (int s = 1; s < k; ++s) {
     nbo_bp_apply(bp, d, Ad);
-    for (int64_t i = 0; i < n; ++i) {
-      const double ri = res[i] - Ad[i];
-      res[i] = ri;
-      const double t = c2[s] * (invD[i] * ri);
-      d[i] = fma(c1[s], d[i], t);
-      x[i] = x[i] + d[i];
-    }
+    for (int f = 0; f < Nf; ++f)
+      for (int64_t i = 0; i < n; ++i) {
+        const int64_t id = i + (int64_t)f * bp->offset;
+        const double ri = res[id] - Ad[id];
+        res[id] = ri;
+        const double t = c2[s] * (invD[i] * ri);
+        d[id] = fma(c1[s], d[id], t);
+        x[id] = x[id] + d[id];
+      }
   }
   nbo_bp_apply(bp, d, Ad);
-  /* restriction: res -= Ad; rc = J^T (w res) per element, x then y then z */
+  /* restriction: res -= Ad; rc = J^T (w res) per element and field, x then y then z */
 #pragma omp parallel
   {
     double* u = (double*)malloc(sizeof(double) * Np);
     double* t1 = (double*)malloc(sizeof(double) * Np);
     double* t2 = (double*)malloc(sizeof(double) * Np);
-#pragma omp for schedule(static)
-    for (int32_t e = 0; e < bp->E; ++e) {
-      for (int q = 0; q < Np; ++q) {
-        const int64_t id = (int64_t)e * Np + q;
-        const double rq = res[id] - Ad[id];
-        res[id] = rq;
-        u[q] = bp->w[id] * rq;
+#pragma omp for schedule(static) collapse(2)
+    for (int f = 0; f < Nf; ++f)
+      for (int32_t e = 0; e < bp->E; ++e) {
+        const int64_t fo = (int64_t)f * bp->offset, fc = (int64_t)f * bc->offset;
+        for (int q = 0; q < Np; ++q) {
+          const int64_t id = (int64_t)e * Np + q;
+          const double rq = res[fo + id] - Ad[fo + id];
+          res[fo + id] = rq;
+          u[q] = bp->w[id] * rq;
+        }
+        for (int q = 0; q < Nqc * Nq * Nq; ++q) {
+          const int a = q % Nqc, jk = q / Nqc;
+          double acc = 0.0;
+          for (int i = 0; i < Nq; ++i) acc = fma(J[i * Nqc + a], u[i + Nq * jk], acc);
+          t1[q] = acc;
+        }
+        for (int q = 0; q < Nqc * Nqc * Nq; ++q) {
+          const int a = q % Nqc, b = (q / Nqc) % Nqc, kk = q / (Nqc * Nqc);
+          double acc = 0.0;
+          for (int j = 0; j < Nq; ++j) acc = fma(J[j * Nqc + b], t1[a + Nqc * (j + Nq * kk)], acc);
+          t2[q] = acc;
+        }
+        for (int q = 0; q < Npc; ++q) {
+          const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
+          double acc = 0.0;
+          for (int kk = 0; kk < Nq; ++kk) acc = fma(J[kk * Nqc + c], t2[ab + Nqc * Nqc * kk], acc);
+          rc[fc + (int64_t)e * Npc + q] = acc;
+        }
       }
-      for (int q = 0; q < Nqc * Nq * Nq; ++q) {
-        const int a = q % Nqc, jk = q / Nqc;
-        double acc = 0.0;
-        for (int i = 0; i < Nq; ++i) acc = fma(J[i * Nqc + a], u[i + Nq * jk], acc);
-        t1[q] = acc;
-      }
-      for (int q = 0; q < Nqc * Nqc * Nq; ++q) {
-        const int a = q % Nqc, b = (q / Nqc) % Nqc, kk = q / (Nqc * Nqc);
-        double acc = 0.0;
-        for (int j = 0; j < Nq; ++j) acc = fma(J[j * Nqc + b], t1[a + Nqc * (j + Nq * kk)], acc);
-        t2[q] = acc;
-      }
-      for (int q = 0; q < Npc; ++q) {
-        const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
-        double acc = 0.0;
-        for (int kk = 0; kk < Nq; ++kk) acc = fma(J[kk * Nqc + c], t2[ab + Nqc * Nqc * kk], acc);
-        rc[(int64_t)e * Npc + q] = acc;
-      }
-    }
     free(u); free(t1); free(t2);
   }
-  nbo_gs_many(bc->ogs, rc, 1, bc->offset, NBO_DOUBLE, NBO_ADD);
-  for (int32_t m = 0; m < bc->Nmasked; ++m) rc[bc->maskIds[m]] = 0.0;
+  nbo_gs_many(bc->ogs, rc, Nf, bc->offset, NBO_DOUBLE, NBO_ADD);
+  for (int f = 0; f < Nf; ++f)
+    for (int32_t m = 0; m < bc->Nmasked; ++m) rc[bc->maskIds[m] + (int64_t)f * bc->offset] = 0.0;
   nbo_mg_vcycle(mg, l + 1, rc, xc);
   /* prolongation: d = J xc; x += d */
 #pragma omp parallel
   {
     double* t1 = (double*)malloc(sizeof(double) * Np);
     double* t2 = (double*)malloc(sizeof(double) * Np);
-#pragma omp for schedule(static)
-    for (int32_t e = 0; e < bp->E; ++e) {
-      const double* uc = xc + (int64_t)e * Npc;
-      for (int q = 0; q < Nq * Nqc * Nqc; ++q) {
-        const int i = q % Nq, bc_ = q / Nq;
-        double acc = 0.0;
-        for (int a = 0; a < Nqc; ++a) acc = fma(J[i * Nqc + a], uc[a + Nqc * bc_], acc);
-        t1[q] = acc;
+#pragma omp for schedule(static) collapse(2)
+    for (int f = 0; f < Nf; ++f)
+      for (int32_t e = 0; e < bp->E; ++e) {
+        const int64_t fo = (int64_t)f * bp->offset, fc = (int64_t)f * bc->offset;
+        const double* uc = xc + fc + (int64_t)e * Npc;
+        for (int q = 0; q < Nq * Nqc * Nqc; ++q) {
+          const int i = q % Nq, bc_ = q / Nq;
+          double acc = 0.0;
+          for (int a = 0; a < Nqc; ++a) acc = fma(J[i * Nqc + a], uc[a + Nqc * bc_], acc);
+          t1[q] = acc;
+        }
+        for (int q = 0; q < Nq * Nq * Nqc; ++q) {
+          const int i = q % Nq, j = (q / Nq) % Nq, c = q / (Nq * Nq);
+          double acc = 0.0;
+          for (int b = 0; b < Nqc; ++b) acc = fma(J[j * Nqc + b], t1[i + Nq * (b + Nqc * c)], acc);
+          t2[q] = acc;
+        }
+        for (int q = 0; q < Np; ++q) {
+          const int ij = q % (Nq * Nq), kk = q / (Nq * Nq);
+          double acc = 0.0;
+          for (int c = 0; c < Nqc; ++c) acc = fma(J[kk * Nqc + c], t2[ij + Nq * Nq * c], acc);
+          const int64_t id = fo + (int64_t)e * Np + q;
+          d[id] = acc;
+          x[id] = x[id] + acc;
+        }
       }
-      for (int q = 0; q < Nq * Nq * Nqc; ++q) {
-        const int i = q % Nq, j = (q / Nq) % Nq, c = q / (Nq * Nq);
-        double acc = 0.0;
-        for (int b = 0; b < Nqc; ++b) acc = fma(J[j * Nqc + b], t1[i + Nq * (b + Nqc * c)], acc);
-        t2[q] = acc;
-      }
-      for (int q = 0; q < Np; ++q) {
-        const int ij = q % (Nq * Nq), kk = q / (Nq * Nq);
-        double acc = 0.0;
-        for (int c = 0; c < Nqc; ++c) acc = fma(J[kk * Nqc + c], t2[ij + Nq * Nq * c], acc);
-        const int64_t id = (int64_t)e * Np + q;
-        d[id] = acc;
-        x[id] = x[id] + acc;
-      }
-    }
     free(t1); free(t2);
   }
   nbo_bp_apply(bp, d, Ad);
   /* post-smoothing */
   for (int s = 0; s < k; ++s) {
     if (s > 0) nbo_bp_apply(bp, d, Ad);
-    for (int64_t i = 0; i < n; ++i) {
-      const double ri = res[i] - Ad[i];
-      res[i] = ri;
-      const double t = c2[s] * (invD[i] * ri);
-      const double di = s == 0 ? t : fma(c1[s], d[i], t);
-      d[i] = di;
-      x[i] = x[i] + di;
-    }
+    for (int f = 0; f < Nf; ++f)
+      for (int64_t i = 0; i < n; ++i) {
+        const int64_t id = i + (int64_t)f * bp->offset;
+        const double ri = res[id] - Ad[id];
+        res[id] = ri;
+        const double t = c2[s] * (invD[i] * ri);
+        const double di = s == 0 ? t : fma(c1[s], d[id], t);
+        d[id] = di;
+        x[id] = x[id] + di;
+      }
   }
   free(res); free(d); free(Ad); free(rc); free(xc);
 }
@@ -584,12 +604,13 @@ int nbo_bp_pcg_mg(const nbo_mg* mg, double* r, double* x, double tol, int maxit,
 static void mg_apply_f(const nbo_mg* mg, int l, const float* q, float* Aq)
 {
   const nbo_bp* bp = &mg->bps[l];
-  if (mg->prec == 3 && bp->N == 7)  /* matrix-core contractions at N = 7 */
+  if (mg->prec == 3 && bp->N == 7)  /* matrix-core contractions at N = 7 (one field) */
     nbo_axhelm_bf16mma(bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
   else
-    nbo_axhelm_f32(bp->N, 1, bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
-  nbo_gs_many(bp->ogs, Aq, 1, bp->offset, NBO_FLOAT, NBO_ADD);
-  for (int32_t m = 0; m < bp->Nmasked; ++m) Aq[bp->maskIds[m]] = 0.f;
+    nbo_axhelm_f32(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, mg->ggeo32[l], mg->D32[l], mg->lam32[l], q, Aq);
+  nbo_gs_many(bp->ogs, Aq, bp->Nf, bp->offset, NBO_FLOAT, NBO_ADD);
+  for (int f = 0; f < bp->Nf; ++f)
+    for (int32_t m = 0; m < bp->Nmasked; ++m) Aq[bp->maskIds[m] + (int64_t)f * bp->offset] = 0.f;
 }
 
 static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
@@ -598,6 +619,8 @@ static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
   const nbo_bp* bc = &mg->bps[l + 1];
   const int Nq = bp->N + 1, Np = Nq * Nq * Nq, Nqc = bc->N + 1, Npc = Nqc * Nqc * Nqc;
   const int64_t n = (int64_t)bp->E * Np;
+  const int Nf = bp->Nf;
+  const int64_t Nall = (int64_t)Nf * bp->offset, Ncall = (int64_t)Nf * bc->offset;
   const int rdbl = l == 0, cnext = l + 2 == mg->nlev;
   const float* invD = mg->invD32[l];
   const float* J = mg->J32[l];
@@ -608,33 +631,36 @@ static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
   const float* rF = (const float*)r;
   double* xD = (double*)x;
   float* xF = (float*)x;
-  float* res = (float*)calloc(bp->offset, sizeof(float));
-  float* d = (float*)calloc(bp->offset, sizeof(float));
-  float* Ad = (float*)calloc(bp->offset, sizeof(float));
+  float* res = (float*)calloc(Nall, sizeof(float));
+  float* d = (float*)calloc(Nall, sizeof(float));
+  float* Ad = (float*)calloc(Nall, sizeof(float));
   const size_t ces = cnext ? sizeof(double) : sizeof(float);
-  void* rc = calloc(bc->offset, ces);
-  void* xc = calloc(bc->offset, ces);
+  void* rc = calloc(Ncall, ces);
+  void* xc = calloc(Ncall, ces);
 #define XADD(i, v) do { if (rdbl) xD[i] = xD[i] + (double)(v); else xF[i] = xF[i] + (v); } while (0)
+#define FIELDS for (int f = 0; f < Nf; ++f) for (int64_t i = 0; i < n; ++i)
   {
     const float c2f = (float)c2[0];
-    for (int64_t i = 0; i < n; ++i) {
-      const float ri = rdbl ? (float)rD[i] : rF[i];
-      res[i] = ri;
+    FIELDS {
+      const int64_t id = i + (int64_t)f * bp->offset;
+      const float ri = rdbl ? (float)rD[id] : rF[id];
+      res[id] = ri;
       const float t = c2f * (invD[i] * ri);
-      d[i] = t;
-      if (rdbl) xD[i] = (double)t; else xF[i] = t;
+      d[id] = t;
+      if (rdbl) xD[id] = (double)t; else xF[id] = t;
     }
   }
   for (int s = 1; s < k; ++s) {
     mg_apply_f(mg, l, d, Ad);
     const float c1f = (float)c1[s], c2f = (float)c2[s];
-    for (int64_t i = 0; i < n; ++i) {
-      const float ri = res[i] - Ad[i];
-      res[i] = ri;
+    FIELDS {
+      const int64_t id = i + (int64_t)f * bp->offset;
+      const float ri = res[id] - Ad[id];
+      res[id] = ri;
       const float t = c2f * (invD[i] * ri);
-      const float di = fmaf(c1f, d[i], t);
-      d[i] = di;
-      XADD(i, di);
+      const float di = fmaf(c1f, d[id], t);
+      d[id] = di;
+      XADD(id, di);
     }
   }
   mg_apply_f(mg, l, d, Ad);
@@ -643,43 +669,47 @@ static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
     float* u = (float*)malloc(sizeof(float) * Np);
     float* t1 = (float*)malloc(sizeof(float) * Np);
     float* t2 = (float*)malloc(sizeof(float) * Np);
-#pragma omp for schedule(static)
-    for (int32_t e = 0; e < bp->E; ++e) {
-      for (int q = 0; q < Np; ++q) {
-        const int64_t id = (int64_t)e * Np + q;
-        const float rq = res[id] - Ad[id];
-        res[id] = rq;
-        u[q] = (float)bp->w[id] * rq;
+#pragma omp for schedule(static) collapse(2)
+    for (int f = 0; f < Nf; ++f)
+      for (int32_t e = 0; e < bp->E; ++e) {
+        const int64_t fo = (int64_t)f * bp->offset, fc = (int64_t)f * bc->offset;
+        for (int q = 0; q < Np; ++q) {
+          const int64_t id = (int64_t)e * Np + q;
+          const float rq = res[fo + id] - Ad[fo + id];
+          res[fo + id] = rq;
+          u[q] = (float)bp->w[id] * rq;
+        }
+        for (int q = 0; q < Nqc * Nq * Nq; ++q) {
+          const int a = q % Nqc, jk = q / Nqc;
+          float acc = 0.f;
+          for (int i = 0; i < Nq; ++i) acc = fmaf(J[i * Nqc + a], u[i + Nq * jk], acc);
+          t1[q] = acc;
+        }
+        for (int q = 0; q < Nqc * Nqc * Nq; ++q) {
+          const int a = q % Nqc, b = (q / Nqc) % Nqc, kk = q / (Nqc * Nqc);
+          float acc = 0.f;
+          for (int j = 0; j < Nq; ++j) acc = fmaf(J[j * Nqc + b], t1[a + Nqc * (j + Nq * kk)], acc);
+          t2[q] = acc;
+        }
+        for (int q = 0; q < Npc; ++q) {
+          const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
+          float acc = 0.f;
+          for (int kk = 0; kk < Nq; ++kk) acc = fmaf(J[kk * Nqc + c], t2[ab + Nqc * Nqc * kk], acc);
+          if (cnext) ((double*)rc)[fc + (int64_t)e * Npc + q] = (double)acc;
+          else ((float*)rc)[fc + (int64_t)e * Npc + q] = acc;
+        }
       }
-      for (int q = 0; q < Nqc * Nq * Nq; ++q) {
-        const int a = q % Nqc, jk = q / Nqc;
-        float acc = 0.f;
-        for (int i = 0; i < Nq; ++i) acc = fmaf(J[i * Nqc + a], u[i + Nq * jk], acc);
-        t1[q] = acc;
-      }
-      for (int q = 0; q < Nqc * Nqc * Nq; ++q) {
-        const int a = q % Nqc, b = (q / Nqc) % Nqc, kk = q / (Nqc * Nqc);
-        float acc = 0.f;
-        for (int j = 0; j < Nq; ++j) acc = fmaf(J[j * Nqc + b], t1[a + Nqc * (j + Nq * kk)], acc);
-        t2[q] = acc;
-      }
-      for (int q = 0; q < Npc; ++q) {
-        const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
-        float acc = 0.f;
-        for (int kk = 0; kk < Nq; ++kk) acc = fmaf(J[kk * Nqc + c], t2[ab + Nqc * Nqc * kk], acc);
-        if (cnext) ((double*)rc)[(int64_t)e * Npc + q] = (double)acc;
-        else ((float*)rc)[(int64_t)e * Npc + q] = acc;
-      }
-    }
     free(u); free(t1); free(t2);
   }
   if (cnext) {
-    nbo_gs_many(bc->ogs, rc, 1, bc->offset, NBO_DOUBLE, NBO_ADD);
-    for (int32_t m = 0; m < bc->Nmasked; ++m) ((double*)rc)[bc->maskIds[m]] = 0.0;
+    nbo_gs_many(bc->ogs, rc, Nf, bc->offset, NBO_DOUBLE, NBO_ADD);
+    for (int f = 0; f < Nf; ++f)
+      for (int32_t m = 0; m < bc->Nmasked; ++m) ((double*)rc)[bc->maskIds[m] + (int64_t)f * bc->offset] = 0.0;
     nbo_mg_vcycle(mg, l + 1, (const double*)rc, (double*)xc);
   } else {
-    nbo_gs_many(bc->ogs, rc, 1, bc->offset, NBO_FLOAT, NBO_ADD);
-    for (int32_t m = 0; m < bc->Nmasked; ++m) ((float*)rc)[bc->maskIds[m]] = 0.f;
+    nbo_gs_many(bc->ogs, rc, Nf, bc->offset, NBO_FLOAT, NBO_ADD);
+    for (int f = 0; f < Nf; ++f)
+      for (int32_t m = 0; m < bc->Nmasked; ++m) ((float*)rc)[bc->maskIds[m] + (int64_t)f * bc->offset] = 0.f;
     mg_vcycle_f(mg, l + 1, rc, xc);
   }
 #pragma omp parallel
@@ -687,46 +717,50 @@ static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
     float* u = (float*)malloc(sizeof(float) * Npc);
     float* t1 = (float*)malloc(sizeof(float) * Np);
     float* t2 = (float*)malloc(sizeof(float) * Np);
-#pragma omp for schedule(static)
-    for (int32_t e = 0; e < bp->E; ++e) {
-      for (int q = 0; q < Npc; ++q)
-        u[q] = cnext ? (float)((double*)xc)[(int64_t)e * Npc + q] : ((float*)xc)[(int64_t)e * Npc + q];
-      for (int q = 0; q < Nq * Nqc * Nqc; ++q) {
-        const int i = q % Nq, bc_ = q / Nq;
-        float acc = 0.f;
-        for (int a = 0; a < Nqc; ++a) acc = fmaf(J[i * Nqc + a], u[a + Nqc * bc_], acc);
-        t1[q] = acc;
+#pragma omp for schedule(static) collapse(2)
+    for (int f = 0; f < Nf; ++f)
+      for (int32_t e = 0; e < bp->E; ++e) {
+        const int64_t fo = (int64_t)f * bp->offset, fc = (int64_t)f * bc->offset;
+        for (int q = 0; q < Npc; ++q)
+          u[q] = cnext ? (float)((double*)xc)[fc + (int64_t)e * Npc + q] : ((float*)xc)[fc + (int64_t)e * Npc + q];
+        for (int q = 0; q < Nq * Nqc * Nqc; ++q) {
+          const int i = q % Nq, bc_ = q / Nq;
+          float acc = 0.f;
+          for (int a = 0; a < Nqc; ++a) acc = fmaf(J[i * Nqc + a], u[a + Nqc * bc_], acc);
+          t1[q] = acc;
+        }
+        for (int q = 0; q < Nq * Nq * Nqc; ++q) {
+          const int i = q % Nq, j = (q / Nq) % Nq, c = q / (Nq * Nq);
+          float acc = 0.f;
+          for (int b = 0; b < Nqc; ++b) acc = fmaf(J[j * Nqc + b], t1[i + Nq * (b + Nqc * c)], acc);
+          t2[q] = acc;
+        }
+        for (int q = 0; q < Np; ++q) {
+          const int ij = q % (Nq * Nq), kk = q / (Nq * Nq);
+          float acc = 0.f;
+          for (int c = 0; c < Nqc; ++c) acc = fmaf(J[kk * Nqc + c], t2[ij + Nq * Nq * c], acc);
+          const int64_t id = fo + (int64_t)e * Np + q;
+          d[id] = acc;
+          XADD(id, acc);
+        }
       }
-      for (int q = 0; q < Nq * Nq * Nqc; ++q) {
-        const int i = q % Nq, j = (q / Nq) % Nq, c = q / (Nq * Nq);
-        float acc = 0.f;
-        for (int b = 0; b < Nqc; ++b) acc = fmaf(J[j * Nqc + b], t1[i + Nq * (b + Nqc * c)], acc);
-        t2[q] = acc;
-      }
-      for (int q = 0; q < Np; ++q) {
-        const int ij = q % (Nq * Nq), kk = q / (Nq * Nq);
-        float acc = 0.f;
-        for (int c = 0; c < Nqc; ++c) acc = fmaf(J[kk * Nqc + c], t2[ij + Nq * Nq * c], acc);
-        const int64_t id = (int64_t)e * Np + q;
-        d[id] = acc;
-        XADD(id, acc);
-      }
-    }
     free(u); free(t1); free(t2);
   }
   mg_apply_f(mg, l, d, Ad);
   for (int s = 0; s < k; ++s) {
     if (s > 0) mg_apply_f(mg, l, d, Ad);
     const float c1f = (float)c1[s], c2f = (float)c2[s];
-    for (int64_t i = 0; i < n; ++i) {
-      const float ri = res[i] - Ad[i];
-      res[i] = ri;
+    FIELDS {
+      const int64_t id = i + (int64_t)f * bp->offset;
+      const float ri = res[id] - Ad[id];
+      res[id] = ri;
       const float t = c2f * (invD[i] * ri);
-      const float di = s == 0 ? t : fmaf(c1f, d[i], t);
-      d[i] = di;
-      XADD(i, di);
+      const float di = s == 0 ? t : fmaf(c1f, d[id], t);
+      d[id] = di;
+      XADD(id, di);
     }
   }
+#undef FIELDS
 #undef XADD
   free(res); free(d); free(Ad); free(rc); free(xc);
 }

@@ -124,4 +124,28 @@ def test_mg_argument_checks(nbx, dev):
         bp.mg_setup([5, 2, 1], prec=nbx.NBX_F16)
     b3 = nbx.BP(nbx.Box.make(5, 2, 2, 2), Nfields=3, precon=nbx.PRECON_MG)
     with pytest.raises(nbx.NbxError):
-        b3.mg_setup([5, 2, 1])
+        b3.mg_setup([5, 2, 1], prec=nbx.NBX_BF16)  # bf16 geometry: one field
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_mg_three_fields_match_oracle(nbx, nbo, dev, prec):
+    """BP6 (three fields, non-BK Helmholtz): the cycle acts field by field
+    with one diagonal and one lambda_max; V-cycle and PCG iterates bit for bit."""
+    import torch
+    kw = dict(perturb=0.1, seed=61)
+    bp = nbx.BP(nbx.Box.make(4, 3, 3, 2, **kw), Nfields=3, bpmode=False, lambda1=0.1, precon=nbx.PRECON_MG)
+    bp.mg_setup([4, 2, 1], 2, 12, prec=PRECS[prec])
+    mg = nbo.OracleMG(nbo.Box.make(4, 3, 3, 2, **kw), [4, 2, 1], degree=2, coarse_iters=12, bpmode=False,
+                      lambda1=0.1, prec=PRECS[prec], Nf=3)
+    n = 3 * bp.fieldOffset
+    r = mg.fine.rhs.copy()
+    z = torch.zeros(n, dtype=torch.float64, device=dev)
+    bp.mg_apply(torch.tensor(r, device=dev), z)
+    assert np.array_equal(H(z), mg.vcycle(r))
+    rr = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(rr)
+    st = bp.pcg(rr, x, 1e-10, 200)
+    xo = np.zeros(n)
+    ost, _, _ = mg.pcg(mg.fine.rhs.copy(), xo, 1e-10, 200)
+    assert st.iterations == ost.iterations and st.converged
+    assert np.array_equal(H(x), xo)
