@@ -314,6 +314,64 @@ def run_config5(nbx, torch, args):
             "elements": 64 * 64 * 32, "learned_training": info, "solvers": out}
 
 
+def config1_measurements(nbx, torch, args, log):
+    """Config 1 mesh (16^3 elements, N=7, perturbed, seed 1): FP64 BK axhelm
+    with 5 rotating buffer sets of 151 MB each (755 MB, beyond the 256 MB
+    Infinity Cache), HIP events on the launch stream; and time to a 1e-8
+    relative residual of FP64 Jacobi PCG against the multigrid solvers on the
+    Dirichlet BP5 problem of the same mesh."""
+    box = nbx.Box.make(7, 16, 16, 16, perturb=0.1, seed=1)
+    bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, fused=True)
+    E, Np, n = bj.Nelements, bj.Np, bj.fieldOffset
+    g0 = bj.device_array("ggeo", E * 7 * Np)
+    D = bj.device_array("D", 64)
+    sets = []
+    for k in range(5):
+        sets.append((g0.clone(), torch.rand(n, dtype=torch.float64, device="cuda") * 2 - 1,
+                     torch.empty(n, dtype=torch.float64, device="cuda")))
+    s = torch.cuda.current_stream()
+    for g, q, a in sets:
+        nbx.axhelm(7, 1, 1, E, n, g, D, None, q, a, stream=s)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    ev0.record(s)
+    for _ in range(reps):
+        for g, q, a in sets:
+            nbx.axhelm(7, 1, 1, E, n, g, D, None, q, a, stream=s)
+    ev1.record(s)
+    ev1.synchronize()
+    t_ax = ev0.elapsed_time(ev1) / 1e3 / (reps * len(sets))
+    bytes_ = E * Np * 9 * 8
+    out = {"axhelm_fp64_us": round(t_ax * 1e6, 2), "axhelm_GB_s": round(bytes_ / t_ax / 1e9, 1),
+           "axhelm_frac_of_8TBs": round(bytes_ / t_ax / 8e12, 4), "axhelm_gdofs": round(E * 343 / t_ax / 1e9, 2),
+           "buffer_sets": len(sets), "bytes_per_apply": bytes_}
+    del sets
+    log(f"[bench] config 1 axhelm: {out}")
+    b = bj.device_array("rhs", n).clone()
+    r, x = torch.empty_like(b), torch.empty_like(b)
+    sync = torch.cuda.synchronize
+    res = {}
+
+    def run(name, h):
+        for _ in range(2):
+            r.copy_(b)
+            x.zero_()
+            st, t = timed(lambda: h.pcg(r, x, 1e-8, 20000), sync)
+        res[name] = {"iterations": st.iterations, "time_s": round(t, 5), "converged": bool(st.converged)}
+
+    run("fp64_jacobi", bj)
+    bm = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
+    for name, prec in (("fp64_mg", nbx.NBX_F64), ("mixed_mg_f32cycle", nbx.NBX_F32),
+                       ("mixed_mg_bf16cycle", nbx.NBX_BF16)):
+        bm.mg_setup([7, 3, 1], degree=3, coarse_iters=20, prec=prec)
+        run(name, bm)
+    for v in res.values():
+        v["speedup_vs_fp64_jacobi"] = round(res["fp64_jacobi"]["time_s"] / v["time_s"], 3)
+    out["time_to_1e-8"] = res
+    log(f"[bench] config 1 time to 1e-8: {res}")
+    return out
+
+
 def profile_mg_pass(nbx, torch, NX):
     import numpy as np
     box = nbx.Box.make(7, NX, NX, NX, perturb=0.1, seed=1)
@@ -473,6 +531,8 @@ def main():
     log(f"[bench] axhelm variants: {variants}")
     del g32, g16, q32, A32
 
+    config1 = None if args.no_compare else config1_measurements(nbx, torch, args, log)
+
     compare = None
     if not args.no_compare:
         t1 = time.perf_counter()
@@ -497,7 +557,7 @@ def main():
                    "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
                    "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "axhelm_variants": variants,
                    "parallelism": "single GPU",
-                   "time_to_1e-8": compare},
+                   "time_to_1e-8": compare, "config1_16cube": config1},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_axhelm<double,8,1,BK>", "bytes_per_launch": ax_bytes,

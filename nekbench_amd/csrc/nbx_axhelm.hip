@@ -76,7 +76,9 @@ struct AxLane {
 
 // One field of one element: out[k] = (A q)(i,j,k) for the lane's column rq.
 // g, l0, l1 point at the lane's node (k = 0) of ggeo, lambda0, lambda1.
-template <typename T, int NQ, bool BK, typename G = T>
+// DL: read the lane's D vectors from LDS instead of registers (frees 4*NQ
+// registers; used where that buys a fourth wave per SIMD)
+template <typename T, int NQ, bool BK, typename G = T, bool DL = false>
 __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&rq)[NQ], T (&out)[NQ],
                                               const T* __restrict__ s_D, T* s_q, T* s_Gr, T* s_Gs,
                                               const G* __restrict__ g, const G* __restrict__ l0,
@@ -95,8 +97,8 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
     T qr = T(0), qs = T(0), qt = T(0);
 #pragma unroll
     for (int m = 0; m < NQ; ++m) {
-      qr = fmaT(L.Di[m], s_q[L.j * NQ + m], qr);
-      qs = fmaT(L.Dj[m], s_q[m * NQ + L.i], qs);
+      qr = fmaT(DL ? s_D[L.i * NQ + m] : L.Di[m], s_q[L.j * NQ + m], qr);
+      qs = fmaT(DL ? s_D[L.j * NQ + m] : L.Dj[m], s_q[m * NQ + L.i], qs);
       qt = fmaT(s_D[k * NQ + m], rq[m], qt);
     }
     const int n = k * NQ2;
@@ -120,8 +122,8 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
     T ar = T(0), as = T(0);
 #pragma unroll
     for (int m = 0; m < NQ; ++m) {
-      ar = fmaT(L.DTi[m], s_Gr[L.j * NQ + m], ar);
-      as = fmaT(L.DTj[m], s_Gs[m * NQ + L.i], as);
+      ar = fmaT(DL ? s_D[m * NQ + L.i] : L.DTi[m], s_Gr[L.j * NQ + m], ar);
+      as = fmaT(DL ? s_D[m * NQ + L.j] : L.DTj[m], s_Gs[m * NQ + L.i], as);
     }
     rArs[k] = ar + as;
   }
@@ -136,8 +138,15 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
   }
 }
 
+// the one-field BK kernel at N = 7 reads D from LDS and fits four waves per
+// SIMD (128 VGPRs): 4096 elements, the 16^3 mesh, fill the chip in one round
+template <int NQ, int NF, bool BK>
+struct AxDL {
+  static constexpr bool value = NQ == 8 && NF == 1 && BK;
+};
+
 template <typename T, int NQ, int NF, bool BK, bool PARTIAL, typename G = T>
-__global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm(const int32_t Nelements, const int32_t offset,
+__global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value, (AxDL<NQ, NF, BK>::value ? 4 : 1)) void k_axhelm(const int32_t Nelements, const int32_t offset,
                                                                   const int32_t* __restrict__ elementList,
                                                                   const G* __restrict__ ggeo,
                                                                   const T* __restrict__ D,
@@ -170,7 +179,8 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm(const int32_t
     T rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) rq[k] = ldnt(&q[base + k * NQ2]);
-    axhelm_column<T, NQ, BK, G>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g, l0, l1);
+    axhelm_column<T, NQ, BK, G, AxDL<NQ, NF, BK>::value>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g,
+                                                         l0, l1);
     if (active) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) stnt(out[k], &Aq[base + k * NQ2]);
