@@ -249,6 +249,10 @@ int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, dou
  * all fields; bf16 geometry: one field), N <= 11.  Uses the handle's lambda. */
 int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters, double lo, double hi,
                     nbx_prec prec);
+/* on (default): the cycle is captured once into a HIP graph (per residual,
+ * output and done-flag address) and replayed; off: launched kernel by kernel.
+ * Both give the same bits. */
+int nbx_bp_mg_use_graph(nbx_bp* bp, int on);
 /* order and lambda_max estimate of a level; returns the number of levels */
 int nbx_bp_mg_info(const nbx_bp* bp, int level, int* N, double* lmax);
 /* z = M r, one V-cycle (device vectors), synchronous */

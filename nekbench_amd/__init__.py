@@ -325,6 +325,9 @@ class BP:
             raise NbxError(f"nbx_bp_mg_info failed ({rc}): {last_error()}")
         return N.value, lam.value
 
+    def mg_use_graph(self, on=True):
+        check(lib().nbx_bp_mg_use_graph(self.h, int(on)), "nbx_bp_mg_use_graph")
+
     def mg_apply(self, r, z):
         check(lib().nbx_bp_mg_apply(self.h, ptr(r), ptr(z)), "nbx_bp_mg_apply")
 
@@ -391,6 +394,7 @@ _SIGS.update({
     "nbx_bp_mg_setup": (_I, [_P, _I, _P, _I, _I, _D, _D, _I]),
     "nbx_bp_mg_info": (_I, [_P, _I, _P, _P]),
     "nbx_bp_mg_apply": (_I, [_P, _P, _P]),
+    "nbx_bp_mg_use_graph": (_I, [_P, _I]),
     "nbx_bp_all_neumann": (_I, [_P]),
     "nbx_net_create": (_P, [C.c_uint64, _P]),
     "nbx_net_free": (None, [_P]),

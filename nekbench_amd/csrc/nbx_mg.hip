@@ -1,4 +1,7 @@
 // p-multigrid preconditioner for the BP PCG (precon NBX_PRECON_MG).
+// (The cycle is enqueued once into a HIP graph and replayed: ~150 small
+// kernels per cycle, most of them on the coarse levels, where launch gaps
+// dominate.)
 //
 // Levels share the mesh: level l discretises the same box at polynomial
 // order N_l (e.g. 7, 3, 1), each with its own BP handle (geometric factors,
@@ -54,6 +57,14 @@ struct nbx_mg {
   nbx_prec prec = NBX_F64;
   nbx::PcgScalars* coarse_template = nullptr;  // device copy of the coarse solve's initial scalars
   std::vector<void*> owned;
+  // the cycle as a HIP graph, captured for one (r, z, skip) and replayed
+  bool use_graph = true;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* g_r = nullptr;
+  void* g_z = nullptr;
+  const int* g_skip = nullptr;
 };
 
 namespace nbx {
@@ -433,10 +444,23 @@ int vcycle(nbx_mg* mg, int l, const void* r, void* x, const int* skip)
 
 namespace nbx {
 
+static void mg_drop_graph(nbx_mg* mg)
+{
+  if (mg->exec) (void)hipGraphExecDestroy(mg->exec);
+  if (mg->graph) (void)hipGraphDestroy(mg->graph);
+  mg->exec = nullptr;
+  mg->graph = nullptr;
+  mg->g_r = nullptr;
+  mg->g_z = nullptr;
+  mg->g_skip = nullptr;
+}
+
 void mg_release(nbx_mg* mg)
 {
   if (!mg) return;
   (void)hipDeviceSynchronize();
+  mg_drop_graph(mg);
+  if (mg->capture) (void)hipStreamDestroy(mg->capture);
   for (auto& L : mg->lv)
     if (L.owned && L.bp) nbx_bp_free(L.bp);
   for (void* p : mg->owned) (void)hipFree(p);
@@ -444,11 +468,58 @@ void mg_release(nbx_mg* mg)
   delete mg;
 }
 
+static int mg_enqueue(nbx_mg* mg, const double* r, double* z, const int* skip)
+{
+  if (mg->prec == NBX_F64) return vcycle<double>(mg, 0, r, z, skip);
+  return vcycle<float>(mg, 0, r, z, skip);
+}
+
+// Capture the cycle for (r, z, skip) on a private stream: every level's
+// handle enqueues on it while capturing, then returns to its own stream.
+static int mg_capture(nbx_mg* mg, nbx_bp* bp, const double* r, double* z, const int* skip)
+{
+  mg_drop_graph(mg);
+  if (!mg->capture) NBX_HIP(hipStreamCreateWithFlags(&mg->capture, hipStreamNonBlocking));
+  std::vector<hipStream_t> saved;
+  for (auto& L : mg->lv) {
+    saved.push_back(L.bp->stream);
+    L.bp->stream = mg->capture;
+  }
+  int rc = NBX_SUCCESS;
+  if (hipStreamBeginCapture(mg->capture, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail(NBX_ERR_HIP, "mg: stream capture could not start");
+  } else {
+    rc = mg_enqueue(mg, r, z, skip);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(mg->capture, &g);
+    if (rc == NBX_SUCCESS && e != hipSuccess) rc = fail(NBX_ERR_HIP, "mg: stream capture failed");
+    if (rc == NBX_SUCCESS && hipGraphInstantiate(&mg->exec, g, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(NBX_ERR_HIP, "mg: graph instantiation failed");
+    mg->graph = g;
+  }
+  for (size_t l = 0; l < mg->lv.size(); ++l) mg->lv[l].bp->stream = saved[l];
+  if (rc) {
+    mg_drop_graph(mg);
+    return rc;
+  }
+  mg->g_r = r;
+  mg->g_z = z;
+  mg->g_skip = skip;
+  (void)bp;
+  return NBX_SUCCESS;
+}
+
 int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip)
 {
-  if (!bp->mg) return fail(NBX_ERR_INVALID, "multigrid preconditioner: not set up (nbx_bp_mg_setup)");
-  if (bp->mg->prec == NBX_F64) return vcycle<double>(bp->mg, 0, r, z, skip);
-  return vcycle<float>(bp->mg, 0, r, z, skip);
+  nbx_mg* mg = bp->mg;
+  if (!mg) return fail(NBX_ERR_INVALID, "multigrid preconditioner: not set up (nbx_bp_mg_setup)");
+  if (!mg->use_graph) return mg_enqueue(mg, r, z, skip);
+  if (!mg->exec || mg->g_r != r || mg->g_z != z || mg->g_skip != skip) {
+    int rc = mg_capture(mg, bp, r, z, skip);
+    if (rc) return rc;
+  }
+  NBX_HIP(hipGraphLaunch(mg->exec, bp->stream));
+  return NBX_SUCCESS;
 }
 
 }  // namespace nbx
@@ -567,6 +638,15 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
     return rc;
   }
   bp->mg = mg;
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_mg_use_graph(nbx_bp* bp, int on)
+{
+  if (!bp || !bp->mg) return fail(NBX_ERR_INVALID, "bp_mg_use_graph: no multigrid set up");
+  (void)hipStreamSynchronize(bp->stream);
+  bp->mg->use_graph = on != 0;
+  if (!on) mg_drop_graph(bp->mg);
   return NBX_SUCCESS;
 }
 

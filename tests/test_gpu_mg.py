@@ -149,3 +149,25 @@ def test_mg_three_fields_match_oracle(nbx, nbo, dev, prec):
     ost, _, _ = mg.pcg(mg.fine.rhs.copy(), xo, 1e-10, 200)
     assert st.iterations == ost.iterations and st.converged
     assert np.array_equal(H(x), xo)
+
+
+def test_mg_graph_replay_equals_direct_launches(nbx, dev):
+    """The cycle replayed from its HIP graph gives the bits of the cycle
+    launched kernel by kernel, over a whole solve and after a change of the
+    residual vector (which re-captures the graph)."""
+    import torch
+    box = nbx.Box.make(7, 4, 3, 3, perturb=0.1, seed=13)
+    bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
+    bp.mg_setup([7, 3, 1], 2, 15, prec=nbx.NBX_F32)
+    n = bp.fieldOffset
+    out = {}
+    for on in (True, False):
+        bp.mg_use_graph(on)
+        for k in range(2):  # a second residual buffer forces a new capture
+            r = bp.device_array("rhs", n).clone()
+            x = torch.zeros_like(r)
+            st = bp.pcg(r, x, 1e-10, 300)
+            out[(on, k)] = (st.iterations, x.cpu())
+    for k in range(2):
+        assert out[(True, k)][0] == out[(False, k)][0]
+        assert torch.equal(out[(True, k)][1], out[(False, k)][1])
