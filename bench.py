@@ -246,9 +246,13 @@ def run_config4(nbx, torch, args):
     for seed in range(9000, 9005):
         bp.make_rhs(seed, b)
         rec = {"seed": seed}
-        for name, h, fb in (("jacobi", bj, None), ("learned", bp, (20, 0.9)), ("learned_nofallback", bp, (10 ** 6, 0.9))):
+        for name, h, fb, mm in (("jacobi", bj, None, False), ("learned", bp, (20, 0.9), False),
+                                ("learned_nofallback", bp, (10 ** 6, 0.9), False),
+                                ("learned_mfma_inference", bp, (20, 0.9), True)):
             if fb:
                 h.set_fallback(*fb)
+            if h is bp:
+                net.set_inference(mm)
             for _ in range(2):
                 r.copy_(b)
                 x.zero_()
@@ -260,7 +264,7 @@ def run_config4(nbx, torch, args):
     def avg(k, f):
         return float(sum(row[k][f] for row in rows) / len(rows))
     summary = {k: {"iterations": avg(k, "iterations"), "time_s": avg(k, "time_s")}
-               for k in ("jacobi", "learned", "learned_nofallback")}
+               for k in ("jacobi", "learned", "learned_nofallback", "learned_mfma_inference")}
     return {"config": "learned element preconditioner, 16^3 elements N=7, 2000 pairs, validation seeds 9000-9004",
             "training": info, "validation": rows, "mean": summary}
 

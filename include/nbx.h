@@ -279,6 +279,10 @@ int nbx_net_set_params(nbx_net* net, const float* host);
 int nbx_net_save(const nbx_net* net, const char* path);        /* header, seed, params, checksum */
 nbx_net* nbx_net_load(const char* path, void* stream);
 int nbx_bp_attach_net(nbx_bp* bp, nbx_net* net);  /* the handle uses the network until either is freed */
+/* inference used by the solve and nbx_net_apply: 0 (default) FP32 on the
+ * vector ALUs, bit-exact with the oracle; 1 bf16 operands on the matrix
+ * cores with FP32 accumulation (training always uses the FP32 forward) */
+int nbx_net_set_inference(nbx_net* net, int mode);
 int nbx_bp_set_fallback(nbx_bp* bp, int stall_window, double stall_ratio);
 int nbx_bp_update_jacobi(nbx_bp* bp);                          /* invDiagA of the handle */
 /* element-local output (weighted: times 1/multiplicity) for r (device) */

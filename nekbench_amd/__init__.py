@@ -411,6 +411,7 @@ _SIGS.update({
     "nbx_net_gradient": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "nbx_net_train": (_I, [_P, _P, _I, _I, C.c_float, C.c_uint64, _P]),
     "nbx_net_train_energy": (_I, [_P, _P, _I, C.c_float, C.c_uint64, _I, _P]),
+    "nbx_net_set_inference": (_I, [_P, _I]),
 })
 
 
@@ -452,6 +453,9 @@ class Net:
     def attach(self, bp: "BP"):
         check(lib().nbx_bp_attach_net(bp.h, self.h), "nbx_bp_attach_net")
         bp._net = self  # the handle uses the weights: keep them alive as long as it
+
+    def set_inference(self, mfma: bool):
+        check(lib().nbx_net_set_inference(self.h, int(bool(mfma))), "nbx_net_set_inference")
 
     def apply(self, bp: "BP", r, zl, weighted=True):
         check(lib().nbx_net_apply(bp.h, ptr(r), ptr(zl), int(weighted)), "nbx_net_apply")

@@ -30,6 +30,7 @@ struct nbx_net {
   int npairs = 0, pair_cap = 0;
   int64_t pairStride = 0;
   std::vector<nbx_bp*> attached;  // handles whose bp->net points here
+  int infer_mfma = 0;              // inference in the solve: 0 FP32 vector ALUs, 1 bf16 matrix cores
   // energy-norm training: three local vectors and 1 / e^T A e per pair
   double* work = nullptr;
   int64_t work_n = 0;
@@ -212,6 +213,9 @@ void net_detach(nbx_net* net, nbx_bp* bp)
 int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallback, const int* skip)
 {
   if (!bp->net || !bp->feat) return fail(NBX_ERR_INVALID, "learned preconditioner: no network attached");
+  if (bp->net->infer_mfma)
+    return net_apply_mma_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, nbx_ogs_inv_degree(bp->ogs), out,
+                                fallback, skip, bp->stream);
   return net_apply_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, nbx_ogs_inv_degree(bp->ogs), out, fallback,
                           skip, bp->stream);
 }
@@ -353,10 +357,19 @@ extern "C" int nbx_bp_set_fallback(nbx_bp* bp, int stall_window, double stall_ra
 extern "C" int nbx_net_apply(nbx_bp* bp, const double* r, double* zl, int weighted)
 {
   if (!bp || !r || !zl || !bp->net) return fail(NBX_ERR_INVALID, "net_apply: bad arguments / no network");
-  int rc = net_apply_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA,
-                            weighted ? nbx_ogs_inv_degree(bp->ogs) : nullptr, zl, nullptr, nullptr, bp->stream);
+  const double* w = weighted ? nbx_ogs_inv_degree(bp->ogs) : nullptr;
+  int rc = bp->net->infer_mfma
+               ? net_apply_mma_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, w, zl, nullptr, nullptr, bp->stream)
+               : net_apply_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, w, zl, nullptr, nullptr, bp->stream);
   if (rc) return rc;
   NBX_HIP(hipStreamSynchronize(bp->stream));
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_net_set_inference(nbx_net* n, int mode)
+{
+  if (!n || (mode != 0 && mode != 1)) return fail(NBX_ERR_INVALID, "net_set_inference: mode 0 or 1");
+  n->infer_mfma = mode;
   return NBX_SUCCESS;
 }
 

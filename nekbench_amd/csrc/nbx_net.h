@@ -14,6 +14,9 @@ constexpr int kNetPartial = 3044;       // floats per workgroup partial
 
 int net_apply_launch(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
                      const double* w, double* zl, const int* fallback, const int* skip, hipStream_t s);
+// the same on the matrix cores (bf16 operands, FP32 accumulation; nbx_net_mma.hip)
+int net_apply_mma_launch(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
+                         const double* w, double* zl, const int* fallback, const int* skip, hipStream_t s);
 size_t net_train_lds_bytes();
 int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64_t pairStride, const float* R,
                      const float* Ev, const double* invNorm, const float* W, const float* feat, const double* invDiag,

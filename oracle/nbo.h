@@ -171,6 +171,8 @@ int nbo_bp_pcg_mg(const nbo_mg* mg, double* r, double* x, double tol, int maxit,
 void nbo_net_apply(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
                    const double* w, double* zl);
 void nbo_net_features(int32_t E, int Np, const double* ggeo, float* feat);
+void nbo_net_apply_mma(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
+                       const double* w, double* zl);
 double nbo_sumsq_inv_f(int64_t n, const float* a);
 void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, int64_t pairStride,
                       const float* R, const float* Ev, const double* invNorm, const float* W, const float* feat,

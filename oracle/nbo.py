@@ -321,6 +321,13 @@ def net_apply(E, W, feat, r, invDiag, w=None):
     return zl
 
 
+def net_apply_mma(E, W, feat, r, invDiag, w=None):
+    zl = np.zeros(E * 512)
+    lib().nbo_net_apply_mma(C.c_int32(E), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat), p(r), p(invDiag),
+                            p(w), p(zl))
+    return zl
+
+
 def net_gradient(samples, chunk, pairStride, R, Ev, invNorm, W, feat, invDiag, G=None):
     """G given: gradient of an external loss whose derivative with respect to
     the unweighted element output invDiag (r + s y) is G (local vector)."""

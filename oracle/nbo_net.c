@@ -263,3 +263,69 @@ double nbo_sumsq_inv_f(int64_t n, const float* a)
   }
   return 1.0 / nbo_tree256(lane);
 }
+
+/* Matrix-core inference (library: k_net_apply_mma, nbx_net_mma.hip): the
+ * same network with the layer inputs (features, hidden activations) and the
+ * weights rounded to bf16 (nearest even), products accumulated in double and
+ * rounded to float (the products are exact in FP32; the matrix cores'
+ * summation order is not specified, so agreement is to FP32 rounding). */
+static float bfr(float x)
+{
+  union { float f; uint32_t u; } a;
+  a.f = x;
+  a.u = (a.u + 0x7FFFu + ((a.u >> 16) & 1u)) & 0xFFFF0000u;
+  return a.f;
+}
+
+void nbo_net_apply_mma(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
+                       const double* w, double* zl)
+{
+#pragma omp parallel for schedule(static)
+  for (int32_t e = 0; e < E; ++e) {
+    const double* re = r + (int64_t)e * P;
+    double s = 0.0;
+    for (int n = 0; n < P; ++n) s = fmax(s, fabs(re[n]));
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    float rh[P];
+    for (int n = 0; n < P; ++n) rh[n] = (float)(re[n] * inv);
+    static float T[P][TAPS];
+#pragma omp threadprivate(T)
+    for (int n = 0; n < P; ++n) {
+      float fv[34];
+      for (int q = 0; q < G; ++q) fv[q] = bfr(feat[((int64_t)e * G + q) * P + n]);
+      for (int tap = 0; tap < TAPS; ++tap) {
+        const int nb = tap_nb(n, tap);
+        fv[7 + tap] = bfr(nb >= 0 ? rh[nb] : 0.f);
+      }
+      float h1[C], h2[C];
+      for (int c = 0; c < C; ++c) {
+        double acc = W[OB1 + c];
+        for (int f = 0; f < 7; ++f) acc += (double)bfr(W[OWG + f * C + c]) * fv[f];
+        for (int tap = 0; tap < TAPS; ++tap) acc += (double)bfr(W[OWR + tap * C + c]) * fv[7 + tap];
+        const float v = (float)acc;
+        h1[c] = bfr(v > 0.f ? v : 0.f);
+      }
+      for (int c = 0; c < C; ++c) {
+        double acc = W[OB2 + c];
+        for (int ci = 0; ci < C; ++ci) acc += (double)bfr(W[OW2 + ci * C + c]) * h1[ci];
+        const float v = (float)acc;
+        h2[c] = bfr(v > 0.f ? v : 0.f);
+      }
+      for (int tap = 0; tap < TAPS; ++tap) {
+        double acc = 0.0;
+        for (int c = 0; c < C; ++c) acc += (double)bfr(W[OW3 + tap * C + c]) * h2[c];
+        T[n][tap] = (float)acc;
+      }
+    }
+    for (int n = 0; n < P; ++n) {
+      float y = W[OB3];
+      for (int tap = 0; tap < TAPS; ++tap) {
+        const int nb = tap_nb(n, tap);
+        if (nb >= 0) y = y + T[nb][tap];
+      }
+      const int64_t id = (int64_t)e * P + n;
+      const double wn = w ? w[id] : 1.0;
+      zl[id] = wn * (invDiag[id] * (re[n] + s * (double)y));
+    }
+  }
+}

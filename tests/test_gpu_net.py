@@ -280,3 +280,30 @@ def test_training_lowers_loss(nbx, dev):
     print("iterations learned", st.iterations, "jacobi", sj.iterations, "fallback", st.fallback_iteration)
     assert st.converged and sj.converged
     assert (x - xj).abs().max().item() <= 1e-6 * xj.abs().max().item()
+
+
+def test_matrix_core_inference_matches_oracle(nbx, nbo, dev):
+    """Inference on the matrix cores: the oracle's bf16 restatement to FP32
+    rounding (the matrix cores' summation order is not specified); the
+    learned PCG with it converges to the FP64 solution."""
+    import torch
+    bp, ob = make(nbx, nbo)
+    W = nbo.bf16_round(random_net(np.random.default_rng(31), scale3=0.3))
+    net = net_with(nbx, nbo, bp, W)
+    net.set_inference(True)
+    n = bp.fieldOffset
+    r = torch.tensor(np.random.default_rng(2).standard_normal(n) * 1e-3, device=dev)
+    zl = torch.zeros_like(r)
+    net.apply(bp, r, zl, True)
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    exp = nbo.net_apply_mma(ob.E, W, feat, H(r), ob.invdiag(), ob.invd)
+    jac = ob.invd * ob.invdiag() * H(r)
+    # a 1-ulp FP32 difference can flip a bf16 rounding of an activation: 5e-3 of the correction
+    assert np.abs(H(zl) - exp).max() <= 5e-3 * np.abs(exp - jac).max()
+    rr = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(rr)
+    st = bp.pcg(rr, x, 1e-10, 1000)
+    x64 = np.zeros(n)
+    ob.pcg(ob.rhs.copy(), x64, 1e-12, 3000)
+    assert st.converged
+    assert np.abs(H(x) - x64).max() <= 1e-8 * np.abs(x64).max()

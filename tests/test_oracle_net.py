@@ -229,3 +229,22 @@ def test_learned_pcg_stall_falls_back_to_jacobi(nbo):
     k = st.fallback_iter
     assert not (hr[k] <= 0.999 * hr[k - 5])
     assert all(hr[i] <= 0.999 * hr[i - 5] for i in range(6, k))
+
+
+def test_matrix_core_forward_tracks_fp32(nbo, data):
+    """bf16-operand inference oracle: within bf16 rounding of the FP32
+    forward (the correction s*y changes by about 1e-2 of itself), exactly
+    Jacobi with a zero output layer."""
+    W = random_net(np.random.default_rng(21))
+    E = data["E"]
+    r = data["R"][:E * NP_].astype(np.float64)
+    w = np.full(E * NP_, 0.5)
+    z32 = nbo.net_apply(E, W, data["feat"], r, data["invDiag"], w)
+    zmm = nbo.net_apply_mma(E, W, data["feat"], r, data["invDiag"], w)
+    jac = 0.5 * (data["invDiag"] * r)
+    corr = np.abs(z32 - jac).max()
+    assert corr > 0
+    assert 1e-6 * corr < np.abs(zmm - z32).max() <= 5e-2 * corr
+    W0 = W.copy()
+    W0[OW3:NPAR] = 0.0
+    assert np.array_equal(nbo.net_apply_mma(E, W0, data["feat"], r, data["invDiag"], w), jac)
