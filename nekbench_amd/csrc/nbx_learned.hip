@@ -99,20 +99,41 @@ void net_release(nbx_net* n)
   delete n;
 }
 
-// sum of squares of each ggeo factor over the mesh (one block per factor)
+// sum of squares of each ggeo factor over the mesh, two phases: workgroup b
+// takes elements [16 b, 16 b + 16), lane t runs an fma chain per factor over
+// those elements and nodes t, t + 256, ...; the 256-lane tree gives one
+// partial per workgroup and factor; one workgroup combines them in order.
+constexpr int kFeatChunk = 16;
+
 __global__ __launch_bounds__(256) void k_factor_sumsq(const int32_t E, const int Np, const double* __restrict__ ggeo,
-                                                      double* __restrict__ out)
+                                                      double* __restrict__ partials)
 {
-  __shared__ double s4[4];
-  const int q = blockIdx.x;
-  double v[1] = {0.0};
-  for (int64_t e = 0; e < E; ++e)
+  __shared__ double s4[7 * 4];
+  double v[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) v[q] = 0.0;
+  const int64_t e0 = (int64_t)blockIdx.x * kFeatChunk;
+  const int64_t e1 = e0 + kFeatChunk < E ? e0 + kFeatChunk : E;
+  for (int64_t e = e0; e < e1; ++e)
     for (int n = threadIdx.x; n < Np; n += 256) {
-      const double g = ggeo[(e * 7 + q) * Np + n];
-      v[0] = __builtin_fma(g, g, v[0]);
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const double g = ggeo[(e * 7 + q) * Np + n];
+        v[q] = __builtin_fma(g, g, v[q]);
+      }
     }
-  nbx::block_sum256<1>(v, s4);
-  if (threadIdx.x == 0) out[q] = v[0];
+  nbx::block_sum256<7>(v, s4);
+  nbx::store_partials<7>(v, partials);
+}
+
+__global__ __launch_bounds__(256) void k_factor_sumsq_finish(const unsigned int nb, const double* __restrict__ partials,
+                                                             double* __restrict__ out)
+{
+  __shared__ double s4[7 * 4];
+  double v[7];
+  nbx::combine_partials<7>(v, partials, nb, s4);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 7; ++q) out[q] = v[q];
 }
 
 __global__ __launch_bounds__(256) void k_to_float_diff(const int64_t n, const double* __restrict__ a,
@@ -322,9 +343,13 @@ extern "C" int nbx_bp_attach_net(nbx_bp* bp, nbx_net* net)
   if (bp->N != 7) return fail(NBX_ERR_UNSUPPORTED, "learned preconditioner is defined on N=7 (8^3 nodes)");
   if (bp->Nf != 1) return fail(NBX_ERR_UNSUPPORTED, "learned preconditioner: one field");
   if (!bp->feat) NBX_HIP(hipMalloc(&bp->feat, sizeof(float) * (size_t)bp->E * 7 * bp->Np));
+  const unsigned int nb = (unsigned int)((bp->E + kFeatChunk - 1) / kFeatChunk);
   double* d_s = nullptr;
-  NBX_HIP(hipMalloc(&d_s, sizeof(double) * 14));
-  hipLaunchKernelGGL(k_factor_sumsq, dim3(7), dim3(256), 0, bp->stream, bp->E, bp->Np, bp->ggeo, d_s);
+  NBX_HIP(hipMalloc(&d_s, sizeof(double) * (14 + 7 * (size_t)nb)));
+  double* d_part = d_s + 14;
+  hipLaunchKernelGGL(k_factor_sumsq, dim3(nb), dim3(256), 0, bp->stream, bp->E, bp->Np, bp->ggeo, d_part);
+  hipLaunchKernelGGL(k_factor_sumsq_finish, dim3(1), dim3(256), 0, bp->stream, nb, d_part, d_s);
+  NBX_HIP_LAUNCH();
   double h[7];
   NBX_HIP(hipMemcpyAsync(h, d_s, sizeof(h), hipMemcpyDeviceToHost, bp->stream));
   NBX_HIP(hipStreamSynchronize(bp->stream));

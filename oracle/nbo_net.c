@@ -229,25 +229,36 @@ void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, i
 }
 
 /* Features of a mesh: each ggeo factor divided by its RMS over the mesh.  The
- * sum of squares follows the library's order: lane t of 256 runs an fma chain
- * over elements, then nodes t, t+256, ... of the element; the 256-lane tree. */
+ * sums of squares follow the library's two-phase order: block b takes
+ * elements [16 b, 16 b + 16), lane t of 256 runs an fma chain per factor over
+ * those elements and nodes t, t+256, ...; the 256-lane tree per block; the
+ * block partials combined by the partials tree. */
 void nbo_net_features(int32_t E, int Np, const double* ggeo, float* feat)
 {
+  const int64_t nb = ((int64_t)E + 15) / 16;
+  double* parts = (double*)malloc(sizeof(double) * G * (size_t)nb);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) {
+    double lane[G][256];
+    for (int t = 0; t < 256; ++t) {
+      double acc[G] = {0, 0, 0, 0, 0, 0, 0};
+      const int64_t e1 = (b + 1) * 16 < E ? (b + 1) * 16 : E;
+      for (int64_t e = b * 16; e < e1; ++e)
+        for (int n = t; n < Np; n += 256)
+          for (int q = 0; q < G; ++q) {
+            const double g = ggeo[(e * G + q) * Np + n];
+            acc[q] = fma(g, g, acc[q]);
+          }
+      for (int q = 0; q < G; ++q) lane[q][t] = acc[q];
+    }
+    for (int q = 0; q < G; ++q) parts[q * nb + b] = nbo_tree256(lane[q]);
+  }
   double inv[G];
   for (int q = 0; q < G; ++q) {
-    double lane[256];
-    for (int t = 0; t < 256; ++t) {
-      double acc = 0.0;
-      for (int64_t e = 0; e < E; ++e)
-        for (int n = t; n < Np; n += 256) {
-          const double g = ggeo[(e * G + q) * Np + n];
-          acc = fma(g, g, acc);
-        }
-      lane[t] = acc;
-    }
-    const double rms = sqrt(nbo_tree256(lane) / ((double)E * Np));
+    const double rms = sqrt(nbo_tree_sum_partials(nb, parts + q * nb) / ((double)E * Np));
     inv[q] = rms > 0 ? 1.0 / rms : 0.0;
   }
+  free(parts);
   for (int64_t i = 0; i < (int64_t)E * G * Np; ++i) feat[i] = (float)(ggeo[i] * inv[(i / Np) % G]);
 }
 
