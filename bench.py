@@ -318,6 +318,31 @@ def run_config5(nbx, torch, args):
             "elements": 64 * 64 * 32, "learned_training": info, "solvers": out}
 
 
+def cpu_reference_axhelm(bj, E, n, g0, D, log):
+    """The reference's own CPU kernel axhelm_bk_v0 (axhelm/kernel/CPU/axhelm.c,
+    compiled by oracle/Makefile into oracle/_ref, OpenMP) on the same 16^3
+    mesh, with axhelm/main.cpp's protocol: one warm-up call, then the mean of
+    100 calls; host cores as OMP_NUM_THREADS gives them."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import nbo
+    if not nbo.ref_available(7):
+        return None
+    ggeo = g0.cpu().numpy()
+    Dh = D.cpu().numpy()
+    q = np.random.default_rng(1).uniform(-1, 1, n)
+    nbo.ref_axhelm(7, "axhelm_bk_v0", E, n, ggeo, Dh, None, q)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        nbo.ref_axhelm(7, "axhelm_bk_v0", E, n, ggeo, Dh, None, q)
+    t = (time.perf_counter() - t0) / 100
+    res = {"kind": "reference", "kernel": "axhelm_bk_v0 of axhelm/kernel/CPU/axhelm.c (g++ -O2 -fopenmp)",
+           "us_per_apply": round(t * 1e6, 1), "GB_s": round(E * 512 * 72 / t / 1e9, 1),
+           "gdofs": round(E * 343 / t / 1e9, 3), "cores": int(nbo.lib().nbo_max_threads())}
+    log(f"[bench] reference CPU axhelm: {res}")
+    return res
+
+
 def config1_measurements(nbx, torch, args, log):
     """Config 1 mesh (16^3 elements, N=7, perturbed, seed 1): FP64 BK axhelm
     with 5 rotating buffer sets of 151 MB each (755 MB, beyond the 256 MB
@@ -349,6 +374,8 @@ def config1_measurements(nbx, torch, args, log):
     out = {"axhelm_fp64_us": round(t_ax * 1e6, 2), "axhelm_GB_s": round(bytes_ / t_ax / 1e9, 1),
            "axhelm_frac_of_8TBs": round(bytes_ / t_ax / 8e12, 4), "axhelm_gdofs": round(E * 343 / t_ax / 1e9, 2),
            "buffer_sets": len(sets), "bytes_per_apply": bytes_}
+    if not args.no_cpu:
+        out["cpu_reference_axhelm"] = cpu_reference_axhelm(bj, E, n, g0, D, log)
     del sets
     log(f"[bench] config 1 axhelm: {out}")
     b = bj.device_array("rhs", n).clone()
