@@ -246,7 +246,8 @@ int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, dou
  * bf16, the N = 7 level's contractions on the matrix cores for MFMA; the
  * cycle reads the FP64 residual and returns an FP64 correction; the coarse
  * PCG is FP64).  One or three fields (one Jacobi diagonal and lambda_max for
- * all fields; bf16 geometry: one field), N <= 11.  Uses the handle's lambda. */
+ * all fields; bf16 geometry: one field), N <= 11.  Uses the handle's lambda.
+ * Not for the singular all-Neumann problem (nbx_bp_all_neumann). */
 int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int degree, int coarse_iters, double lo, double hi,
                     nbx_prec prec);
 /* on (default): the cycle is captured once into a HIP graph (per residual,

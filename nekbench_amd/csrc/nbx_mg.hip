@@ -532,6 +532,8 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
     return fail(NBX_ERR_INVALID, "bp_mg_setup: bad arguments");
   if (prec != NBX_F64 && prec != NBX_F32 && prec != NBX_BF16 && prec != NBX_BF16_MFMA)
     return fail(NBX_ERR_UNSUPPORTED, "bp_mg_setup: precision %d (F64, F32, BF16 or BF16_MFMA)", (int)prec);
+  if (bp->allNeumann)  // singular: the coarse problem has the constants in its null space
+    return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: all-Neumann problem (no mask, lambda1 = 0)");
   if ((prec == NBX_BF16 || prec == NBX_BF16_MFMA) && bp->Nf != 1)
     return fail(NBX_ERR_UNSUPPORTED, "multigrid preconditioner: bf16 geometry with one field only");
   if (orders[0] != bp->N) return fail(NBX_ERR_INVALID, "bp_mg_setup: orders[0] must be the handle's N");

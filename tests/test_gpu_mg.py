@@ -122,6 +122,9 @@ def test_mg_argument_checks(nbx, dev):
             bp.mg_setup(orders)
     with pytest.raises(nbx.NbxError):
         bp.mg_setup([5, 2, 1], prec=nbx.NBX_F16)
+    bn = nbx.BP(nbx.Box.make(5, 3, 3, 3, periodic=1), bpmode=True, dirichlet=False, precon=nbx.PRECON_MG)
+    with pytest.raises(nbx.NbxError, match="all-Neumann"):
+        bn.mg_setup([5, 2, 1])
     b3 = nbx.BP(nbx.Box.make(5, 2, 2, 2), Nfields=3, precon=nbx.PRECON_MG)
     with pytest.raises(nbx.NbxError):
         b3.mg_setup([5, 2, 1], prec=nbx.NBX_BF16)  # bf16 geometry: one field
