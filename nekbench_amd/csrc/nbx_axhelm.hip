@@ -45,6 +45,19 @@ __device__ __forceinline__ float ldg<float, uint16_t>(const uint16_t* p)
 {
   return __uint_as_float((uint32_t)ldnt(p) << 16);
 }
+// NT = false: an ordinary (cached) load, for data that other lanes of the
+// workgroup read again right away
+template <typename T, typename G, bool NT>
+__device__ __forceinline__ T ldgc(const G* p)
+{
+  if constexpr (NT) {
+    return ldg<T, G>(p);
+  } else if constexpr (sizeof(G) == 2) {
+    return __uint_as_float((uint32_t)(*p) << 16);
+  } else {
+    return (T)(*p);
+  }
+}
 
 // Elements per workgroup: one element (Nq*Nq lanes) from Nq = 8 up; small
 // orders pack several elements (threadIdx.y = slot) so that a workgroup
@@ -78,7 +91,9 @@ struct AxLane {
 // g, l0, l1 point at the lane's node (k = 0) of ggeo, lambda0, lambda1.
 // DL: read the lane's D vectors from LDS instead of registers (frees 4*NQ
 // registers; used where that buys a fourth wave per SIMD)
-template <typename T, int NQ, bool BK, typename G = T, bool DL = false>
+// NTG: nontemporal loads of the geometric factors and lambda (false when
+// other thread groups of the workgroup read the same values)
+template <typename T, int NQ, bool BK, typename G = T, bool DL = false, bool NTG = true>
 __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&rq)[NQ], T (&out)[NQ],
                                               const T* __restrict__ s_D, T* s_q, T* s_Gr, T* s_Gs,
                                               const G* __restrict__ g, const G* __restrict__ l0,
@@ -102,14 +117,14 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
       qt = fmaT(s_D[k * NQ + m], rq[m], qt);
     }
     const int n = k * NQ2;
-    const T G00 = ldg<T, G>(&g[kG00 * NP + n]), G01 = ldg<T, G>(&g[kG01 * NP + n]);
-    const T G02 = ldg<T, G>(&g[kG02 * NP + n]), G11 = ldg<T, G>(&g[kG11 * NP + n]);
-    const T G12 = ldg<T, G>(&g[kG12 * NP + n]), G22 = ldg<T, G>(&g[kG22 * NP + n]);
+    const T G00 = ldgc<T, G, NTG>(&g[kG00 * NP + n]), G01 = ldgc<T, G, NTG>(&g[kG01 * NP + n]);
+    const T G02 = ldgc<T, G, NTG>(&g[kG02 * NP + n]), G11 = ldgc<T, G, NTG>(&g[kG11 * NP + n]);
+    const T G12 = ldgc<T, G, NTG>(&g[kG12 * NP + n]), G22 = ldgc<T, G, NTG>(&g[kG22 * NP + n]);
     T Gr = fmaT(G02, qt, fmaT(G01, qs, G00 * qr));
     T Gs = fmaT(G12, qt, fmaT(G11, qs, G01 * qr));
     T Gt = fmaT(G22, qt, fmaT(G12, qs, G02 * qr));
     if (!BK) {
-      const T lam0 = ldg<T, G>(&l0[n]);
+      const T lam0 = ldgc<T, G, NTG>(&l0[n]);
       Gr = lam0 * Gr;
       Gs = lam0 * Gs;
       Gt = lam0 * Gt;
@@ -132,7 +147,7 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
     T res = rArs[k] + rAt[k];
     if (!BK) {
       const int n = k * NQ2;
-      res = fmaT(ldg<T, G>(&g[kGWJ * NP + n]) * ldg<T, G>(&l1[n]), rq[k], res);
+      res = fmaT(ldgc<T, G, NTG>(&g[kGWJ * NP + n]) * ldgc<T, G, NTG>(&l1[n]), rq[k], res);
     }
     out[k] = res;
   }
@@ -145,8 +160,34 @@ struct AxDL {
   static constexpr bool value = NQ == 8 && NF == 1 && BK;
 };
 
+// Thread groups of NQ*NQ lanes per workgroup: Epb elements at small orders;
+// for the three-field Helmholtz (non-BK) operator with 8 <= Nq <= 12, one
+// group per field of one element, reading the shared geometric factors and
+// lambda with cached loads; each field's arithmetic is the one-field
+// column's.  Measured on 32^3 N = 7: 0.805 -> 0.695 ms; the BK form is
+// faster with the field loop and keeps it.
+template <int NQ, int NF, bool BK>
+struct AxGroups {
+  static constexpr bool fields = NF == 3 && !BK && Epb<NQ>::value == 1 && NQ <= 12;
+  static constexpr int value = fields ? NF : Epb<NQ>::value;
+};
+
+// The FP64 field-group kernels read D from LDS as well (4 Nq fewer doubles
+// in registers) and keep three waves per SIMD up to Nq = 10 (168 VGPRs; left
+// alone they take 256 and run two); the one-field BK kernel at N = 7 keeps
+// four (128 VGPRs).
+template <typename T, int NQ, int NF, bool BK>
+struct AxUseDL {
+  static constexpr bool value = AxDL<NQ, NF, BK>::value || (AxGroups<NQ, NF, BK>::fields && sizeof(T) == 8);
+};
+template <typename T, int NQ, int NF, bool BK>
+struct AxMinWaves {
+  static constexpr int value =
+      AxDL<NQ, NF, BK>::value ? 4 : (AxGroups<NQ, NF, BK>::fields && sizeof(T) == 8 && NQ <= 10 ? 3 : 1);
+};
+
 template <typename T, int NQ, int NF, bool BK, bool PARTIAL, typename G = T>
-__global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value, (AxDL<NQ, NF, BK>::value ? 4 : 1)) void k_axhelm(const int32_t Nelements, const int32_t offset,
+__global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWaves<T, NQ, NF, BK>::value)) void k_axhelm(const int32_t Nelements, const int32_t offset,
                                                                   const int32_t* __restrict__ elementList,
                                                                   const G* __restrict__ ggeo,
                                                                   const T* __restrict__ D,
@@ -157,10 +198,12 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value, (AxDL<NQ, NF, BK>::value ? 
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int EPB = Epb<NQ>::value;
+  constexpr bool FG = AxGroups<NQ, NF, BK>::fields;
+  constexpr int GR = AxGroups<NQ, NF, BK>::value;
   if (skip && *skip) return;
-  __shared__ T s_D[NQ2], s_q[EPB][NQ2], s_Gr[EPB][NQ2], s_Gs[EPB][NQ2];
-  const int slot = EPB > 1 ? (int)threadIdx.y : 0;
-  const int32_t idx = (int32_t)blockIdx.x * EPB + slot;
+  __shared__ T s_D[NQ2], s_q[GR][NQ2], s_Gr[GR][NQ2], s_Gs[GR][NQ2];
+  const int slot = GR > 1 ? (int)threadIdx.y : 0;  // element slot, or field when FG
+  const int32_t idx = FG ? (int32_t)blockIdx.x : (int32_t)blockIdx.x * EPB + slot;
   const bool active = EPB == 1 || idx < Nelements;
   // an idle slot works on the last element without storing: every lane
   // reaches the workgroup barriers
@@ -173,18 +216,23 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value, (AxDL<NQ, NF, BK>::value ? 
   const G* g = ggeo + e * (int64_t)(kNggeo * NP) + L.t;
   const G* l0 = lambda + e * NP + L.t;
   const G* l1 = lambda + e * NP + offset + L.t;
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
+  auto field = [&](int f) {
     const int64_t base = e * NP + (int64_t)f * offset + L.t;
     T rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) rq[k] = ldnt(&q[base + k * NQ2]);
-    axhelm_column<T, NQ, BK, G, AxDL<NQ, NF, BK>::value>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g,
-                                                         l0, l1);
+    axhelm_column<T, NQ, BK, G, AxUseDL<T, NQ, NF, BK>::value, !FG>(L, rq, out, s_D, s_q[slot], s_Gr[slot],
+                                                                     s_Gs[slot], g, l0, l1);
     if (active) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) stnt(out[k], &Aq[base + k * NQ2]);
     }
+  };
+  if constexpr (FG) {
+    field(slot);
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) field(f);
   }
 }
 
@@ -193,11 +241,13 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value, (AxDL<NQ, NF, BK>::value ? 
 // Because p is continuous and zero on the mask, sum_local p (A_L p) equals
 // <p, gs(A_L p)>_w exactly in real arithmetic (the weights 1/multiplicity
 // undo the gather-scatter); this value is what BPAxOperator's dot returns.
-// Lane (i,j) accumulates fma(p, Ap) over fields then k; the NQ*NQ lanes are
-// summed by a halving tree over the next power of two (zero padded); the
-// element partials go to partials[e].
+// Lane (i,j) accumulates fma(p, Ap) over fields then k -- with one thread
+// group per field (three fields, non-BK, 8 <= Nq <= 12) each field's chain runs over k and
+// the lane takes (field0 + field1) + field2; the NQ*NQ lanes are summed by a
+// halving tree over the next power of two (zero padded); the element
+// partials go to partials[e].
 template <int NQ, int NF, bool BK, typename T, typename G>
-__global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
+__global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWaves<T, NQ, NF, BK>::value)) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
                                                                       const G* __restrict__ ggeo,
                                                                       const T* __restrict__ D,
                                                                       const G* __restrict__ lambda,
@@ -210,13 +260,15 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm_pcg(const int
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int EPB = Epb<NQ>::value;
+  constexpr bool FG = AxGroups<NQ, NF, BK>::fields;
+  constexpr int GR = AxGroups<NQ, NF, BK>::value;
   constexpr int P2 = NQ2 <= 1 ? 1 : (NQ2 <= 2 ? 2 : (NQ2 <= 4 ? 4 : (NQ2 <= 8 ? 8 : (NQ2 <= 16 ? 16 : (NQ2 <= 32 ? 32 : (NQ2 <= 64 ? 64 : (NQ2 <= 128 ? 128 : 256)))))));
   if (S->done) return;
   const double beta = S->beta;
-  __shared__ T s_D[NQ2], s_q[EPB][NQ2], s_Gr[EPB][NQ2], s_Gs[EPB][NQ2];
-  __shared__ double s_red[EPB][P2];
-  const int slot = EPB > 1 ? (int)threadIdx.y : 0;
-  const int32_t idx = (int32_t)blockIdx.x * EPB + slot;
+  __shared__ T s_D[NQ2], s_q[GR][NQ2], s_Gr[GR][NQ2], s_Gs[GR][NQ2];
+  __shared__ double s_red[GR][P2];
+  const int slot = GR > 1 ? (int)threadIdx.y : 0;  // element slot, or field when FG
+  const int32_t idx = FG ? (int32_t)blockIdx.x : (int32_t)blockIdx.x * EPB + slot;
   const bool active = EPB == 1 || idx < Nelements;
   // idle slots compute on the last element and store nothing
   const int64_t e = EPB == 1 ? (int64_t)blockIdx.x : (active ? idx : Nelements - 1);
@@ -228,8 +280,7 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm_pcg(const int
   const G* l0 = lambda + e * NP + L.t;
   const G* l1 = lambda + e * NP + offset + L.t;
   double acc = 0.0;
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
+  auto field = [&](int f) {
     const int64_t base = e * NP + (int64_t)f * offset + L.t;
     double pk[NQ];
     T rq[NQ], out[NQ];
@@ -243,13 +294,25 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm_pcg(const int
 #pragma unroll
       for (int k = 0; k < NQ; ++k) p[base + k * NQ2] = pk[k];
     }
-    axhelm_column<T, NQ, BK, G>(L, rq, out, s_D, s_q[slot], s_Gr[slot], s_Gs[slot], g, l0, l1);
+    axhelm_column<T, NQ, BK, G, AxUseDL<T, NQ, NF, BK>::value, !FG>(L, rq, out, s_D, s_q[slot], s_Gr[slot],
+                                                                     s_Gs[slot], g, l0, l1);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const double o = (double)out[k];
       if (active) stnt(o, &Ap[base + k * NQ2]);
       acc = __builtin_fma(pk[k], o, acc);
     }
+  };
+  if constexpr (FG) {
+    field(slot);
+    double* fa = s_red[slot];
+    fa[L.t] = acc;
+    __syncthreads();
+    acc = (s_red[0][L.t] + s_red[1][L.t]) + s_red[2][L.t];
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) field(f);
   }
   double sum;
   if (NQ2 == 64) {
@@ -265,7 +328,7 @@ __global__ __launch_bounds__(NQ* NQ* Epb<NQ>::value) void k_axhelm_pcg(const int
     __syncthreads();
     sum = red[0];
   }
-  if (L.t == 0 && active) partials[e] = sum;
+  if (L.t == 0 && active && (!FG || slot == 0)) partials[e] = sum;
 }
 
 // Local Jacobi diagonal, nekBone/kernel/updateJacobi.okl:27-108, same
@@ -315,7 +378,8 @@ static int launch_axhelm_t(int32_t Nelements, int32_t offset, const int32_t* ele
                            const int* skip, hipStream_t s)
 {
   constexpr int EPB = Epb<NQ>::value;
-  hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL, G>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, EPB), 0,
+  constexpr int GR = AxGroups<NQ, NF, BK>::value;
+  hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL, G>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, GR), 0,
                      s, Nelements, offset, elementList, (const G*)ggeo, (const T*)D, (const G*)lambda,
                      (const T*)q, (T*)Aq, skip);
   NBX_HIP_LAUNCH();
@@ -419,11 +483,11 @@ static int axhelm_pcg_launch_t(int N, int Nf, int bk, int32_t Nelements, int32_t
                          0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
                          partials, S);                                                                       \
     else if (bk)                                                                                             \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), Epb<NV + 1>::value), 0, \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), AxGroups<NV + 1, 3, true>::value), 0, \
                          s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,       \
                          partials, S);                                                                       \
     else                                                                                                     \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), Epb<NV + 1>::value),   \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), AxGroups<NV + 1, 3, false>::value),   \
                          0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
                          partials, S);                                                                       \
     break;

@@ -73,9 +73,12 @@ double nbo_bp_wdot(const nbo_bp* bp, const double* x, const double* y)
 }
 
 /* Element shares of pAp = sum_local p (A_L p) in the library's fused order:
- * lane (i,j) runs an fma chain over fields then k; the Nq*Nq lanes are summed
- * by a halving tree over the next power of two (zero padded); the element
- * shares are combined like tile partials. */
+ * lane (i,j) runs an fma chain over fields then k -- for three fields of the
+ * non-BK operator with 8 <= Nq <= 12 (one thread group per field) a chain over
+ * k per field and then
+ * (field0 + field1) + field2; the Nq*Nq lanes are summed by a halving tree
+ * over the next power of two (zero padded); the element shares are combined
+ * like tile partials. */
 static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
 {
   const int Nq = bp->N + 1, Nq2 = Nq * Nq, Np = Nq2 * Nq;
@@ -86,14 +89,18 @@ static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
   for (int32_t e = 0; e < bp->E; ++e) {
     double lanes[256];
     for (int t = 0; t < P2; ++t) lanes[t] = 0.0;
+    const int field_groups = bp->Nf == 3 && !bp->bk && Nq2 >= 64 && Nq <= 12;
     for (int t = 0; t < Nq2; ++t) {
-      double acc = 0.0;
+      double acc = 0.0, accf[3] = {0.0, 0.0, 0.0};
       for (int f = 0; f < bp->Nf; ++f)
         for (int k = 0; k < Nq; ++k) {
           const int64_t n = (int64_t)e * Np + (int64_t)k * Nq2 + t + (int64_t)f * bp->offset;
-          acc = fma(p[n], Ap[n], acc);
+          if (field_groups)
+            accf[f] = fma(p[n], Ap[n], accf[f]);
+          else
+            acc = fma(p[n], Ap[n], acc);
         }
-      lanes[t] = acc;
+      lanes[t] = field_groups ? (accf[0] + accf[1]) + accf[2] : acc;
     }
     for (int h = P2 / 2; h >= 1; h >>= 1)
       for (int t = 0; t < h; ++t) lanes[t] = lanes[t] + lanes[t + h];
