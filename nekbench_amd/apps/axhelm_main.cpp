@@ -123,8 +123,8 @@ int main(int argc, char** argv)
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
   std::printf("word size: %zu bytes\n", sizeof(double));
-  std::printf("active mode: HIP (%s, %s), thread model argument '%s', arch argument '%s'\n", prop.name,
-              prop.gcnArchName, argv[4], argv[5]);
+  std::printf("active mode: HIP (%s, %s), thread model argument '%s', arch argument '%s'\n",
+              prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName, argv[4], argv[5]);
   if (BKmode) std::printf("BK mode enabled\n");
 
   std::string kernelName = BKmode ? "axhelm_bk" : "axhelm";

@@ -189,7 +189,7 @@ int main(int argc, char** argv)
 
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
-  std::printf("active mode: HIP (%s, %s)\n", prop.name, prop.gcnArchName);
+  std::printf("active mode: HIP (%s, %s)\n", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName);
   if (s.bpmode) std::printf("BP mode enabled\n");
   std::printf("overlap disabled\n");
 
