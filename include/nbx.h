@@ -155,6 +155,23 @@ int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const
                                const double* x, const double* y, double* result_host, void* stream);
 int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream);
 
+/* Bandwidth probes of the bw benchmark (bw/main.cpp:59-140, bw/kernel/bwKernels.okl)
+ * plus the copy and read streams: the measured HBM and LDS ceilings.  Device
+ * arrays, 16-byte aligned (hipMalloc'd); asynchronous on `stream`.
+ *   nbx_triad:     c = a + alpha b                (bwKernels.okl:1-11 "triad")
+ *   nbx_copy:      c = a
+ *   nbx_read_sum:  work[0] = sum of a[0..N) in a fixed order; work is a device
+ *                  buffer of NBX_READ_SUM_WORK doubles (block partials after
+ *                  work[0]); calls sharing a buffer must not overlap
+ *   nbx_lds_probe: per block of 128 values, 128 rounds of s[(i+j)%128] += s[0]
+ *                  through LDS (bwKernels.okl:13-42 "smem"); a[b*128+i] becomes
+ *                  a[b*128+i] + 128 sequential additions of a[b*128] */
+int nbx_triad(nbx_dlong N, double alpha, const double* a, const double* b, double* c, void* stream);
+int nbx_copy(nbx_dlong N, const double* a, double* c, void* stream);
+#define NBX_READ_SUM_WORK 2049
+int nbx_read_sum(nbx_dlong N, const double* a, double* work, void* stream);
+int nbx_lds_probe(nbx_dlong Nblocks, double* a, void* stream);
+
 /* ======================= 3. BP driver ====================================
  * Replaces nekBone's setup()/solveSetup() (nekBone/setup.cpp:36-367),
  * BPPCG (nekBone/solve.cpp:44-113) and BPAxOperator (solve.cpp:181-235) on

@@ -88,6 +88,10 @@ _SIGS = {
     "nbx_box_mesh": (_I, [C.POINTER(Box), _I, _P, _P, _P, _P, _P, _P]),
     "nbx_weighted_inner_product": (_I, [C.c_int32, C.c_int32, _I, _P, _P, _P, _P, _P]),
     "nbx_scaled_add": (_I, [C.c_int32, _D, _P, _D, _P, _P]),
+    "nbx_triad": (_I, [C.c_int32, _D, _P, _P, _P, _P]),
+    "nbx_copy": (_I, [C.c_int32, _P, _P, _P]),
+    "nbx_read_sum": (_I, [C.c_int32, _P, _P, _P]),
+    "nbx_lds_probe": (_I, [C.c_int32, _P, _P]),
     "nbx_bp_setup": (_P, [C.POINTER(BPOptions), _P]),
     "nbx_bp_free": (None, [_P]),
     "nbx_bp_sizes": (_I, [_P, _P, _P, _P, _P]),

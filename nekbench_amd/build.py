@@ -93,8 +93,10 @@ def build(verbose: bool = True) -> Path:
 
 
 # command-line drivers with the reference's interfaces: bin/axhelm
-# (axhelm/main.cpp) and bin/nekBone (nekBone/main.cpp); host C++ over libnbx.so
-APP_SOURCES = {"axhelm": "axhelm_main.cpp", "nekBone": "nekbone_main.cpp"}
+# (axhelm/main.cpp), bin/nekBone (nekBone/main.cpp), bin/bw (bw/main.cpp) and
+# bin/dot (dot/main.cpp); host C++ over libnbx.so
+APP_SOURCES = {"axhelm": "axhelm_main.cpp", "nekBone": "nekbone_main.cpp", "bw": "bw_main.cpp",
+               "dot": "dot_main.cpp"}
 
 
 def build_apps(verbose: bool = True) -> None:

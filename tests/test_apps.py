@@ -34,7 +34,7 @@ def write(tmp_path, text, name="case.ini"):
 
 
 def test_binaries_built():
-    for name in ("axhelm", "nekBone"):
+    for name in ("axhelm", "nekBone", "bw", "dot"):
         assert (BIN / name).is_file(), f"{name} not built (run __graft_entry__.build())"
 
 
@@ -45,6 +45,10 @@ def test_usage_messages():
     assert r.returncode == 1 and r.stdout.strip() == "usage: ./nekBone setupfile"
     r = run(BIN / "nekBone", ROOT / "no-such-file.ini")
     assert r.returncode == 1 and "cannot read" in r.stderr
+    r = run(BIN / "bw")
+    assert r.returncode == 1 and r.stdout.strip() == "Usage: ./bw SERIAL|CUDA|OPENCL"
+    r = run(BIN / "dot", 7, 10, "HIP")
+    assert r.returncode == 1 and r.stdout.startswith("Usage: ./dot N numElements")
 
 
 def test_axhelm_rejects_bad_sizes():
