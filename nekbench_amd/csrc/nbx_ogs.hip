@@ -13,6 +13,7 @@
 // start from the first member (gatherScatterMany.okl:58-86, 148-176, 300-330,
 // 455-485), so every result is bit-identical to the reference kernel's.
 #include "nbx_internal.h"
+#include <hipcub/hipcub.hpp>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <string>
 #include <vector>
 
 struct nbx_ogs {
@@ -307,6 +309,304 @@ static void ogs_release(nbx_ogs* o)
   delete o;
 }
 
+namespace nbx {
+namespace {
+
+// ---- device construction of the local CSR (ogsSetup.cpp:137-212, 325-352) ----
+// 1. key = |id| (id 0 -> a sentinel past every id), value = local index;
+//    stable radix sort by key gives the (|id|, local index) order.
+// 2. a run starts where the signed id differs from its predecessor; runs are
+//    numbered by an inclusive scan of the start flags.
+// 3. a group's final number is the rank of its first (smallest) member among
+//    all first members: flags over local indices, exclusive scan.
+// 4. offsets = exclusive scan of the group sizes in final order; members,
+//    1/multiplicity and the one-byte degree are scattered in one pass.
+// 5. the groups with >= 2 members are split by size (2, 4, 8, other) in final
+//    order with one exclusive scan per class.
+
+constexpr int kOgsBlock = 256;
+
+inline dim3 ogs_grid(int64_t n) { return dim3((unsigned)((n + kOgsBlock - 1) / kOgsBlock)); }
+
+__global__ void k_ogs_keys(int32_t N, const int64_t* __restrict__ ids, uint64_t sentinel, uint64_t* __restrict__ keys,
+                           int32_t* __restrict__ vals)
+{
+  const int64_t i = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (i >= N) return;
+  const int64_t v = ids[i];
+  keys[i] = v == 0 ? sentinel : (uint64_t)(v < 0 ? -v : v);
+  vals[i] = (int32_t)i;
+}
+
+__global__ void k_ogs_starts(int32_t Nlocal, const int64_t* __restrict__ ids, const int32_t* __restrict__ loc,
+                             int32_t* __restrict__ start)
+{
+  const int64_t n = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (n >= Nlocal) return;
+  start[n] = (n == 0 || ids[loc[n]] != ids[loc[n - 1]]) ? 1 : 0;
+}
+
+__global__ void k_ogs_runs(int32_t Nlocal, const int32_t* __restrict__ start, const int32_t* __restrict__ incl,
+                           const int32_t* __restrict__ loc, int32_t* __restrict__ runStart,
+                           int32_t* __restrict__ owner)
+{
+  const int64_t n = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (n >= Nlocal || !start[n]) return;
+  runStart[incl[n] - 1] = (int32_t)n;
+  owner[loc[n]] = 1;
+}
+
+__global__ void k_ogs_final(int32_t G, int32_t Nlocal, const int32_t* __restrict__ runStart,
+                            const int32_t* __restrict__ loc, const int32_t* __restrict__ rank,
+                            int32_t* __restrict__ finOfRun, int32_t* __restrict__ cntFin)
+{
+  const int64_t r = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (r >= G) return;
+  const int32_t s = runStart[r], e = r + 1 < G ? runStart[r + 1] : Nlocal;
+  const int32_t fin = rank[loc[s]];
+  finOfRun[r] = fin;
+  cntFin[fin] = e - s;
+}
+
+__global__ void k_ogs_members(int32_t Nlocal, const int32_t* __restrict__ incl, const int32_t* __restrict__ runStart,
+                              const int32_t* __restrict__ finOfRun, const int32_t* __restrict__ offsets,
+                              const int32_t* __restrict__ cntFin, const int32_t* __restrict__ loc,
+                              int32_t* __restrict__ gids, double* __restrict__ invDeg, uint8_t* __restrict__ deg,
+                              int* __restrict__ maxcnt)
+{
+  const int64_t n = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (n >= Nlocal) return;
+  const int32_t r = incl[n] - 1;
+  const int32_t m = (int32_t)n - runStart[r];
+  const int32_t fin = finOfRun[r];
+  const int32_t cnt = cntFin[fin];
+  const int32_t l = loc[n];
+  gids[offsets[fin] + m] = l;
+  invDeg[l] = 1.0 / (double)cnt;
+  deg[l] = (uint8_t)(cnt < 255 ? cnt : 255);
+  if (m == 0 && cnt > 255) atomicMax(maxcnt, cnt);
+}
+
+__global__ void k_ogs_init_weights(int32_t N, double* __restrict__ invDeg, uint8_t* __restrict__ deg)
+{
+  const int64_t i = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (i >= N) return;
+  invDeg[i] = 1.0;
+  deg[i] = 1;
+}
+
+// class flags in final order: 2, 4, 8 members, other sizes >= 2, and the
+// member count of the latter
+__global__ void k_ogs_classes(int32_t G, const int32_t* __restrict__ cntFin, int32_t* __restrict__ f2,
+                              int32_t* __restrict__ f4, int32_t* __restrict__ f8, int32_t* __restrict__ fg,
+                              int32_t* __restrict__ gc)
+{
+  const int64_t g = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (g >= G) return;
+  const int32_t c = cntFin[g];
+  const bool gen = c > 1 && c != 2 && c != 4 && c != 8;
+  f2[g] = c == 2;
+  f4[g] = c == 4;
+  f8[g] = c == 8;
+  fg[g] = gen;
+  gc[g] = gen ? c : 0;
+}
+
+__global__ void k_ogs_class_fill(int32_t G, const int32_t* __restrict__ offsets, const int32_t* __restrict__ cntFin,
+                                 const int32_t* __restrict__ gids, const int32_t* __restrict__ p2,
+                                 const int32_t* __restrict__ p4, const int32_t* __restrict__ p8,
+                                 const int32_t* __restrict__ pg, const int32_t* __restrict__ goff,
+                                 int32_t* __restrict__ fix2, int32_t* __restrict__ fix4, int32_t* __restrict__ fix8,
+                                 int32_t* __restrict__ sOff, int32_t* __restrict__ sIds)
+{
+  const int64_t g = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (g >= G) return;
+  const int32_t c = cntFin[g];
+  if (c < 2) return;
+  const int32_t* src = gids + offsets[g];
+  int32_t* dst;
+  if (c == 2) {
+    dst = fix2 + 2 * (int64_t)p2[g];
+  } else if (c == 4) {
+    dst = fix4 + 4 * (int64_t)p4[g];
+  } else if (c == 8) {
+    dst = fix8 + 8 * (int64_t)p8[g];
+  } else {
+    sOff[pg[g]] = goff[g];
+    dst = sIds + goff[g];
+  }
+  for (int32_t m = 0; m < c; ++m) dst[m] = src[m];
+}
+
+// device buffers of one setup, released together
+struct OgsTmp {
+  std::vector<void*> ptrs;
+  template <typename T>
+  int get(T** p, size_t n)
+  {
+    *p = nullptr;
+    if (hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess)
+      return fail(NBX_ERR_ALLOC, "ogs_setup: device allocation of %zu bytes failed", sizeof(T) * n);
+    ptrs.push_back(*p);
+    return NBX_SUCCESS;
+  }
+  ~OgsTmp()
+  {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+template <typename T>
+int exclusive_sum(OgsTmp& tmp, const T* in, T* out, int32_t n)
+{
+  size_t bytes = 0;
+  NBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n));
+  void* w;
+  if (int rc = tmp.get((char**)&w, bytes)) return rc;
+  NBX_HIP(hipcub::DeviceScan::ExclusiveSum(w, bytes, in, out, n));
+  return NBX_SUCCESS;
+}
+
+template <typename T>
+int inclusive_sum(OgsTmp& tmp, const T* in, T* out, int32_t n)
+{
+  size_t bytes = 0;
+  NBX_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out, n));
+  void* w;
+  if (int rc = tmp.get((char**)&w, bytes)) return rc;
+  NBX_HIP(hipcub::DeviceScan::InclusiveSum(w, bytes, in, out, n));
+  return NBX_SUCCESS;
+}
+
+int32_t read_i32(const int32_t* d)
+{
+  int32_t v = 0;
+  (void)hipMemcpy(&v, d, sizeof(int32_t), hipMemcpyDeviceToHost);
+  return v;
+}
+
+int ogs_build(nbx_ogs* o, int32_t N, const nbx_hlong* ids)
+{
+  // |id| range and participating count (host, one pass)
+  uint64_t amax = 0;
+  int32_t Nlocal = 0;
+  for (int32_t i = 0; i < N; ++i) {
+    const int64_t v = ids[i];
+    if (v == 0) continue;
+    const uint64_t a = (uint64_t)(v < 0 ? -v : v);
+    amax = std::max(amax, a);
+    ++Nlocal;
+  }
+  o->Nlocal = Nlocal;
+  NBX_HIP(hipMalloc(&o->d_invDegree, sizeof(double) * std::max<int32_t>(N, 1)));
+  NBX_HIP(hipMalloc(&o->d_degree, (size_t)std::max<int32_t>(N, 1)));
+  if (N > 0) {
+    hipLaunchKernelGGL(k_ogs_init_weights, ogs_grid(N), dim3(kOgsBlock), 0, nullptr, N, o->d_invDegree, o->d_degree);
+    NBX_HIP_LAUNCH();
+  }
+  o->h_offsets.assign(1, 0);
+  o->h_ids.assign(1, 0);
+  if (Nlocal == 0) {
+    o->NlocalGather = 0;
+    NBX_HIP(hipMalloc(&o->d_offsets, sizeof(int32_t)));
+    NBX_HIP(hipMemset(o->d_offsets, 0, sizeof(int32_t)));
+    NBX_HIP(hipMalloc(&o->d_ids, sizeof(int32_t)));
+    return NBX_SUCCESS;
+  }
+  int nbits = 1;
+  while (nbits < 63 && (amax >> nbits) != 0) ++nbits;
+  const uint64_t sentinel = 1ull << nbits;  // above every |id|; sorts the excluded nodes last
+
+  OgsTmp tmp;
+  int64_t* d_id;
+  uint64_t *k0, *k1;
+  int32_t *v0, *loc;
+  int rc;
+  if ((rc = tmp.get(&d_id, N)) || (rc = tmp.get(&k0, N)) || (rc = tmp.get(&k1, N)) || (rc = tmp.get(&v0, N)) ||
+      (rc = tmp.get(&loc, N)))
+    return rc;
+  NBX_HIP(hipMemcpy(d_id, ids, sizeof(int64_t) * N, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_ogs_keys, ogs_grid(N), dim3(kOgsBlock), 0, nullptr, N, d_id, sentinel, k0, v0);
+  NBX_HIP_LAUNCH();
+  {
+    size_t bytes = 0;
+    NBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k0, k1, v0, loc, N, 0, nbits + 1));
+    void* w;
+    if ((rc = tmp.get((char**)&w, bytes))) return rc;
+    NBX_HIP(hipcub::DeviceRadixSort::SortPairs(w, bytes, k0, k1, v0, loc, N, 0, nbits + 1));
+  }
+  // runs of equal signed id over the first Nlocal sorted nodes
+  int32_t *start, *incl, *owner, *rank;
+  if ((rc = tmp.get(&start, Nlocal)) || (rc = tmp.get(&incl, Nlocal)) || (rc = tmp.get(&owner, N)) ||
+      (rc = tmp.get(&rank, N)))
+    return rc;
+  hipLaunchKernelGGL(k_ogs_starts, ogs_grid(Nlocal), dim3(kOgsBlock), 0, nullptr, Nlocal, d_id, loc, start);
+  NBX_HIP_LAUNCH();
+  if ((rc = inclusive_sum(tmp, start, incl, Nlocal))) return rc;
+  const int32_t G = read_i32(incl + Nlocal - 1);
+  o->NlocalGather = G;
+  int32_t *runStart, *finOfRun, *cntFin, *offsets, *gids;
+  if ((rc = tmp.get(&runStart, G)) || (rc = tmp.get(&finOfRun, G)) || (rc = tmp.get(&cntFin, (size_t)G + 1)) ||
+      (rc = tmp.get(&offsets, (size_t)G + 1)) || (rc = tmp.get(&gids, Nlocal)))
+    return rc;
+  NBX_HIP(hipMemset(owner, 0, sizeof(int32_t) * N));
+  hipLaunchKernelGGL(k_ogs_runs, ogs_grid(Nlocal), dim3(kOgsBlock), 0, nullptr, Nlocal, start, incl, loc, runStart,
+                     owner);
+  NBX_HIP_LAUNCH();
+  if ((rc = exclusive_sum(tmp, owner, rank, N))) return rc;
+  NBX_HIP(hipMemset(cntFin + G, 0, sizeof(int32_t)));
+  hipLaunchKernelGGL(k_ogs_final, ogs_grid(G), dim3(kOgsBlock), 0, nullptr, G, Nlocal, runStart, loc, rank, finOfRun,
+                     cntFin);
+  NBX_HIP_LAUNCH();
+  if ((rc = exclusive_sum(tmp, cntFin, offsets, G + 1))) return rc;
+  int* maxcnt;
+  if ((rc = tmp.get(&maxcnt, 1))) return rc;
+  NBX_HIP(hipMemset(maxcnt, 0, sizeof(int)));
+  hipLaunchKernelGGL(k_ogs_members, ogs_grid(Nlocal), dim3(kOgsBlock), 0, nullptr, Nlocal, incl, runStart, finOfRun,
+                     offsets, cntFin, loc, gids, o->d_invDegree, o->d_degree, maxcnt);
+  NBX_HIP_LAUNCH();
+  // size classes of the shared groups
+  int32_t* fl[5];
+  int32_t* ps[5];
+  for (int c = 0; c < 5; ++c)
+    if ((rc = tmp.get(&fl[c], (size_t)G + 1)) || (rc = tmp.get(&ps[c], (size_t)G + 1))) return rc;
+  for (int c = 0; c < 5; ++c) NBX_HIP(hipMemset(fl[c] + G, 0, sizeof(int32_t)));
+  hipLaunchKernelGGL(k_ogs_classes, ogs_grid(G), dim3(kOgsBlock), 0, nullptr, G, cntFin, fl[0], fl[1], fl[2], fl[3],
+                     fl[4]);
+  NBX_HIP_LAUNCH();
+  for (int c = 0; c < 5; ++c)
+    if ((rc = exclusive_sum(tmp, fl[c], ps[c], G + 1))) return rc;
+  int32_t tot[5];
+  for (int c = 0; c < 5; ++c) tot[c] = read_i32(ps[c] + G);
+  for (int c = 0; c < 3; ++c) {
+    o->Nfix[c] = tot[c];
+    if (tot[c]) NBX_HIP(hipMalloc(&o->d_fix[c], sizeof(int32_t) * (size_t)tot[c] * (2 << c)));
+  }
+  o->Ngen = tot[3];
+  NBX_HIP(hipMalloc(&o->d_offsets, sizeof(int32_t) * ((size_t)tot[3] + 1)));
+  NBX_HIP(hipMalloc(&o->d_ids, sizeof(int32_t) * std::max<size_t>((size_t)tot[4], 1)));
+  NBX_HIP(hipMemcpy(o->d_offsets + tot[3], &tot[4], sizeof(int32_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_ogs_class_fill, ogs_grid(G), dim3(kOgsBlock), 0, nullptr, G, offsets, cntFin, gids, ps[0],
+                     ps[1], ps[2], ps[3], ps[4], o->d_fix[0], o->d_fix[1], o->d_fix[2], o->d_offsets, o->d_ids);
+  NBX_HIP_LAUNCH();
+  o->NsharedGather = tot[0] + tot[1] + tot[2] + tot[3];
+  o->Nshared = 2 * tot[0] + 4 * tot[1] + 8 * tot[2] + tot[4];
+  // host copy of the full CSR (nbx_ogs_copy_csr)
+  o->h_offsets.resize((size_t)G + 1);
+  o->h_ids.resize(Nlocal);
+  NBX_HIP(hipMemcpy(o->h_offsets.data(), offsets, sizeof(int32_t) * ((size_t)G + 1), hipMemcpyDeviceToHost));
+  NBX_HIP(hipMemcpy(o->h_ids.data(), gids, sizeof(int32_t) * (size_t)Nlocal, hipMemcpyDeviceToHost));
+  if (read_i32(maxcnt) > 255) {  // the one-byte degree cannot hold every group size
+    NBX_HIP(hipFree(o->d_degree));
+    o->d_degree = nullptr;
+  }
+  NBX_HIP(hipDeviceSynchronize());
+  return NBX_SUCCESS;
+}
+
+}  // namespace
+}  // namespace nbx
+
 extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose)
 {
   if (N < 0 || (N > 0 && !ids)) {
@@ -315,110 +615,16 @@ extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose
   }
   nbx_ogs* o = new nbx_ogs();
   o->N = N;
-  // (|id|, local index) for every participating node, sorted (ogsSetup.cpp:42-54,156);
-  // a group starts wherever the signed id differs from its predecessor
-  // (ogsSetup.cpp:163), so equal |id| with mixed signs splits into runs
-  struct Node {
-    uint64_t a;
-    int32_t local;
-    int64_t id;
-  };
-  std::vector<Node> nodes;
-  nodes.reserve(N);
-  for (int32_t i = 0; i < N; ++i)
-    if (ids[i] != 0) nodes.push_back(Node{(uint64_t)std::llabs(ids[i]), i, ids[i]});
-  std::sort(nodes.begin(), nodes.end(),
-            [](const Node& x, const Node& y) { return x.a < y.a || (x.a == y.a && x.local < y.local); });
-  const int32_t Nlocal = (int32_t)nodes.size();
-  o->Nlocal = Nlocal;
-  // a group's final number is the rank of its smallest member among all
-  // groups' smallest members (ogsSetup.cpp:177-188)
-  std::vector<int32_t> runStart;
-  for (int32_t n = 0; n < Nlocal; ++n)
-    if (n == 0 || nodes[n].id != nodes[n - 1].id) runStart.push_back(n);
-  const int32_t G = (int32_t)runStart.size();
-  runStart.push_back(Nlocal);
-  o->NlocalGather = G;
-  std::vector<int32_t> rankOf(N > 0 ? N : 1, -1);
-  {
-    std::vector<char> owner(N > 0 ? N : 1, 0);
-    for (int32_t g = 0; g < G; ++g) owner[nodes[runStart[g]].local] = 1;
-    int32_t c = 0;
-    for (int32_t i = 0; i < N; ++i)
-      if (owner[i]) rankOf[i] = c++;
+  const int rc = nbx::ogs_build(o, N, ids);
+  if (rc != NBX_SUCCESS) {
+    const std::string msg = nbx_last_error();
+    ogs_release(o);
+    fail(rc, "%s", msg.c_str());
+    return nullptr;
   }
-  o->h_offsets.assign((size_t)G + 1, 0);
-  std::vector<int32_t> order(G);  // run index of final group number
-  for (int32_t g = 0; g < G; ++g) {
-    const int32_t fin = rankOf[nodes[runStart[g]].local];
-    order[fin] = g;
-    o->h_offsets[fin + 1] = runStart[g + 1] - runStart[g];
-  }
-  for (int32_t g = 0; g < G; ++g) o->h_offsets[g + 1] += o->h_offsets[g];
-  o->h_ids.resize(Nlocal > 0 ? Nlocal : 1);
-  std::vector<double> invDeg(N > 0 ? N : 1, 1.0);
-  std::vector<uint8_t> deg(N > 0 ? N : 1, 1);
-  bool degFits = true;
-  std::vector<int32_t> sOff(1, 0), sIds, fix[3];
-  for (int32_t fin = 0; fin < G; ++fin) {
-    const int32_t r = order[fin];
-    const int32_t cnt = runStart[r + 1] - runStart[r];
-    int32_t* dst = o->h_ids.data() + o->h_offsets[fin];
-    for (int32_t m = 0; m < cnt; ++m) {
-      dst[m] = nodes[runStart[r] + m].local;
-      invDeg[dst[m]] = 1.0 / (double)cnt;  // ogsSetup.cpp:342-343
-      deg[dst[m]] = (uint8_t)std::min(cnt, 255);
-    }
-    if (cnt > 255) degFits = false;
-    if (cnt > 1) {
-      o->NsharedGather++;
-      o->Nshared += cnt;
-      const int c = cnt == 2 ? 0 : cnt == 4 ? 1 : cnt == 8 ? 2 : -1;
-      if (c >= 0) {
-        fix[c].insert(fix[c].end(), dst, dst + cnt);
-      } else {
-        sIds.insert(sIds.end(), dst, dst + cnt);
-        sOff.push_back((int32_t)sIds.size());
-      }
-    }
-  }
-  o->Ngen = (int32_t)sOff.size() - 1;
   if (verbose)
-    fprintf(stderr, "nbx_ogs_setup: N=%d Nlocal=%d groups=%d shared groups=%d shared nodes=%d\n", N, Nlocal, G,
-            o->NsharedGather, o->Nshared);
-  if (hipMalloc(&o->d_offsets, sizeof(int32_t) * sOff.size()) != hipSuccess ||
-      hipMalloc(&o->d_ids, sizeof(int32_t) * std::max<size_t>(sIds.size(), 1)) != hipSuccess ||
-      hipMalloc(&o->d_invDegree, sizeof(double) * std::max<int32_t>(N, 1)) != hipSuccess) {
-    fail(NBX_ERR_ALLOC, "ogs_setup: device allocation failed");
-    ogs_release(o);
-    return nullptr;
-  }
-  if (hipMemcpy(o->d_offsets, sOff.data(), sizeof(int32_t) * sOff.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      (sIds.size() &&
-       hipMemcpy(o->d_ids, sIds.data(), sizeof(int32_t) * sIds.size(), hipMemcpyHostToDevice) != hipSuccess) ||
-      (N > 0 && hipMemcpy(o->d_invDegree, invDeg.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)) {
-    fail(NBX_ERR_HIP, "ogs_setup: upload failed");
-    ogs_release(o);
-    return nullptr;
-  }
-  for (int c = 0; c < 3; ++c) {
-    if (fix[c].empty()) continue;
-    o->Nfix[c] = (int32_t)(fix[c].size() / (2 << c));
-    if (hipMalloc(&o->d_fix[c], sizeof(int32_t) * fix[c].size()) != hipSuccess ||
-        hipMemcpy(o->d_fix[c], fix[c].data(), sizeof(int32_t) * fix[c].size(), hipMemcpyHostToDevice) != hipSuccess) {
-      fail(NBX_ERR_HIP, "ogs_setup: class upload failed");
-      ogs_release(o);
-      return nullptr;
-    }
-  }
-  if (degFits && N > 0) {
-    if (hipMalloc(&o->d_degree, (size_t)N) != hipSuccess ||
-        hipMemcpy(o->d_degree, deg.data(), (size_t)N, hipMemcpyHostToDevice) != hipSuccess) {
-      fail(NBX_ERR_HIP, "ogs_setup: degree upload failed");
-      ogs_release(o);
-      return nullptr;
-    }
-  }
+    fprintf(stderr, "nbx_ogs_setup: N=%d Nlocal=%d groups=%d shared groups=%d shared nodes=%d\n", N, o->Nlocal,
+            o->NlocalGather, o->NsharedGather, o->Nshared);
   return o;
 }
 

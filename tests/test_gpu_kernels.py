@@ -111,10 +111,15 @@ def test_jacobi_local_matches_oracle(nbx, nbo, dev, meshes, N):
 
 def test_ogs_csr_matches_oracle(nbx, nbo):
     rng = np.random.default_rng(3)
+    big = np.array([3, 1 << 40, -(1 << 40), 7 << 33], dtype=np.int64)
     for ids in (rng.integers(-500, 501, size=5000).astype(np.int64),
                 nbo.box_mesh(nbo.Box.make(3, 3, 2, 2), mask_boundary=True)["ids"],
                 nbo.box_mesh(nbo.Box.make(2, 3, 3, 3, periodic=1))["ids"],
-                np.zeros(10, np.int64), np.arange(1, 11, dtype=np.int64)):
+                np.zeros(10, np.int64), np.arange(1, 11, dtype=np.int64),
+                np.tile(np.array([5, 0, 9], np.int64), 400),            # groups of 400 members
+                rng.choice(big, size=3000),                             # |id| up to 2^40, signed splits
+                rng.integers(1, 700, size=6000).astype(np.int64),       # mixed group sizes
+                nbo.box_mesh(nbo.Box.make(7, 5, 4, 3, perturb=0.1, seed=4), mask_boundary=True)["ids"]):
         o = nbx.Ogs(ids)
         oo, off, gid, invd = nbo.ogs_setup(ids)
         try:
@@ -165,6 +170,28 @@ def test_gs_many_matches_oracle(nbx, nbo, dev, type_, dtype, op):
     finally:
         nbo.ogs_free(oo)
         o.free()
+
+
+@pytest.mark.parametrize("op", ["add", "min"])
+def test_gs_large_and_mixed_groups(nbx, nbo, dev, op):
+    """Groups larger than 255 members (no one-byte degree) and groups of every
+    size up to ~20 through the size-class lists and the generic CSR."""
+    import torch
+    rng = np.random.default_rng(21)
+    for ids in (np.tile(np.array([5, 0, 9], np.int64), 400), rng.integers(1, 700, size=6000).astype(np.int64)):
+        q = rng.standard_normal(ids.size)
+        oo, *_ = nbo.ogs_setup(ids)
+        o = nbx.Ogs(ids)
+        try:
+            ref = q.copy()
+            nbo.gs_many(oo, ref, 1, ids.size, "double", op)
+            dq = T(dev, q)
+            o.gather_scatter(dq, "double", op)
+            torch.cuda.synchronize()
+            assert np.array_equal(H(dq), ref)
+        finally:
+            nbo.ogs_free(oo)
+            o.free()
 
 
 def test_gs_degree_round_trip_gpu(nbx, dev):
