@@ -105,6 +105,10 @@ int nbx_update_jacobi_local(int N, nbx_dlong Nelements, nbx_dlong offset, const 
  * (3rdParty/ogs/okl/gatherScatterMany.okl:58-86). */
 typedef struct nbx_ogs nbx_ogs;
 nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose);
+/* Same, with ids in device memory (for ids produced on the GPU, e.g. by
+ * nbx_box_mesh_device).  The CSR is built on the GPU in both forms: stable
+ * radix sort of (|id|, local index), then scans. */
+nbx_ogs* nbx_ogs_setup_device(nbx_dlong N, const nbx_hlong* ids, int verbose);
 int nbx_ogs_gather_scatter_many(nbx_ogs* ogs, void* v, int k, nbx_dlong stride,
                                 nbx_type type, nbx_op op, void* stream);
 int nbx_ogs_gather_scatter(nbx_ogs* ogs, void* v, nbx_type type, nbx_op op, void* stream);
@@ -144,6 +148,10 @@ int nbx_urand_fill(uint64_t seed, uint64_t stream, int64_t n, double* out);
  * Returns the minimum Jacobian through minJ (may be NULL). */
 int nbx_box_mesh(const nbx_box* box, int mask_boundary, double* x, double* y, double* z,
                  double* ggeo, nbx_hlong* globalIds, double* minJ);
+/* nbx_box_mesh's ggeo and global ids computed on the GPU into device arrays
+ * (either may be NULL), equal to the host ones bit for bit; synchronous on
+ * `stream`.  (core/meshSetup.cpp:266-360, :1638-1735 on the device.) */
+int nbx_box_mesh_device(const nbx_box* box, int mask_boundary, double* ggeo, nbx_hlong* globalIds, void* stream);
 
 /* ======================= BLAS-1 of the PCG loop ==========================
  * nekBone/kernel/utils.okl.  Reductions return their value through a device

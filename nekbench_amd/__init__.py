@@ -88,6 +88,8 @@ _SIGS = {
     "nbx_box_mesh": (_I, [C.POINTER(Box), _I, _P, _P, _P, _P, _P, _P]),
     "nbx_weighted_inner_product": (_I, [C.c_int32, C.c_int32, _I, _P, _P, _P, _P, _P]),
     "nbx_scaled_add": (_I, [C.c_int32, _D, _P, _D, _P, _P]),
+    "nbx_box_mesh_device": (_I, [C.POINTER(Box), _I, _P, _P, _P]),
+    "nbx_ogs_setup_device": (_P, [C.c_int32, _P, _I]),
     "nbx_triad": (_I, [C.c_int32, _D, _P, _P, _P, _P]),
     "nbx_copy": (_I, [C.c_int32, _P, _P, _P]),
     "nbx_read_sum": (_I, [C.c_int32, _P, _P, _P]),

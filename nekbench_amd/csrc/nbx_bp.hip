@@ -301,6 +301,14 @@ static void bp_release(nbx_bp* bp)
   delete bp;
 }
 
+__global__ __launch_bounds__(256) void k_fill_lambda(const int64_t n, const double l1, double* __restrict__ lam)
+{
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  lam[i] = 1.0;
+  lam[i + n] = l1;
+}
+
 static int bp_setup_impl(nbx_bp* bp)
 {
   const nbx_bp_options& o = bp->opt;
@@ -326,22 +334,10 @@ static int bp_setup_impl(nbx_bp* bp)
   bp->allNeumann = lambda1 == 0.0 && !o.dirichlet;
   const bool manufactured = o.rhs_kind == 1;
 
-  // host mesh inputs
-  std::vector<double> ggeo((size_t)E * 7 * bp->Np), x, y, z;
-  std::vector<nbx_hlong> ids(bp->Nlocal);
-  if (manufactured) {
-    x.resize(bp->Nlocal);
-    y.resize(bp->Nlocal);
-    z.resize(bp->Nlocal);
-  }
-  int rc = nbx_box_mesh(&b, o.dirichlet ? 1 : 0, manufactured ? x.data() : nullptr,
-                        manufactured ? y.data() : nullptr, manufactured ? z.data() : nullptr, ggeo.data(),
-                        ids.data(), nullptr);
-  if (rc) return rc;
   std::vector<double> zg(bp->Nq), wg(bp->Nq), D(bp->Nq * bp->Nq);
   nbx_gll(bp->N, zg.data(), wg.data(), D.data());
-
-  if ((rc = bp->alloc(&bp->D, D.size())) || (rc = bp->alloc(&bp->ggeo, (int64_t)ggeo.size())) ||
+  int rc;
+  if ((rc = bp->alloc(&bp->D, D.size())) || (rc = bp->alloc(&bp->ggeo, (int64_t)E * 7 * bp->Np)) ||
       (rc = bp->alloc(&bp->lambda, 2 * (int64_t)bp->fieldOffset)) || (rc = bp->alloc(&bp->invDiagA, bp->Nlocal)) ||
       (rc = bp->alloc(&bp->rhs, bp->Nall)) || (rc = bp->alloc(&bp->p, bp->Nall)) ||
       (rc = bp->alloc(&bp->z, bp->Nall)) || (rc = bp->alloc(&bp->Ap, bp->Nall)) || (rc = bp->alloc(&bp->Ax, bp->Nall)) ||
@@ -349,44 +345,61 @@ static int bp_setup_impl(nbx_bp* bp)
       (rc = bp->alloc(&bp->d, bp->Nall)))
     return rc;
   NBX_HIP(hipMemcpy(bp->D, D.data(), sizeof(double) * D.size(), hipMemcpyHostToDevice));
-  NBX_HIP(hipMemcpy(bp->ggeo, ggeo.data(), sizeof(double) * ggeo.size(), hipMemcpyHostToDevice));
-  {
-    std::vector<double> lam(2 * (size_t)bp->fieldOffset);
-    for (int32_t i = 0; i < bp->fieldOffset; ++i) {  // setup.cpp:104-108
-      lam[i] = 1.0;
-      lam[i + (size_t)bp->fieldOffset] = lambda1;
-    }
-    NBX_HIP(hipMemcpy(bp->lambda, lam.data(), sizeof(double) * lam.size(), hipMemcpyHostToDevice));
-  }
-  // gather-scatter on the (masked) ids
-  bp->ogs = nbx_ogs_setup(bp->Nlocal, ids.data(), o.verbose);
-  if (!bp->ogs) return NBX_ERR_HIP;
-  std::vector<int32_t> mask;
-  for (int32_t i = 0; i < bp->Nlocal; ++i)
-    if (ids[i] == 0) mask.push_back(i);
-  bp->Nmasked = (int32_t)mask.size();
-  NBX_HIP(hipMalloc(&bp->maskIds, sizeof(int32_t) * std::max<size_t>(mask.size(), 1)));
-  if (!mask.empty())
-    NBX_HIP(hipMemcpy(bp->maskIds, mask.data(), sizeof(int32_t) * mask.size(), hipMemcpyHostToDevice));
+  // lambda0 = 1, lambda1 per node (setup.cpp:104-108)
+  hipLaunchKernelGGL(k_fill_lambda, grid_for(bp->fieldOffset), dim3(256), 0, bp->stream, (int64_t)bp->fieldOffset,
+                     lambda1, bp->lambda);
+  NBX_HIP_LAUNCH();
 
-  // right-hand side (setup.cpp:77-99): manufactured cosine or seeded uniform,
-  // masked, then assembled
-  {
-    std::vector<double> r(bp->Nall);
-    if (manufactured) {
-      for (int32_t e = 0; e < bp->E; ++e)
-        for (int n = 0; n < bp->Np; ++n) {
-          const int64_t id = (int64_t)e * bp->Np + n;
-          const double JW = ggeo[(size_t)bp->Np * (e * 7 + 0) + n];
-          const double v = (3. * M_PI * M_PI + lambda1) * JW * cos(M_PI * x[id]) * cos(M_PI * y[id]) *
-                           cos(M_PI * z[id]);
-          for (int f = 0; f < bp->Nf; ++f) r[id + (int64_t)f * bp->fieldOffset] = v;
-        }
-    } else {
-      for (int f = 0; f < bp->Nf; ++f) {
-        nbx_urand_fill(b.seed, 3 + f, bp->Nlocal, r.data() + (int64_t)f * bp->fieldOffset);
-      }
+  if (!manufactured) {
+    // everything on the device: geometry, global ids, gather-scatter CSR,
+    // Dirichlet list, seeded right-hand side (masked, assembled)
+    if ((rc = box_geometry_launch(b, bp->ggeo, nullptr, nullptr, nullptr, bp->stream))) return rc;
+    int64_t* d_ids = nullptr;
+    NBX_HIP(hipMalloc(&d_ids, sizeof(int64_t) * std::max<int64_t>(bp->Nlocal, 1)));
+    rc = box_ids_launch(b, o.dirichlet ? 1 : 0, d_ids, bp->stream);
+    if (rc == NBX_SUCCESS && hipStreamSynchronize(bp->stream) != hipSuccess) rc = fail(NBX_ERR_HIP, "bp: ids failed");
+    if (rc == NBX_SUCCESS) {
+      bp->ogs = nbx_ogs_setup_device(bp->Nlocal, d_ids, o.verbose);
+      if (!bp->ogs) rc = NBX_ERR_HIP;
     }
+    if (rc == NBX_SUCCESS) rc = mask_list_from_ids(bp->Nlocal, d_ids, &bp->maskIds, &bp->Nmasked, bp->stream);
+    (void)hipFree(d_ids);
+    if (rc) return rc;
+    for (int f = 0; f < bp->Nf && rc == NBX_SUCCESS; ++f) {
+      double* rf = bp->rhs + (int64_t)f * bp->fieldOffset;
+      rc = urand_fill_launch(b.seed, 3 + f, bp->Nlocal, rf, bp->stream);
+      if (rc == NBX_SUCCESS) rc = mask_launch(bp->Nmasked, bp->maskIds, rf, nullptr, bp->stream);
+    }
+    if (rc) return rc;
+    rc = gs_launch(bp->ogs, bp->rhs, bp->Nf, bp->fieldOffset, NBX_DOUBLE, NBX_ADD, nullptr, bp->stream);
+    if (rc) return rc;
+  } else {
+    // the manufactured cosine needs node coordinates and the host cos:
+    // host mesh, uploaded
+    std::vector<double> ggeo((size_t)E * 7 * bp->Np), x(bp->Nlocal), y(bp->Nlocal), z(bp->Nlocal);
+    std::vector<nbx_hlong> ids(bp->Nlocal);
+    rc = nbx_box_mesh(&b, o.dirichlet ? 1 : 0, x.data(), y.data(), z.data(), ggeo.data(), ids.data(), nullptr);
+    if (rc) return rc;
+    NBX_HIP(hipMemcpy(bp->ggeo, ggeo.data(), sizeof(double) * ggeo.size(), hipMemcpyHostToDevice));
+    bp->ogs = nbx_ogs_setup(bp->Nlocal, ids.data(), o.verbose);
+    if (!bp->ogs) return NBX_ERR_HIP;
+    std::vector<int32_t> mask;
+    for (int32_t i = 0; i < bp->Nlocal; ++i)
+      if (ids[i] == 0) mask.push_back(i);
+    bp->Nmasked = (int32_t)mask.size();
+    NBX_HIP(hipMalloc(&bp->maskIds, sizeof(int32_t) * std::max<size_t>(mask.size(), 1)));
+    if (!mask.empty())
+      NBX_HIP(hipMemcpy(bp->maskIds, mask.data(), sizeof(int32_t) * mask.size(), hipMemcpyHostToDevice));
+    // right-hand side (setup.cpp:77-99), masked, then assembled
+    std::vector<double> r(bp->Nall);
+    for (int32_t e = 0; e < bp->E; ++e)
+      for (int n = 0; n < bp->Np; ++n) {
+        const int64_t id = (int64_t)e * bp->Np + n;
+        const double JW = ggeo[(size_t)bp->Np * (e * 7 + 0) + n];
+        const double v = (3. * M_PI * M_PI + lambda1) * JW * cos(M_PI * x[id]) * cos(M_PI * y[id]) *
+                         cos(M_PI * z[id]);
+        for (int f = 0; f < bp->Nf; ++f) r[id + (int64_t)f * bp->fieldOffset] = v;
+      }
     for (int f = 0; f < bp->Nf; ++f)
       for (int32_t m : mask) r[m + (int64_t)f * bp->fieldOffset] = 0.0;
     NBX_HIP(hipMemcpy(bp->rhs, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice));

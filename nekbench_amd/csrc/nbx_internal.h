@@ -45,6 +45,12 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
 int gs_launch(const nbx_ogs* o, void* v, int k, int32_t stride, nbx_type type, nbx_op op, const int* skip,
               hipStream_t s);
 int mask_launch(int32_t Nmasked, const int32_t* maskIds, double* q, const int* skip, hipStream_t s);
+
+// device mesh producers (nbx_mesh_dev.hip); equal to nbx_box_mesh bit for bit
+int box_geometry_launch(const nbx_box& b, double* ggeo, double* x, double* y, double* z, hipStream_t s);
+int box_ids_launch(const nbx_box& b, int mask_boundary, int64_t* ids, hipStream_t s);
+int urand_fill_launch(uint64_t seed, uint64_t stream, int64_t n, double* out, hipStream_t s);
+int mask_list_from_ids(int32_t N, const int64_t* ids, int32_t** maskIds, int32_t* Nmasked, hipStream_t s);
 // p = z + beta p (FP64); Ap = A p (local) with the operator in precision prec
 // (F64: double ggeo/D/lambda; F32: float; BF16: bf16 ggeo/lambda, float D and
 // float arithmetic); partials[e] = element share of pAp (FP64)

@@ -97,6 +97,43 @@ void zplanes(const nbx_box& b, std::vector<double>& zp)
 
 }  // namespace
 
+namespace nbx {
+
+// Vertex coordinates of the box, 3 doubles per vertex (I, J, K) at
+// 3 * (I + (NX+1) * (J + (NY+1) * K)): planes x0 + hx I, y0 + hy J, z-plane K;
+// interior vertices moved by perturb * (hx, hy, hz) * U[-1,1) from stream 1,
+// hz the smaller neighbouring z spacing.
+void box_vertices(const nbx_box& b, std::vector<double>& V)
+{
+  std::vector<double> zp;
+  zplanes(b, zp);
+  const int NX = b.NX, NY = b.NY, NZ = b.NZ;
+  const double hx = (b.x1 - b.x0) / NX, hy = (b.y1 - b.y0) / NY;
+  const Urand vrand(b.seed, 1);
+  V.resize((size_t)3 * (NX + 1) * (NY + 1) * (NZ + 1));
+#pragma omp parallel for schedule(static)
+  for (int K = 0; K <= NZ; ++K)
+    for (int J = 0; J <= NY; ++J)
+      for (int I = 0; I <= NX; ++I) {
+        double px = b.x0 + hx * I, py = b.y0 + hy * J, pz = zp[K];
+        const uint64_t vid = (uint64_t)(I + (int64_t)(NX + 1) * (J + (int64_t)(NY + 1) * K));
+        if (b.perturb != 0.0 && I > 0 && I < NX && J > 0 && J < NY && K > 0 && K < NZ) {
+          const double hz = std::min(zp[K] - zp[K - 1], zp[K + 1] - zp[K]);
+          px += b.perturb * hx * vrand(3 * vid + 0);
+          py += b.perturb * hy * vrand(3 * vid + 1);
+          pz += b.perturb * hz * vrand(3 * vid + 2);
+        }
+        V[3 * vid + 0] = px;
+        V[3 * vid + 1] = py;
+        V[3 * vid + 2] = pz;
+      }
+}
+
+// key of a splitmix64 uniform stream (the device fill uses it)
+uint64_t urand_key(uint64_t seed, uint64_t stream) { return Urand(seed, stream).key; }
+
+}  // namespace nbx
+
 extern "C" int nbx_gll(int N, double* z, double* w, double* D)
 {
   if (N < 1 || N > 31 || !z || !w) return nbx::fail(NBX_ERR_INVALID, "gll: bad arguments");
@@ -141,10 +178,8 @@ extern "C" int nbx_box_mesh(const nbx_box* box, int mask_boundary, double* x, do
   if (E * Np > INT32_MAX) return nbx::fail(NBX_ERR_INVALID, "box_mesh: E*Np exceeds dlong range");
   std::vector<double> zr(Nq), wr(Nq), D(Nq2);
   nbx_gll(N, zr.data(), wr.data(), D.data());
-  std::vector<double> zp;
-  zplanes(b, zp);
-  const double hx = (b.x1 - b.x0) / NX, hy = (b.y1 - b.y0) / NY;
-  const Urand vrand(b.seed, 1);
+  std::vector<double> V;
+  nbx::box_vertices(b, V);
   static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                 {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
   double minJ = 1e300;
@@ -157,15 +192,8 @@ extern "C" int nbx_box_mesh(const nbx_box* box, int mask_boundary, double* x, do
       double vx[8], vy[8], vz[8];
       for (int v = 0; v < 8; ++v) {
         const int I = ex + off[v][0], J = ey + off[v][1], K = ez + off[v][2];
-        double px = b.x0 + hx * I, py = b.y0 + hy * J, pz = zp[K];
-        if (b.perturb != 0.0 && I > 0 && I < NX && J > 0 && J < NY && K > 0 && K < NZ) {
-          const uint64_t vid = (uint64_t)(I + (int64_t)(NX + 1) * (J + (int64_t)(NY + 1) * K));
-          const double hz = std::min(zp[K] - zp[K - 1], zp[K + 1] - zp[K]);
-          px += b.perturb * hx * vrand(3 * vid + 0);
-          py += b.perturb * hy * vrand(3 * vid + 1);
-          pz += b.perturb * hz * vrand(3 * vid + 2);
-        }
-        vx[v] = px; vy[v] = py; vz[v] = pz;
+        const size_t vid = (size_t)I + (size_t)(NX + 1) * (J + (size_t)(NY + 1) * K);
+        vx[v] = V[3 * vid + 0]; vy[v] = V[3 * vid + 1]; vz[v] = V[3 * vid + 2];
       }
       // trilinear map, core/meshSetup.cpp:2198-2230 expression order
       for (int n = 0; n < Np; ++n) {

@@ -328,6 +328,18 @@ constexpr int kOgsBlock = 256;
 
 inline dim3 ogs_grid(int64_t n) { return dim3((unsigned)((n + kOgsBlock - 1) / kOgsBlock)); }
 
+// largest |id| and number of nonzero ids (order-independent totals)
+__global__ void k_ogs_range(int32_t N, const int64_t* __restrict__ ids, unsigned long long* __restrict__ amax,
+                            int* __restrict__ count)
+{
+  const int64_t i = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
+  if (i >= N) return;
+  const int64_t v = ids[i];
+  if (v == 0) return;
+  atomicMax(amax, (unsigned long long)(v < 0 ? -v : v));
+  atomicAdd(count, 1);
+}
+
 __global__ void k_ogs_keys(int32_t N, const int64_t* __restrict__ ids, uint64_t sentinel, uint64_t* __restrict__ keys,
                            int32_t* __restrict__ vals)
 {
@@ -485,17 +497,31 @@ int32_t read_i32(const int32_t* d)
   return v;
 }
 
-int ogs_build(nbx_ogs* o, int32_t N, const nbx_hlong* ids)
+// ids: host array, or device array when on_device
+int ogs_build(nbx_ogs* o, int32_t N, const nbx_hlong* ids, bool on_device)
 {
-  // |id| range and participating count (host, one pass)
+  // |id| range and participating count
   uint64_t amax = 0;
   int32_t Nlocal = 0;
-  for (int32_t i = 0; i < N; ++i) {
-    const int64_t v = ids[i];
-    if (v == 0) continue;
-    const uint64_t a = (uint64_t)(v < 0 ? -v : v);
-    amax = std::max(amax, a);
-    ++Nlocal;
+  if (!on_device) {
+    for (int32_t i = 0; i < N; ++i) {
+      const int64_t v = ids[i];
+      if (v == 0) continue;
+      const uint64_t a = (uint64_t)(v < 0 ? -v : v);
+      amax = std::max(amax, a);
+      ++Nlocal;
+    }
+  } else if (N > 0) {
+    unsigned long long* d_r = nullptr;
+    NBX_HIP(hipMalloc(&d_r, 2 * sizeof(unsigned long long)));
+    (void)hipMemset(d_r, 0, 2 * sizeof(unsigned long long));
+    hipLaunchKernelGGL(k_ogs_range, ogs_grid(N), dim3(kOgsBlock), 0, nullptr, N, ids, d_r, (int*)(d_r + 1));
+    unsigned long long h[2] = {0, 0};
+    const hipError_t e1 = hipGetLastError(), e2 = hipMemcpy(h, d_r, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d_r);
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(NBX_ERR_HIP, "ogs_setup: id range failed");
+    amax = h[0];
+    Nlocal = (int32_t)(h[1] & 0xffffffffu);
   }
   o->Nlocal = Nlocal;
   NBX_HIP(hipMalloc(&o->d_invDegree, sizeof(double) * std::max<int32_t>(N, 1)));
@@ -518,14 +544,18 @@ int ogs_build(nbx_ogs* o, int32_t N, const nbx_hlong* ids)
   const uint64_t sentinel = 1ull << nbits;  // above every |id|; sorts the excluded nodes last
 
   OgsTmp tmp;
-  int64_t* d_id;
+  const int64_t* d_id = ids;
   uint64_t *k0, *k1;
   int32_t *v0, *loc;
   int rc;
-  if ((rc = tmp.get(&d_id, N)) || (rc = tmp.get(&k0, N)) || (rc = tmp.get(&k1, N)) || (rc = tmp.get(&v0, N)) ||
-      (rc = tmp.get(&loc, N)))
+  if ((rc = tmp.get(&k0, N)) || (rc = tmp.get(&k1, N)) || (rc = tmp.get(&v0, N)) || (rc = tmp.get(&loc, N)))
     return rc;
-  NBX_HIP(hipMemcpy(d_id, ids, sizeof(int64_t) * N, hipMemcpyHostToDevice));
+  if (!on_device) {
+    int64_t* up;
+    if ((rc = tmp.get(&up, N))) return rc;
+    NBX_HIP(hipMemcpy(up, ids, sizeof(int64_t) * N, hipMemcpyHostToDevice));
+    d_id = up;
+  }
   hipLaunchKernelGGL(k_ogs_keys, ogs_grid(N), dim3(kOgsBlock), 0, nullptr, N, d_id, sentinel, k0, v0);
   NBX_HIP_LAUNCH();
   {
@@ -607,7 +637,7 @@ int ogs_build(nbx_ogs* o, int32_t N, const nbx_hlong* ids)
 }  // namespace
 }  // namespace nbx
 
-extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose)
+static nbx_ogs* ogs_setup_common(nbx_dlong N, const nbx_hlong* ids, int verbose, bool on_device)
 {
   if (N < 0 || (N > 0 && !ids)) {
     fail(NBX_ERR_INVALID, "ogs_setup: bad arguments");
@@ -615,7 +645,7 @@ extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose
   }
   nbx_ogs* o = new nbx_ogs();
   o->N = N;
-  const int rc = nbx::ogs_build(o, N, ids);
+  const int rc = nbx::ogs_build(o, N, ids, on_device);
   if (rc != NBX_SUCCESS) {
     const std::string msg = nbx_last_error();
     ogs_release(o);
@@ -626,6 +656,16 @@ extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose
     fprintf(stderr, "nbx_ogs_setup: N=%d Nlocal=%d groups=%d shared groups=%d shared nodes=%d\n", N, o->Nlocal,
             o->NlocalGather, o->NsharedGather, o->Nshared);
   return o;
+}
+
+extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose)
+{
+  return ogs_setup_common(N, ids, verbose, false);
+}
+
+extern "C" nbx_ogs* nbx_ogs_setup_device(nbx_dlong N, const nbx_hlong* ids, int verbose)
+{
+  return ogs_setup_common(N, ids, verbose, true);
 }
 
 extern "C" void nbx_ogs_free(nbx_ogs* o) { ogs_release(o); }

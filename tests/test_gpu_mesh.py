@@ -1,0 +1,72 @@
+"""Device mesh producers against the host ones, bit for bit: nbx_box_mesh_device
+(ggeo of core/meshSetup.cpp:266-360 over the trilinear map, global ids of
+core/meshSetup.cpp:1638-1735 in closed form) and nbx_ogs_setup_device (the
+gather-scatter CSR from device ids).  The host producers are pinned against
+the oracle and the reference's own numbers elsewhere (test_oracle_mesh.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import nekbench_amd as nbx
+
+pytestmark = pytest.mark.gpu
+
+BOXES = [
+    dict(N=7, dims=(5, 4, 3), kw=dict(perturb=0.1, seed=3), mask=1),
+    dict(N=7, dims=(6, 5, 8), kw=dict(perturb=0.1, seed=5, zaspect=10.0), mask=1),
+    dict(N=3, dims=(4, 4, 4), kw=dict(periodic=1), mask=0),
+    dict(N=5, dims=(3, 5, 4), kw=dict(periodic=1, perturb=0.05, seed=2), mask=0),
+    dict(N=1, dims=(3, 3, 3), kw=dict(), mask=1),
+    dict(N=2, dims=(2, 3, 2), kw=dict(perturb=0.2, seed=9), mask=0),
+    dict(N=9, dims=(3, 3, 3), kw=dict(perturb=0.1, seed=7), mask=1),
+    dict(N=15, dims=(2, 2, 1), kw=dict(perturb=0.1, seed=1), mask=1),
+]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.mark.parametrize("case", BOXES, ids=[f"N{c['N']}_{'x'.join(map(str, c['dims']))}" for c in BOXES])
+def test_box_mesh_device_equals_host(torch, case):
+    N, (NX, NY, NZ) = case["N"], case["dims"]
+    print(f"[mesh] N={N} {NX}x{NY}x{NZ} {case['kw']}", flush=True)
+    box = nbx.Box.make(N, NX, NY, NZ, **case["kw"])
+    host = nbx.box_mesh(box, mask_boundary=bool(case["mask"]))
+    E, Np = NX * NY * NZ, (N + 1) ** 3
+    g = torch.empty(E * 7 * Np, dtype=torch.float64, device="cuda")
+    ids = torch.empty(E * Np, dtype=torch.int64, device="cuda")
+    rc = nbx.lib().nbx_box_mesh_device(C.byref(box), case["mask"], g.data_ptr(), ids.data_ptr(), None)
+    assert rc == 0, nbx.last_error()
+    assert np.array_equal(g.cpu().numpy(), host["ggeo"])
+    assert np.array_equal(ids.cpu().numpy(), host["ids"])
+
+
+def test_box_mesh_device_arguments(torch):
+    L = nbx.lib()
+    g = torch.empty(8, dtype=torch.float64, device="cuda")
+    assert L.nbx_box_mesh_device(None, 0, g.data_ptr(), None, None) == -1
+    assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(7, 2, 2, 2, periodic=1)), 0, g.data_ptr(), None, None) == -1
+    assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(16, 1, 1, 1)), 0, g.data_ptr(), None, None) == -1
+    assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(3, 2, 2, 2)), 0, None, None, None) == 0
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_ogs_setup_device_equals_host(torch, periodic):
+    box = nbx.Box.make(7, 4, 3, 5, periodic=periodic, perturb=0.1, seed=4)
+    ids_h = nbx.box_mesh(box, mask_boundary=not periodic)["ids"]
+    ids_d = torch.from_numpy(ids_h).cuda()
+    h = nbx.lib().nbx_ogs_setup_device(ids_h.size, ids_d.data_ptr(), 0)
+    assert h, nbx.last_error()
+    od, oh = nbx.Ogs(None, handle=h), nbx.Ogs(ids_h)
+    try:
+        assert od.sizes() == oh.sizes()
+        (a0, a1), (b0, b1) = od.csr(), oh.csr()
+        assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+    finally:
+        nbx.lib().nbx_ogs_free(C.c_void_p(h))
+        oh.free()
