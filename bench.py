@@ -481,7 +481,9 @@ def main():
     box = nbx.Box.make(N, NX, NX, NX, perturb=0.1, seed=1)
     t0 = time.perf_counter()
     bp = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI)  # Poisson: lambda0=1, lambda1=0
-    log(f"[bench] setup {time.perf_counter() - t0:.1f} s: E={bp.Nelements} Nlocal={bp.Nlocal}")
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"[bench] setup {setup_s:.2f} s: E={bp.Nelements} Nlocal={bp.Nlocal}")
     n = bp.fieldOffset
     b = bp.device_array("rhs", n).clone()
     r = torch.empty_like(b)
@@ -588,7 +590,7 @@ def main():
                    "iterations": its, "converged": bool(st.converged),
                    "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
                    "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "axhelm_variants": variants,
-                   "parallelism": "single GPU",
+                   "parallelism": "single GPU", "setup_s": round(setup_s, 3),
                    "time_to_1e-8": compare, "config1_16cube": config1},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
