@@ -52,7 +52,7 @@ def cpu_baseline(N, NX, NY, NZ, iters):
     return {
         "value": iters * E * N ** 3 / t / 1e9, "unit": "GDOF/s", "cores": int(nbo.lib().nbo_max_threads()),
         "kind": "port",
-        "sample": f"CPU oracle (nbo_bp_pcg, OpenMP): same 32^3 N=7 problem, Jacobi PCG fixed {iters} iterations "
+        "sample": f"CPU oracle (nbo_bp_pcg, OpenMP, {cpu_model()}): same 32^3 N=7 problem, Jacobi PCG fixed {iters} iterations "
                   f"incl. r=b-Ax and the Jacobi setup, {t:.2f} s (+{t_setup:.1f} s mesh/ogs setup untimed)",
     }
 
@@ -318,11 +318,23 @@ def run_config5(nbx, torch, args):
             "elements": 64 * 64 * 32, "learned_training": info, "solvers": out}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_reference_axhelm(bj, E, n, g0, D, log):
     """The reference's own CPU kernel axhelm_bk_v0 (axhelm/kernel/CPU/axhelm.c,
     compiled by oracle/Makefile into oracle/_ref, OpenMP) on the same 16^3
     mesh, with axhelm/main.cpp's protocol: one warm-up call, then the mean of
-    100 calls; host cores as OMP_NUM_THREADS gives them."""
+    100 calls into a preallocated output; host cores as OMP_NUM_THREADS gives
+    them.  The oracle's restatement (nbo_axhelm) is timed the same way beside
+    it, which calibrates the oracle as a CPU baseline."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
     import nbo
@@ -331,14 +343,21 @@ def cpu_reference_axhelm(bj, E, n, g0, D, log):
     ggeo = g0.cpu().numpy()
     Dh = D.cpu().numpy()
     q = np.random.default_rng(1).uniform(-1, 1, n)
-    nbo.ref_axhelm(7, "axhelm_bk_v0", E, n, ggeo, Dh, None, q)
-    t0 = time.perf_counter()
-    for _ in range(100):
-        nbo.ref_axhelm(7, "axhelm_bk_v0", E, n, ggeo, Dh, None, q)
-    t = (time.perf_counter() - t0) / 100
+    out = np.zeros(n)
+
+    def mean_time(fn, reps=100):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    t = mean_time(lambda: nbo.ref_axhelm(7, "axhelm_bk_v0", E, n, ggeo, Dh, None, q, out=out))
+    t_port = mean_time(lambda: nbo.axhelm(7, 1, 1, E, n, ggeo, Dh, None, q, out=out))
     res = {"kind": "reference", "kernel": "axhelm_bk_v0 of axhelm/kernel/CPU/axhelm.c (g++ -O2 -fopenmp)",
            "us_per_apply": round(t * 1e6, 1), "GB_s": round(E * 512 * 72 / t / 1e9, 1),
-           "gdofs": round(E * 343 / t / 1e9, 3), "cores": int(nbo.lib().nbo_max_threads())}
+           "gdofs": round(E * 343 / t / 1e9, 3), "cores": int(nbo.lib().nbo_max_threads()), "cpu": cpu_model(),
+           "oracle_port_us_per_apply": round(t_port * 1e6, 1), "oracle_port_over_reference": round(t_port / t, 3)}
     log(f"[bench] reference CPU axhelm: {res}")
     return res
 

@@ -168,11 +168,11 @@ def _ref(name):
     return _ref_cache[name]
 
 
-def ref_axhelm(N, kernel, E, offset, ggeo, D, lam, q):
+def ref_axhelm(N, kernel, E, offset, ggeo, D, lam, q, out=None):
     """Call the reference's compiled CPU kernel axhelm[_bk][_n3]_v0 (axhelm/kernel/CPU/axhelm.c)."""
     l = _ref(f"libaxhelm_N{N}.so")
     f = getattr(l, kernel)
-    Aq = np.zeros_like(q)
+    Aq = np.zeros_like(q) if out is None else out
     if lam is None:
         lam = np.zeros(2 * offset)
     f(C.byref(C.c_int(E)), C.byref(C.c_int(offset)), p(ggeo), p(D), p(lam), p(q), p(Aq))
