@@ -161,14 +161,15 @@ struct AxDL {
 };
 
 // Thread groups of NQ*NQ lanes per workgroup: Epb elements at small orders;
-// for the three-field Helmholtz (non-BK) operator with 8 <= Nq <= 12, one
-// group per field of one element, reading the shared geometric factors and
-// lambda with cached loads; each field's arithmetic is the one-field
-// column's.  Measured on 32^3 N = 7: 0.805 -> 0.695 ms; the BK form is
-// faster with the field loop and keeps it.
-template <int NQ, int NF, bool BK>
+// for the three-field Helmholtz (non-BK) operator with 8 <= Nq <= 12 and
+// FP64 or FP32 geometry, one group per field of one element, reading the
+// shared geometric factors and lambda with cached loads; each field's
+// arithmetic is the one-field column's.  Measured on 32^3 N = 7: 0.805 ->
+// 0.695 ms; the BK form and bf16 geometry are faster with the field loop and
+// keep it.
+template <int NQ, int NF, bool BK, typename G = double>
 struct AxGroups {
-  static constexpr bool fields = NF == 3 && !BK && Epb<NQ>::value == 1 && NQ <= 12;
+  static constexpr bool fields = NF == 3 && !BK && Epb<NQ>::value == 1 && NQ <= 12 && sizeof(G) != 2;
   static constexpr int value = fields ? NF : Epb<NQ>::value;
 };
 
@@ -187,7 +188,7 @@ struct AxMinWaves {
 };
 
 template <typename T, int NQ, int NF, bool BK, bool PARTIAL, typename G = T>
-__global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWaves<T, NQ, NF, BK>::value)) void k_axhelm(const int32_t Nelements, const int32_t offset,
+__global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK, G>::value), (AxMinWaves<T, NQ, NF, BK>::value)) void k_axhelm(const int32_t Nelements, const int32_t offset,
                                                                   const int32_t* __restrict__ elementList,
                                                                   const G* __restrict__ ggeo,
                                                                   const T* __restrict__ D,
@@ -198,8 +199,8 @@ __global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWave
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int EPB = Epb<NQ>::value;
-  constexpr bool FG = AxGroups<NQ, NF, BK>::fields;
-  constexpr int GR = AxGroups<NQ, NF, BK>::value;
+  constexpr bool FG = AxGroups<NQ, NF, BK, G>::fields;
+  constexpr int GR = AxGroups<NQ, NF, BK, G>::value;
   if (skip && *skip) return;
   __shared__ T s_D[NQ2], s_q[GR][NQ2], s_Gr[GR][NQ2], s_Gs[GR][NQ2];
   const int slot = GR > 1 ? (int)threadIdx.y : 0;  // element slot, or field when FG
@@ -247,7 +248,7 @@ __global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWave
 // halving tree over the next power of two (zero padded); the element
 // partials go to partials[e].
 template <int NQ, int NF, bool BK, typename T, typename G>
-__global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWaves<T, NQ, NF, BK>::value)) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
+__global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK, G>::value), (AxMinWaves<T, NQ, NF, BK>::value)) void k_axhelm_pcg(const int32_t Nelements, const int32_t offset,
                                                                       const G* __restrict__ ggeo,
                                                                       const T* __restrict__ D,
                                                                       const G* __restrict__ lambda,
@@ -260,8 +261,8 @@ __global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK>::value), (AxMinWave
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int EPB = Epb<NQ>::value;
-  constexpr bool FG = AxGroups<NQ, NF, BK>::fields;
-  constexpr int GR = AxGroups<NQ, NF, BK>::value;
+  constexpr bool FG = AxGroups<NQ, NF, BK, G>::fields;
+  constexpr int GR = AxGroups<NQ, NF, BK, G>::value;
   constexpr int P2 = NQ2 <= 1 ? 1 : (NQ2 <= 2 ? 2 : (NQ2 <= 4 ? 4 : (NQ2 <= 8 ? 8 : (NQ2 <= 16 ? 16 : (NQ2 <= 32 ? 32 : (NQ2 <= 64 ? 64 : (NQ2 <= 128 ? 128 : 256)))))));
   if (S->done) return;
   const double beta = S->beta;
@@ -378,7 +379,7 @@ static int launch_axhelm_t(int32_t Nelements, int32_t offset, const int32_t* ele
                            const int* skip, hipStream_t s)
 {
   constexpr int EPB = Epb<NQ>::value;
-  constexpr int GR = AxGroups<NQ, NF, BK>::value;
+  constexpr int GR = AxGroups<NQ, NF, BK, G>::value;
   hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL, G>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, GR), 0,
                      s, Nelements, offset, elementList, (const G*)ggeo, (const T*)D, (const G*)lambda,
                      (const T*)q, (T*)Aq, skip);
@@ -483,11 +484,11 @@ static int axhelm_pcg_launch_t(int N, int Nf, int bk, int32_t Nelements, int32_t
                          0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
                          partials, S);                                                                       \
     else if (bk)                                                                                             \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), AxGroups<NV + 1, 3, true>::value), 0, \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, true, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), AxGroups<NV + 1, 3, true, G>::value), 0, \
                          s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,       \
                          partials, S);                                                                       \
     else                                                                                                     \
-      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), AxGroups<NV + 1, 3, false>::value),   \
+      hipLaunchKernelGGL((k_axhelm_pcg<NV + 1, 3, false, T, G>), dim3((Nelements + Epb<NV + 1>::value - 1) / Epb<NV + 1>::value), dim3((NV + 1) * (NV + 1), AxGroups<NV + 1, 3, false, G>::value),   \
                          0, s, Nelements, offset, (const G*)ggeo, (const T*)D, (const G*)lambda, z, p, Ap,    \
                          partials, S);                                                                       \
     break;

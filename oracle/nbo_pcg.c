@@ -74,12 +74,12 @@ double nbo_bp_wdot(const nbo_bp* bp, const double* x, const double* y)
 
 /* Element shares of pAp = sum_local p (A_L p) in the library's fused order:
  * lane (i,j) runs an fma chain over fields then k -- for three fields of the
- * non-BK operator with 8 <= Nq <= 12 (one thread group per field) a chain over
- * k per field and then
+ * non-BK operator with 8 <= Nq <= 12 and FP64/FP32 geometry (one thread group
+ * per field) a chain over k per field and then
  * (field0 + field1) + field2; the Nq*Nq lanes are summed by a halving tree
  * over the next power of two (zero padded); the element shares are combined
  * like tile partials. */
-static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
+static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap, int bf16_geometry)
 {
   const int Nq = bp->N + 1, Nq2 = Nq * Nq, Np = Nq2 * Nq;
   int P2 = 1;
@@ -89,7 +89,7 @@ static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
   for (int32_t e = 0; e < bp->E; ++e) {
     double lanes[256];
     for (int t = 0; t < P2; ++t) lanes[t] = 0.0;
-    const int field_groups = bp->Nf == 3 && !bp->bk && Nq2 >= 64 && Nq <= 12;
+    const int field_groups = bp->Nf == 3 && !bp->bk && Nq2 >= 64 && Nq <= 12 && !bf16_geometry;
     for (int t = 0; t < Nq2; ++t) {
       double acc = 0.0, accf[3] = {0.0, 0.0, 0.0};
       for (int f = 0; f < bp->Nf; ++f)
@@ -113,7 +113,7 @@ static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap)
 
 /* operands of a low-precision operator: float copies (prec 1) or bf16
  * values held in float (prec 2, round-to-nearest-even of the float) */
-typedef struct { float *g, *D, *lam; } lo_ops;
+typedef struct { float *g, *D, *lam; int bf16; } lo_ops;
 
 static float to_bf16f(double v)
 {
@@ -127,6 +127,7 @@ static void lo_make(const nbo_bp* bp, int prec, lo_ops* lo)
 {
   const int Nq = bp->N + 1, Np = Nq * Nq * Nq;
   const int64_t ng = (int64_t)bp->E * 7 * Np, nl = 2 * (int64_t)bp->offset;
+  lo->bf16 = prec == 2;
   lo->g = (float*)malloc(sizeof(float) * ng);
   lo->lam = (float*)malloc(sizeof(float) * nl);
   lo->D = (float*)malloc(sizeof(float) * Nq * Nq);
@@ -254,7 +255,7 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
     if (fused || lo || ln) {  /* pAp from the local operator output, then gs + mask */
       if (lo) lo_axhelm(bp, lo, p, Ap);
       else nbo_axhelm(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, p, Ap);
-      pAp = fused_pap(bp, p, Ap);
+      pAp = fused_pap(bp, p, Ap, lo && lo->bf16);
       nbo_gs_many(bp->ogs, Ap, bp->Nf, bp->offset, NBO_DOUBLE, NBO_ADD);
       for (int f = 0; f < bp->Nf; ++f)
         for (int32_t m = 0; m < bp->Nmasked; ++m) Ap[bp->maskIds[m] + (int64_t)f * bp->offset] = 0.0;

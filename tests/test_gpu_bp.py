@@ -23,6 +23,9 @@ CASES = [
     (4, (2, 2, 2), 3, True, 0.0, 0, False, 0, 0),      # BP6 (3 fields)
     (3, (3, 4, 3), 3, False, 0.3, 0, True, 0, 0),      # vector Helmholtz, flexible
     (5, (3, 3, 3), 1, False, 1.1, 1, False, 1, 1),     # periodic manufactured (reference nekBone)
+    (7, (2, 2, 2), 3, False, 0.3, 1, True, 0, 0),      # vector Helmholtz N=7: one thread group per field
+    (9, (2, 2, 1), 3, False, 0.2, 0, False, 0, 0),     # vector Helmholtz N=9: field groups, D from LDS
+    (7, (2, 2, 2), 3, True, 0.0, 0, False, 0, 0),      # BP6 N=7: field loop
 ]
 
 

@@ -27,7 +27,9 @@ def make(nbx, nbo, N, dims, Nf=1, bpmode=True, l1=0.0, precon=1, flexible=False)
 @pytest.mark.parametrize("prec", ["f32", "bf16", "f64"])
 @pytest.mark.parametrize("case", [(3, (3, 3, 2), 1, True, 0.0, 1, False),
                                   (7, (2, 2, 2), 1, False, 0.5, 1, False),
-                                  (4, (2, 2, 2), 3, False, 0.1, 0, True)], ids=["poisson", "helmholtz", "bp6flex"])
+                                  (4, (2, 2, 2), 3, False, 0.1, 0, True),
+                                  (7, (2, 2, 2), 3, False, 0.2, 1, False)],
+                         ids=["poisson", "helmholtz", "bp6flex", "vector_n7"])
 def test_mixed_matches_oracle(nbx, nbo, dev, prec, case):
     import torch
     N, dims, Nf, bpmode, l1, precon, flex = case
