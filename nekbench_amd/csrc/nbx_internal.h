@@ -11,6 +11,18 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// true when p is device (or managed) memory the GPU may dereference; host
+// memory that was never registered with HIP is refused before any launch
+inline bool device_accessible(const void* p)
+{
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();  // clear the sticky "invalid value" of an unknown pointer
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
 constexpr int kGWJ = 0, kG00 = 1, kG01 = 2, kG02 = 3, kG11 = 4, kG12 = 5, kG22 = 6, kNggeo = 7;
 
 // reduction tile: 256 lanes x 8 values (see nbx_blas.hip)

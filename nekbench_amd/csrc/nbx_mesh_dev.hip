@@ -282,6 +282,8 @@ extern "C" int nbx_box_mesh_device(const nbx_box* box, int mask_boundary, double
   const int64_t Np = (int64_t)(b.N + 1) * (b.N + 1) * (b.N + 1);
   if ((int64_t)b.NX * b.NY * b.NZ * Np > INT32_MAX)
     return nbx::fail(NBX_ERR_INVALID, "box_mesh_device: E*Np exceeds dlong range");
+  if ((ggeo && !nbx::device_accessible(ggeo)) || (globalIds && !nbx::device_accessible(globalIds)))
+    return nbx::fail(NBX_ERR_INVALID, "box_mesh_device: outputs must be device memory");
   hipStream_t s = nbx::as_stream(stream);
   int rc;
   if (ggeo && (rc = nbx::box_geometry_launch(b, ggeo, nullptr, nullptr, nullptr, s))) return rc;

@@ -665,6 +665,10 @@ extern "C" nbx_ogs* nbx_ogs_setup(nbx_dlong N, const nbx_hlong* ids, int verbose
 
 extern "C" nbx_ogs* nbx_ogs_setup_device(nbx_dlong N, const nbx_hlong* ids, int verbose)
 {
+  if (N > 0 && ids && !nbx::device_accessible(ids)) {
+    fail(NBX_ERR_INVALID, "ogs_setup_device: ids must be device memory");
+    return nullptr;
+  }
   return ogs_setup_common(N, ids, verbose, true);
 }
 

@@ -53,6 +53,10 @@ def test_box_mesh_device_arguments(torch):
     assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(7, 2, 2, 2, periodic=1)), 0, g.data_ptr(), None, None) == -1
     assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(16, 1, 1, 1)), 0, g.data_ptr(), None, None) == -1
     assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(3, 2, 2, 2)), 0, None, None, None) == 0
+    host = np.zeros(8 * 7 * 64)
+    assert L.nbx_box_mesh_device(C.byref(nbx.Box.make(3, 2, 2, 2)), 0, host.ctypes.data, None, None) == -1
+    ids_h = np.ones(64, dtype=np.int64)
+    assert not L.nbx_ogs_setup_device(64, ids_h.ctypes.data, 0)
 
 
 @pytest.mark.parametrize("periodic", [0, 1])
