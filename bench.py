@@ -608,7 +608,11 @@ def main():
                                f"tol {args.tol:g} or {args.maxit} iterations", "elements": E, "N": N,
                    "iterations": its, "converged": bool(st.converged),
                    "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
-                   "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3), "axhelm_variants": variants,
+                   "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3),
+                   # axhelm/main.cpp:165-168: (12 Nq + 15) flop per node (BK); FP64 vector peak 78.6 TFLOP/s
+                   "axhelm_fp64_tflops": round(E * bp.Np * (12 * (N + 1) + 15) / ax_s / 1e12, 2),
+                   "axhelm_fp64_frac_of_78p6": round(E * bp.Np * (12 * (N + 1) + 15) / ax_s / 78.6e12, 4),
+                   "axhelm_variants": variants,
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3),
                    "time_to_1e-8": compare, "config1_16cube": config1},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
