@@ -328,16 +328,24 @@ constexpr int kOgsBlock = 256;
 
 inline dim3 ogs_grid(int64_t n) { return dim3((unsigned)((n + kOgsBlock - 1) / kOgsBlock)); }
 
-// largest |id| and number of nonzero ids (order-independent totals)
+// largest |id| and number of nonzero ids (order-independent totals): each
+// wave reduces with shuffles, one atomic pair per wave
 __global__ void k_ogs_range(int32_t N, const int64_t* __restrict__ ids, unsigned long long* __restrict__ amax,
                             int* __restrict__ count)
 {
   const int64_t i = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
-  if (i >= N) return;
-  const int64_t v = ids[i];
-  if (v == 0) return;
-  atomicMax(amax, (unsigned long long)(v < 0 ? -v : v));
-  atomicAdd(count, 1);
+  const int64_t v = i < N ? ids[i] : 0;
+  unsigned long long a = (unsigned long long)(v < 0 ? -v : v);
+  int c = v != 0 ? 1 : 0;
+  for (int h = 32; h >= 1; h >>= 1) {
+    const unsigned long long ao = __shfl_xor(a, h, 64);
+    a = ao > a ? ao : a;
+    c += __shfl_xor(c, h, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && c > 0) {
+    atomicMax(amax, a);
+    atomicAdd(count, c);
+  }
 }
 
 __global__ void k_ogs_keys(int32_t N, const int64_t* __restrict__ ids, uint64_t sentinel, uint64_t* __restrict__ keys,
