@@ -115,7 +115,7 @@ def timed(fn, sync):
     return st, time.perf_counter() - t0
 
 
-def run_config3(nbx, torch, args):
+def run_bp6_n9_24cube(nbx, torch, args):
     """BP6 vector Helmholtz, 3 fields, lambda0 = 1, lambda1 = 0.1 (axhelm_n3),
     N = 9, 24^3 elements, perturbed Dirichlet box; FP64 flexible Jacobi PCG,
     the FP32 / bf16-geometry iterative-refinement solvers and the multigrid
@@ -124,7 +124,7 @@ def run_config3(nbx, torch, args):
     box = nbx.Box.make(9, 24, 24, 24, perturb=0.1, seed=1)
     t0 = time.perf_counter()
     bp = nbx.BP(box, Nfields=3, bpmode=False, lambda1=0.1, precon=nbx.PRECON_JACOBI, flexible=True, fused=True)
-    log(f"[bench3] setup {time.perf_counter() - t0:.1f} s E={bp.Nelements} Np={bp.Np}")
+    log(f"[bp6] setup {time.perf_counter() - t0:.1f} s E={bp.Nelements} Np={bp.Np}")
     n = 3 * bp.fieldOffset
     b = bp.device_array("rhs", n).clone()
     r, x = torch.empty_like(b), torch.empty_like(b)
@@ -158,7 +158,7 @@ def run_config3(nbx, torch, args):
         else:
             rec["err_vs_fp64_per_field"] = field_errors(x, x64, 3, bp.fieldOffset, bp.Nlocal)
         out[name] = rec
-        log(f"[bench3] {name}: {rec}")
+        log(f"[bp6] {name}: {rec}")
     for v in out.values():
         v["speedup_vs_fp64_jacobi"] = round(out["fp64_jacobi"]["time_s"] / v["time_s"], 3)
     return {"config": "BP6 vector Helmholtz N=9, 24^3 elements, 3 fields, lambda1=0.1, tol 1e-10",
@@ -227,7 +227,7 @@ def train_net(nbx, torch, bp, nseeds, snapshots, steps, lr, log, seed0=1000, mod
 SNAPS_2000 = [0, 1, 2, 4, 6, 9, 13, 18, 25, 35, 50, 70, 95, 130, 175, 230, 300, 380, 470, 560]
 
 
-def run_config4(nbx, torch, args):
+def run_learned_16cube(nbx, torch, args):
     """Learned element preconditioner on the 16^3, N=7 perturbed box: 2000
     (residual, error) pairs from 100 seeded FP64 Jacobi PCG solves (seeds
     1000..1099, 20 snapshots each), energy-norm training with model selection
@@ -260,7 +260,7 @@ def run_config4(nbx, torch, args):
             rec[name] = {"iterations": st.iterations, "time_s": round(t, 5), "converged": bool(st.converged),
                          "fallback_iteration": st.fallback_iteration}
         rows.append(rec)
-        log(f"[bench4] {rec}")
+        log(f"[learned] {rec}")
     def avg(k, f):
         return float(sum(row[k][f] for row in rows) / len(rows))
     summary = {k: {"iterations": avg(k, "iterations"), "time_s": avg(k, "time_s")}
@@ -269,7 +269,7 @@ def run_config4(nbx, torch, args):
             "training": info, "validation": rows, "mean": summary}
 
 
-def run_config5(nbx, torch, args):
+def run_graded_64x64x32(nbx, torch, args):
     """Stress case: 64x64x32 elements N=7 (~45M unknowns), z graded to
     element aspect 1:10 at the bottom, +-0.1 perturbation, tol 1e-10: FP64
     Jacobi PCG, the multigrid PCG with bf16-geometry V-cycle, and the learned
@@ -279,7 +279,7 @@ def run_config5(nbx, torch, args):
     sync = torch.cuda.synchronize
     t0 = time.perf_counter()
     bj = nbx.BP(box, bpmode=True, precon=nbx.PRECON_JACOBI, fused=True)
-    log(f"[bench5] setup {time.perf_counter() - t0:.1f} s E={bj.Nelements}")
+    log(f"[graded] setup {time.perf_counter() - t0:.1f} s E={bj.Nelements}")
     n = bj.fieldOffset
     b = bj.device_array("rhs", n).clone()
     r, x = torch.empty_like(b), torch.empty_like(b)
@@ -294,10 +294,10 @@ def run_config5(nbx, torch, args):
         if "fp64_jacobi" in out:
             rec["err_vs_fp64"] = float((x - x64[0]).abs().max() / x64[0].abs().max())
         out[name] = rec
-        log(f"[bench5] {name}: {rec}")
+        log(f"[graded] {name}: {rec}")
 
     x64 = []
-    run("fp64_jacobi", lambda: bj.pcg(r, x, 1e-10, args.maxit5))
+    run("fp64_jacobi", lambda: bj.pcg(r, x, 1e-10, args.maxit_graded))
     x64.append(x.clone())
     bj.free()
     bm = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
@@ -310,7 +310,7 @@ def run_config5(nbx, torch, args):
                           mode=args.learn_mode, select_seed=8999)
     bl = nbx.BP(box, bpmode=True, precon=nbx.PRECON_LEARNED)
     net.attach(bl)
-    run("learned_mixed_f32", lambda: bl.pcg_mixed(r, x, 1e-10, args.maxit5, prec=nbx.NBX_F32, inner_tol=1e-5,
+    run("learned_mixed_f32", lambda: bl.pcg_mixed(r, x, 1e-10, args.maxit_graded, prec=nbx.NBX_F32, inner_tol=1e-5,
                                                   max_cycles=20))
     for v in out.values():
         v["speedup_vs_fp64_jacobi"] = round(out["fp64_jacobi"]["time_s"] / v["time_s"], 3)
@@ -362,8 +362,8 @@ def cpu_reference_axhelm(bj, E, n, g0, D, log):
     return res
 
 
-def config1_measurements(nbx, torch, args, log, axhelm_only=False):
-    """Config 1 mesh (16^3 elements, N=7, perturbed, seed 1): FP64 BK axhelm
+def mesh16_measurements(nbx, torch, args, log, axhelm_only=False):
+    """The 16^3-element mesh (N=7, perturbed, seed 1): FP64 BK axhelm
     with 5 rotating buffer sets of 151 MB each (755 MB, beyond the 256 MB
     Infinity Cache), HIP events on the launch stream; and time to a 1e-8
     relative residual of FP64 Jacobi PCG against the multigrid solvers on the
@@ -396,7 +396,7 @@ def config1_measurements(nbx, torch, args, log, axhelm_only=False):
     if not args.no_cpu:
         out["cpu_reference_axhelm"] = cpu_reference_axhelm(bj, E, n, g0, D, log)
     del sets
-    log(f"[bench] config 1 axhelm: {out}")
+    log(f"[bench] 16^3 axhelm: {out}")
     if axhelm_only:
         return out
     b = bj.device_array("rhs", n).clone()
@@ -421,7 +421,7 @@ def config1_measurements(nbx, torch, args, log, axhelm_only=False):
     for v in res.values():
         v["speedup_vs_fp64_jacobi"] = round(res["fp64_jacobi"]["time_s"] / v["time_s"], 3)
     out["time_to_1e-8"] = res
-    log(f"[bench] config 1 time to 1e-8: {res}")
+    log(f"[bench] 16^3 time to 1e-8: {res}")
     return out
 
 
@@ -465,14 +465,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the time-to-1e-8 comparison of the FP64, mixed and learned solvers")
-    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
-                    help="2: the PCG benchmark line (default); 1: the 16^3 FP64 axhelm alone (rotating buffer "
-                         "sets); 3: BP6 N=9 mixed precision; 4: learned preconditioner on 16^3; 5: 64x64x32 "
-                         "graded stress case (1, 3-5 print one JSON line each)")
+    ap.add_argument("--study", default="pcg32", choices=("pcg32", "axhelm16", "bp6", "learned16", "graded"),
+                    help="pcg32: the PCG benchmark line on 32^3 (default); axhelm16: the 16^3 FP64 axhelm alone "
+                         "(rotating buffer sets); bp6: BP6 N=9 24^3 mixed precision; learned16: learned "
+                         "preconditioner on 16^3; graded: 64x64x32 graded stress case (all but pcg32 print "
+                         "one JSON line of their own)")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--learn-mode", type=int, default=1, choices=(0, 1),
                     help="energy-norm loss: 0 fixed scale, 1 scale-invariant")
-    ap.add_argument("--maxit5", type=int, default=30000)
+    ap.add_argument("--maxit-graded", type=int, default=30000)
     ap.add_argument("--profile-mg", action="store_true",
                     help="short sequence for a rocprofv3 kernel trace: two multigrid PCG solves (bf16-geometry "
                          "V-cycle, degree 3, 20 coarse iterations) to 1e-8 and five network applies; prints no JSON")
@@ -491,12 +492,13 @@ def main():
     if args.profile_mg:
         profile_mg_pass(nbx, torch, args.nx)
         return
-    if args.config != 2:
+    if args.study != "pcg32":
         torch.cuda.set_device(0)
-        if args.config == 1:
-            res = config1_measurements(nbx, torch, args, log, axhelm_only=True)
+        if args.study == "axhelm16":
+            res = mesh16_measurements(nbx, torch, args, log, axhelm_only=True)
         else:
-            res = {3: run_config3, 4: run_config4, 5: run_config5}[args.config](nbx, torch, args)
+            res = {"bp6": run_bp6_n9_24cube, "learned16": run_learned_16cube,
+                   "graded": run_graded_64x64x32}[args.study](nbx, torch, args)
         print(json.dumps(res), flush=True)
         return
 
@@ -590,7 +592,7 @@ def main():
     log(f"[bench] axhelm variants: {variants}")
     del g32, g16, q32, A32
 
-    config1 = None if args.no_compare else config1_measurements(nbx, torch, args, log)
+    mesh16 = None if args.no_compare else mesh16_measurements(nbx, torch, args, log)
 
     compare = None
     if not args.no_compare:
@@ -620,7 +622,7 @@ def main():
                    "axhelm_fp64_frac_of_78p6": round(E * bp.Np * (12 * (N + 1) + 15) / ax_s / 78.6e12, 4),
                    "axhelm_variants": variants,
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3),
-                   "time_to_1e-8": compare, "config1_16cube": config1},
+                   "time_to_1e-8": compare, "mesh16": mesh16},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_axhelm<double,8,1,BK>", "bytes_per_launch": ax_bytes,
