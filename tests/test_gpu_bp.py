@@ -128,7 +128,7 @@ def test_manufactured_solution_error(nbx, dev):
     assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4, errs
 
 
-def test_config2_full_size_properties(nbx, dev):
+def test_32cube_full_size_properties(nbx, dev):
     """32^3 elements, N=7, Dirichlet, perturbed: Jacobi PCG to 1e-8 or 500
     iterations; the final residual is checked by recomputing r = b - A x."""
     import torch
