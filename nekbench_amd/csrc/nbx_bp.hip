@@ -831,11 +831,14 @@ extern "C" int nbx_bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out)
   return NBX_SUCCESS;
 }
 
+// drawn on the device (the same splitmix64 values as nbx_urand_fill), then
+// assembled and masked: masked nodes belong to no group, so this equals the
+// setup's mask-then-assemble order
 int bp_make_rhs(nbx_bp* bp, uint64_t seed, double* out)
 {
-  std::vector<double> r(bp->Nall);
-  for (int f = 0; f < bp->Nf; ++f) nbx_urand_fill(seed, 3 + f, bp->Nlocal, r.data() + (int64_t)f * bp->fieldOffset);
-  NBX_HIP(hipMemcpyAsync(out, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, bp->stream));
-  NBX_HIP(hipStreamSynchronize(bp->stream));
+  for (int f = 0; f < bp->Nf; ++f) {
+    const int rc = urand_fill_launch(seed, 3 + f, bp->Nlocal, out + (int64_t)f * bp->fieldOffset, bp->stream);
+    if (rc) return rc;
+  }
   return gs_add_mask_launch(bp->ogs, out, bp->Nf, bp->fieldOffset, bp->Nmasked, bp->maskIds, nullptr, bp->stream);
 }

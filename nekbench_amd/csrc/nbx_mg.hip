@@ -287,14 +287,9 @@ int mg_alloc(nbx_mg* mg, P** p, int64_t n, size_t elem = sizeof(P))
 int power_lmax(nbx_bp* bp, uint64_t seed, uint64_t stream_id, int iters, double* v, double* w, double* out)
 {
   const int64_t n = bp->Nlocal;
-  std::vector<double> h(n);
-  int rc = nbx_urand_fill(seed, stream_id, n, h.data());
+  int rc = urand_fill_launch(seed, stream_id, n, v, bp->stream);
   if (rc) return rc;
-  std::vector<int32_t> mids(bp->Nmasked);
-  if (bp->Nmasked)
-    NBX_HIP(hipMemcpy(mids.data(), bp->maskIds, sizeof(int32_t) * bp->Nmasked, hipMemcpyDeviceToHost));
-  for (int32_t m : mids) h[m] = 0.0;
-  NBX_HIP(hipMemcpyAsync(v, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, bp->stream));
+  if ((rc = mask_launch(bp->Nmasked, bp->maskIds, v, nullptr, bp->stream))) return rc;
   if ((rc = gs_add_mask_launch(bp->ogs, v, 1, bp->fieldOffset, 0, nullptr, nullptr, bp->stream))) return rc;
   const double* wt = nbx_ogs_inv_degree(bp->ogs);
   double nrm = 0.0;
