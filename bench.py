@@ -578,7 +578,7 @@ def main():
     g16 = ggeo.to(torch.bfloat16).view(torch.int16)
     A32 = torch.empty_like(q32)
     for name, prec, g, bpn in (("f32", nbx.NBX_F32, g32, 36), ("bf16_geometry", nbx.NBX_BF16, g16, 22),
-                               ("bf16_mfma", nbx.NBX_BF16_MFMA, g16, 22)):
+                               ("bf16_mfma", nbx.NBX_BF16_MFMA, g16, 22), ("f16_mfma", nbx.NBX_F16, g32, 36)):
         for _ in range(3):
             nbx.axhelm(N, 1, 1, E, n, g, D32, None, q32, A32, prec=prec, stream=s)
         ev0.record(s)

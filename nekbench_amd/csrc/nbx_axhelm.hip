@@ -461,7 +461,10 @@ int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32
                                          skip, s);
     case NBX_BF16_MFMA:  // as NBX_BF16 with the contractions on the matrix cores
       if (partial) return fail(NBX_ERR_UNSUPPORTED, "axhelm bf16-MFMA: no element list");
-      return axhelm_mma_launch(N, Nf, bk, true, Nelements, offset, ggeo, D, lambda, q, Aq, skip, s);
+      return axhelm_mma_launch(N, Nf, bk, true, false, Nelements, offset, ggeo, D, lambda, q, Aq, skip, s);
+    case NBX_F16:  // fp16 matrix-core operands, FP32 geometry, vectors and accumulation
+      if (partial) return fail(NBX_ERR_UNSUPPORTED, "axhelm fp16-MFMA: no element list");
+      return axhelm_mma_launch(N, Nf, bk, false, true, Nelements, offset, ggeo, D, lambda, q, Aq, skip, s);
     default:
       return fail(NBX_ERR_UNSUPPORTED, "axhelm: precision %d not supported by nbx_axhelm", (int)prec);
   }

@@ -47,10 +47,11 @@ struct PcgFlags {
 
 // launchers shared between translation units; `skip` (device int, may be
 // null) turns the launch into a no-op when *skip != 0
-// bf16-MFMA axhelm (nbx_axhelm_mma.hip): N = 7, one field, FP32 vectors
-int axhelm_mma_launch(int N, int Nf, int bk, bool bf16_geometry, int32_t Nelements, int32_t offset,
-                      const void* ggeo, const void* D, const void* lambda, const void* q, void* Aq, const int* skip,
-                      hipStream_t s);
+// matrix-core axhelm (nbx_axhelm_mma.hip): N = 7, one field, FP32 vectors;
+// bf16 operands with bf16 or FP32 geometry, or fp16 operands with FP32 geometry
+int axhelm_mma_launch(int N, int Nf, int bk, bool bf16_geometry, bool f16_operands, int32_t Nelements,
+                      int32_t offset, const void* ggeo, const void* D, const void* lambda, const void* q, void* Aq,
+                      const int* skip, hipStream_t s);
 int axhelm_launch(int N, int Nf, int bk, nbx_prec prec, int32_t Nelements, int32_t offset,
                   const int32_t* elementList, const void* ggeo, const void* D, const void* lambda,
                   const void* q, void* Aq, const int* skip, hipStream_t s);

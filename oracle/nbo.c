@@ -691,13 +691,14 @@ double nbo_tree_dot(int64_t n, const double* x, const double* y)
 #include <omp.h>
 int nbo_max_threads(void) { return omp_get_max_threads(); }
 
-/* axhelm with bf16 matrix-core contractions (library: k_axhelm_mma,
+/* axhelm with matrix-core contractions (library: k_axhelm_mma,
  * nbx_axhelm_mma.hip), N = 7, one field: q, D and the G-scaled gradients are
- * rounded to bf16 (nearest even); the 8-term contractions are accumulated in
- * double and rounded to float (the products are exact in FP32; the matrix
- * cores' internal summation order is not specified, so agreement is to FP32
- * rounding, not bit for bit); geometric factors applied in FP32 with the
- * FP32 kernel's fma order; direction results summed r, then s, then t. */
+ * rounded to the operand type (bf16 or fp16, nearest even); the 8-term
+ * contractions are accumulated in double and rounded to float (the products
+ * are exact in FP32; the matrix cores' internal summation order is not
+ * specified, so agreement is to FP32 rounding, not bit for bit); geometric
+ * factors applied in FP32 with the FP32 kernel's fma order; direction
+ * results summed r, then s, then t. */
 static float bf16r(float x)
 {
   union { float f; uint32_t u; } a;
@@ -706,12 +707,47 @@ static float bf16r(float x)
   return a.f;
 }
 
+/* IEEE binary16 rounding (nearest even) of a float, returned as a float:
+ * 10 fraction bits, normal range from 2^-14, subnormal quantum 2^-24,
+ * overflow to infinity from 65520 (half-way between 65504 and 2^16).  Every
+ * scaling is by a power of two and nearbyintf rounds to nearest even, so
+ * each step is exact. */
+static float f16r(float x)
+{
+  const float ax = fabsf(x);
+  if (!(ax < 65520.0f)) return isnan(x) ? x : copysignf(INFINITY, x);
+  if (ax < 0x1p-14f) return nearbyintf(x * 0x1p24f) * 0x1p-24f;
+  int ex;
+  frexpf(x, &ex); /* |x| in [2^(ex-1), 2^ex): quantum 2^(ex-11) */
+  return ldexpf(nearbyintf(ldexpf(x, 11 - ex)), ex - 11);
+}
+
+void nbo_f16_round_array(int64_t n, const float* x, float* y)
+{
+  for (int64_t i = 0; i < n; ++i) y[i] = f16r(x[i]);
+}
+
+static void axhelm_mma_impl(float (*h)(float), int bk, int32_t E, int32_t offset, const float* ggeo, const float* D,
+                            const float* lambda, const float* q, float* Aq);
+
+void nbo_axhelm_f16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
+                       const float* q, float* Aq)
+{
+  axhelm_mma_impl(f16r, bk, E, offset, ggeo, D, lambda, q, Aq);
+}
+
 void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
                         const float* q, float* Aq)
 {
+  axhelm_mma_impl(bf16r, bk, E, offset, ggeo, D, lambda, q, Aq);
+}
+
+static void axhelm_mma_impl(float (*h)(float), int bk, int32_t E, int32_t offset, const float* ggeo, const float* D,
+                            const float* lambda, const float* q, float* Aq)
+{
   enum { Q = 8, P = 512 };
   float Db[Q * Q];
-  for (int i = 0; i < Q * Q; ++i) Db[i] = bf16r(D[i]);
+  for (int i = 0; i < Q * Q; ++i) Db[i] = h(D[i]);
 #pragma omp parallel
   {
     float qb[P], qr[P], qs[P], qt[P], gr[P], gs[P], gt[P];
@@ -719,7 +755,7 @@ void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, co
     for (int32_t e = 0; e < E; ++e) {
       const float* qe = q + (int64_t)e * P;
       const float* g = ggeo + (int64_t)e * 7 * P;
-      for (int n = 0; n < P; ++n) qb[n] = bf16r(qe[n]);
+      for (int n = 0; n < P; ++n) qb[n] = h(qe[n]);
       for (int n = 0; n < P; ++n) {
         const int i = n % Q, j = (n / Q) % Q, k = n / (Q * Q);
         double ar = 0, as = 0, at = 0;
@@ -737,7 +773,7 @@ void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, co
         float Gs = fmaf(G12, qt[n], fmaf(G11, qs[n], G01 * qr[n]));
         float Gt = fmaf(G22, qt[n], fmaf(G12, qs[n], G02 * qr[n]));
         if (!bk) { const float l0 = lambda[(int64_t)e * P + n]; Gr = l0 * Gr; Gs = l0 * Gs; Gt = l0 * Gt; }
-        gr[n] = bf16r(Gr); gs[n] = bf16r(Gs); gt[n] = bf16r(Gt);
+        gr[n] = h(Gr); gs[n] = h(Gs); gt[n] = h(Gt);
       }
       for (int n = 0; n < P; ++n) {
         const int i = n % Q, j = (n / Q) % Q, k = n / (Q * Q);

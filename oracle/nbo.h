@@ -80,9 +80,15 @@ int nbo_gs_many(const nbo_ogs* o, void* q, int k, int32_t stride, int type, int 
 void nbo_axhelm(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const int32_t* elementList,
                 const double* ggeo, const double* D, const double* lambda,
                 const double* q, double* Aq);
-/* single-precision restatement (same loop nest, float arithmetic) */
+/* matrix-core restatements, N = 7, one field: bf16 operands (ggeo and lambda
+ * hold bf16 values) or fp16 operands (FP32 ggeo and lambda) */
 void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
                         const float* q, float* Aq);
+void nbo_axhelm_f16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
+                       const float* q, float* Aq);
+/* IEEE binary16 rounding (nearest even, subnormals kept) of each value */
+void nbo_f16_round_array(int64_t n, const float* x, float* y);
+/* single-precision restatement (same loop nest, float arithmetic) */
 void nbo_axhelm_f32(int N, int Nf, int bk, int32_t Nelements, int32_t offset,
                     const float* ggeo, const float* D, const float* lambda,
                     const float* q, float* Aq);

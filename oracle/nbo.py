@@ -302,6 +302,21 @@ def axhelm_bf16mma(bk, E, offset, ggeo, D, lam, q):
     return Aq
 
 
+def axhelm_f16mma(bk, E, offset, ggeo, D, lam, q):
+    """float32 arrays (FP32 ggeo and lam); fp16 matrix-core operands"""
+    Aq = np.zeros_like(q)
+    lib().nbo_axhelm_f16mma(int(bk), C.c_int32(E), C.c_int32(offset), p(ggeo), p(D), p(lam), p(q), p(Aq))
+    return Aq
+
+
+def f16_round(x):
+    """float32 -> nearest-even IEEE binary16 value (as float32), by the C restatement."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib().nbo_f16_round_array(C.c_int64(x.size), p(x), p(y))
+    return y
+
+
 def sumsq_inv_f(a):
     f = lib().nbo_sumsq_inv_f
     f.restype = C.c_double

@@ -261,3 +261,48 @@ def test_axhelm_bf16_mfma_matches_oracle(nbx, nbo, dev, E, bk):
     assert np.abs(out - exp).max() <= 1e-5 * np.abs(exp).max()
     ref = nbo.axhelm(7, 1, bk, E, off, m["ggeo"], m["D"], lam, q)
     assert np.abs(out - ref).max() <= 3e-2 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("E", [5, 8])
+@pytest.mark.parametrize("bk", [1, 0])
+def test_axhelm_f16_mfma_matches_oracle(nbx, nbo, dev, E, bk):
+    """fp16 matrix-core axhelm (NBX_F16, N=7, FP32 geometry): equal to the
+    oracle to FP32 rounding of the 8-term contractions (the matrix cores'
+    summation order is not specified), within fp16 rounding of the FP64
+    operator; an odd element count exercises the idle second slot."""
+    import torch
+    box = nbo.Box.make(7, E, 1, 1, perturb=0.1, seed=22)
+    m = nbo.box_mesh(box)
+    off = E * 512
+    rng = np.random.default_rng(10 * E + bk)
+    q = rng.uniform(-1, 1, off)
+    lam = np.concatenate([rng.uniform(0.5, 1.5, off), rng.uniform(0.1, 0.5, off)])
+    g32, l32, D32, q32 = (a.astype(np.float32) for a in (m["ggeo"], lam, m["D"], q))
+    T = lambda a: torch.tensor(a, device=dev)
+    dq = T(q32)
+    dA = torch.zeros_like(dq)
+    nbx.axhelm(7, 1, bk, E, off, T(g32), T(D32), T(l32), dq, dA, prec=nbx.NBX_F16)
+    torch.cuda.synchronize()
+    out = dA.cpu().numpy()
+    exp = nbo.axhelm_f16mma(bk, E, off, g32, D32, l32, q32)
+    assert np.abs(out - exp).max() <= 1e-5 * np.abs(exp).max()
+    ref = nbo.axhelm(7, 1, bk, E, off, m["ggeo"], m["D"], lam, q)
+    assert np.abs(out - ref).max() <= 4e-3 * np.abs(ref).max()
+
+
+def test_axhelm_f16_mfma_limits(nbx, dev):
+    """NBX_F16 covers N = 7, one field, all elements: the other cases are
+    refused before any launch."""
+    import torch
+    E, off = 2, 2 * 512
+    g = torch.zeros(E * 7 * 512, device=dev)
+    D = torch.zeros(64, device=dev)
+    q = torch.zeros(3 * off, device=dev)
+    A = torch.zeros_like(q)
+    el = torch.arange(E, dtype=torch.int32, device=dev)
+    with pytest.raises(RuntimeError, match="fp16-MFMA"):
+        nbx.axhelm(5, 1, 1, E, off, g, D, None, q, A, prec=nbx.NBX_F16)
+    with pytest.raises(RuntimeError, match="fp16-MFMA"):
+        nbx.axhelm(7, 3, 1, E, off, g, D, None, q, A, prec=nbx.NBX_F16)
+    with pytest.raises(RuntimeError, match="fp16-MFMA"):
+        nbx.axhelm(7, 1, 1, E, off, g, D, None, q, A, elementList=el, prec=nbx.NBX_F16)

@@ -162,3 +162,46 @@ def test_bf16_mfma_oracle_tracks_fp64(nbo):
     one = nbo.axhelm_bf16mma(1, E, off, g32, D32, l32, np.ones(off, np.float32))
     ref1 = nbo.axhelm(7, 1, 1, E, off, m["ggeo"], m["D"], lam, q)
     assert np.abs(one).max() < 3e-2 * np.abs(ref1).max()
+
+
+def test_f16_rounding_matches_numpy(nbo):
+    """The oracle's IEEE binary16 rounding against numpy's float16 conversion:
+    random values over the whole range, exact ties between neighbouring fp16
+    values (normal and subnormal), the overflow edge and signed zeros."""
+    rng = np.random.default_rng(17)
+    x = (rng.uniform(-1, 1, 200000) * 2.0 ** rng.integers(-30, 18, 200000)).astype(np.float32)
+    h = np.arange(0, 0x7C00, dtype=np.uint16).view(np.float16).astype(np.float64)  # finite fp16 >= 0
+    mids = ((h[:-1] + h[1:]) / 2).astype(np.float32)  # exact in float32: ties
+    assert np.array_equal(mids.astype(np.float64), (h[:-1] + h[1:]) / 2)
+    edge = np.array([0.0, -0.0, 65504, 65519.996, 65520, 1e6, 2.0 ** -24, 2.0 ** -25, 2.0 ** -25 * 1.0001,
+                     2.0 ** -26, 2.0 ** -14, 2.0 ** -14 * (1 - 2.0 ** -12), 6e-8], dtype=np.float32)
+    for a in (x, mids, -mids, edge, -edge):
+        got = nbo.f16_round(a)
+        with np.errstate(over="ignore"):
+            exp = a.astype(np.float16).astype(np.float32)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+def test_f16_mfma_oracle_tracks_fp64(nbo):
+    """Matrix-core form with fp16 operands (FP32 geometry): within fp16
+    rounding of the FP64 operator for BK and non-BK, several times closer
+    than the bf16 form on the same data (3 more significand bits), and the
+    constant vector stays near the null space."""
+    box = nbo.Box.make(7, 2, 2, 1, perturb=0.1, seed=12)
+    m = nbo.box_mesh(box)
+    E, Np = box.E, 512
+    off = E * Np
+    rng = np.random.default_rng(4)
+    q = rng.uniform(-1, 1, off)
+    lam = np.concatenate([rng.uniform(0.5, 1.5, off), rng.uniform(0.1, 0.5, off)])
+    g32, l32, D32 = m["ggeo"].astype(np.float32), lam.astype(np.float32), m["D"].astype(np.float32)
+    for bk in (1, 0):
+        ref = nbo.axhelm(7, 1, bk, E, off, m["ggeo"], m["D"], lam, q)
+        out = nbo.axhelm_f16mma(bk, E, off, g32, D32, l32, q.astype(np.float32))
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        bf = nbo.axhelm_bf16mma(bk, E, off, nbo.bf16_round(g32), D32, nbo.bf16_round(l32), q.astype(np.float32))
+        err_bf = np.abs(bf - ref).max() / np.abs(ref).max()
+        assert 1e-6 < err < 4e-3 and err < err_bf / 4, (bk, err, err_bf)
+    one = nbo.axhelm_f16mma(1, E, off, g32, D32, l32, np.ones(off, np.float32))
+    ref1 = nbo.axhelm(7, 1, 1, E, off, m["ggeo"], m["D"], lam, q)
+    assert np.abs(one).max() < 4e-3 * np.abs(ref1).max()
