@@ -625,7 +625,7 @@ def main():
                    "time_to_1e-8": compare, "mesh16": mesh16},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_axhelm<double,8,1,BK>", "bytes_per_launch": ax_bytes,
+                     "kernel": "k_axhelm_bk8_split", "bytes_per_launch": ax_bytes,
                      "us_per_launch": round(ax_s * 1e6, 3)},
         "cpu_baseline": None,
     }

@@ -237,6 +237,83 @@ __global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK, G>::value), (AxMinW
   }
 }
 
+// FP64 BK one-field operator at N = 7 with each element split over two
+// waves: wave w owns the slabs k = 4w .. 4w+3.  The element's q sits in LDS
+// (4 KB), so the t derivative reads the column from there and no wave keeps
+// a q column in registers; each wave walks four slabs (half the chain of
+// dependent load rounds), and eight waves per SIMD (64 VGPRs) keep a 16^3
+// mesh (8192 waves) in one round.  After the slab loop, Gt of every slab
+// replaces q in LDS and each output node takes the t transpose as the
+// one-wave column does, an fma chain over the slabs in increasing order:
+// the arithmetic is the one-wave kernel's, bit for bit.
+__global__ __launch_bounds__(128, 8) void k_axhelm_bk8_split(const int32_t Nelements,
+                                                             const double* ggeo,
+                                                             const double* __restrict__ D,
+                                                             const double* __restrict__ q, double* __restrict__ Aq,
+                                                             const int* __restrict__ skip)
+{
+  constexpr int NQ = 8, NQ2 = 64, NP = 512, H = 4;
+  if (skip && *skip) return;
+  __shared__ double s_D[NQ2], s_qt[NP], s_Gr[2][2][NQ2], s_Gs[2][2][NQ2];
+  // w is uniform in each wave: D's row and column k come from scalar loads
+  const int t = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y), i = t & 7, j = t >> 3;
+  const int64_t e = blockIdx.x;
+  if (w == 0) s_D[t] = D[t];
+  // uniform element bases, one 32-bit lane offset: the loads take a scalar
+  // base and a shared lane register rather than a 64-bit address each
+  const double* qe = q + e * NP;
+  const int o = H * w * NQ2 + t;
+#pragma unroll
+  for (int kk = 0; kk < H; ++kk) s_qt[o + kk * NQ2] = ldnt(&qe[o + kk * NQ2]);
+  __syncthreads();
+  const double* g = ggeo + e * (int64_t)(kNggeo * NP);
+  const double *g00 = g + kG00 * NP, *g01 = g + kG01 * NP, *g02 = g + kG02 * NP;
+  const double *g11 = g + kG11 * NP, *g12 = g + kG12 * NP, *g22 = g + kG22 * NP;
+  double Gt[H], ars[H];
+#pragma unroll
+  for (int kk = 0; kk < H; ++kk) {
+    const int k = H * w + kk;
+    double qr = 0.0, qs = 0.0, qt = 0.0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      qr = __builtin_fma(s_D[i * NQ + m], s_qt[k * NQ2 + j * NQ + m], qr);
+      qs = __builtin_fma(s_D[j * NQ + m], s_qt[k * NQ2 + m * NQ + i], qs);
+      qt = __builtin_fma(D[k * NQ + m], s_qt[m * NQ2 + t], qt);
+      if (m % 2 == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the reads the scheduler batches
+    }
+    const int n = o + kk * NQ2;
+    const double G00 = ldnt(&g00[n]), G01 = ldnt(&g01[n]), G02 = ldnt(&g02[n]);
+    const double G11 = ldnt(&g11[n]), G12 = ldnt(&g12[n]), G22 = ldnt(&g22[n]);
+    double* sr = s_Gr[w][kk & 1];
+    double* ss = s_Gs[w][kk & 1];
+    sr[t] = __builtin_fma(G02, qt, __builtin_fma(G01, qs, G00 * qr));
+    ss[t] = __builtin_fma(G12, qt, __builtin_fma(G11, qs, G01 * qr));
+    Gt[kk] = __builtin_fma(G22, qt, __builtin_fma(G12, qs, G02 * qr));
+    __syncthreads();
+    double ar = 0.0, as = 0.0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      ar = __builtin_fma(s_D[m * NQ + i], sr[j * NQ + m], ar);
+      as = __builtin_fma(s_D[m * NQ + j], ss[m * NQ + i], as);
+      if (m % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    ars[kk] = ar + as;
+  }
+  // every q read of both waves precedes their last barrier above
+#pragma unroll
+  for (int kk = 0; kk < H; ++kk) s_qt[(H * w + kk) * NQ2 + t] = Gt[kk];
+  __syncthreads();
+  double* Ae = Aq + e * NP;
+#pragma unroll
+  for (int kk = 0; kk < H; ++kk) {
+    const int k = H * w + kk;
+    double at = 0.0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) at = __builtin_fma(D[m * NQ + k], s_qt[m * NQ2 + t], at);
+    stnt(ars[kk] + at, &Ae[o + kk * NQ2]);
+  }
+}
+
 // PCG form (FP64): p = z + beta p (BPScaledAdd, nekBone/solve.cpp:100), then
 // Ap_local = A p, and the element's share of pAp = sum_nodes p * Ap_local.
 // Because p is continuous and zero on the mask, sum_local p (A_L p) equals
@@ -378,6 +455,14 @@ static int launch_axhelm_t(int32_t Nelements, int32_t offset, const int32_t* ele
                            const void* ggeo, const void* D, const void* lambda, const void* q, void* Aq,
                            const int* skip, hipStream_t s)
 {
+  if constexpr (std::is_same<T, double>::value && std::is_same<G, double>::value && NQ == 8 && NF == 1 && BK &&
+                !PARTIAL) {
+    if (Nelements <= 0) return NBX_SUCCESS;
+    hipLaunchKernelGGL(k_axhelm_bk8_split, dim3(Nelements), dim3(64, 2), 0, s, Nelements, (const double*)ggeo,
+                       (const double*)D, (const double*)q, (double*)Aq, skip);
+    NBX_HIP_LAUNCH();
+    return NBX_SUCCESS;
+  }
   constexpr int EPB = Epb<NQ>::value;
   constexpr int GR = AxGroups<NQ, NF, BK, G>::value;
   hipLaunchKernelGGL((k_axhelm<T, NQ, NF, BK, PARTIAL, G>), dim3((Nelements + EPB - 1) / EPB), dim3(NQ * NQ, GR), 0,
