@@ -328,23 +328,44 @@ constexpr int kOgsBlock = 256;
 
 inline dim3 ogs_grid(int64_t n) { return dim3((unsigned)((n + kOgsBlock - 1) / kOgsBlock)); }
 
-// largest |id| and number of nonzero ids (order-independent totals): each
-// wave reduces with shuffles, one atomic pair per wave
-__global__ void k_ogs_range(int32_t N, const int64_t* __restrict__ ids, unsigned long long* __restrict__ amax,
-                            int* __restrict__ count)
+// largest |id| and number of nonzero ids (order-independent totals): a
+// fixed grid strides over the ids, each block reduces with wave shuffles and
+// LDS, one atomic pair per block (per-wave atomics on the two words took
+// milliseconds at 16.7M ids)
+constexpr int kOgsRangeBlocks = 1024;
+
+__global__ __launch_bounds__(kOgsBlock) void k_ogs_range(int32_t N, const int64_t* __restrict__ ids,
+                                                         unsigned long long* __restrict__ amax, int* __restrict__ count)
 {
-  const int64_t i = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x;
-  const int64_t v = i < N ? ids[i] : 0;
-  unsigned long long a = (unsigned long long)(v < 0 ? -v : v);
-  int c = v != 0 ? 1 : 0;
+  __shared__ unsigned long long s_a[kOgsBlock / 64];
+  __shared__ int s_c[kOgsBlock / 64];
+  unsigned long long a = 0;
+  int c = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kOgsBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kOgsBlock) {
+    const int64_t v = ids[i];
+    const unsigned long long u = (unsigned long long)(v < 0 ? -v : v);
+    a = u > a ? u : a;
+    c += v != 0 ? 1 : 0;
+  }
   for (int h = 32; h >= 1; h >>= 1) {
     const unsigned long long ao = __shfl_xor(a, h, 64);
     a = ao > a ? ao : a;
     c += __shfl_xor(c, h, 64);
   }
-  if ((threadIdx.x & 63) == 0 && c > 0) {
-    atomicMax(amax, a);
-    atomicAdd(count, c);
+  if ((threadIdx.x & 63) == 0) {
+    s_a[threadIdx.x >> 6] = a;
+    s_c[threadIdx.x >> 6] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kOgsBlock / 64; ++w) {
+      a = s_a[w] > a ? s_a[w] : a;
+      c += s_c[w];
+    }
+    if (c > 0) {
+      atomicMax(amax, a);
+      atomicAdd(count, c);
+    }
   }
 }
 
@@ -523,7 +544,8 @@ int ogs_build(nbx_ogs* o, int32_t N, const nbx_hlong* ids, bool on_device)
     unsigned long long* d_r = nullptr;
     NBX_HIP(hipMalloc(&d_r, 2 * sizeof(unsigned long long)));
     (void)hipMemset(d_r, 0, 2 * sizeof(unsigned long long));
-    hipLaunchKernelGGL(k_ogs_range, ogs_grid(N), dim3(kOgsBlock), 0, nullptr, N, ids, d_r, (int*)(d_r + 1));
+    hipLaunchKernelGGL(k_ogs_range, dim3((unsigned)std::min<int64_t>(ogs_grid(N).x, kOgsRangeBlocks)), dim3(kOgsBlock), 0,
+                       nullptr, N, ids, d_r, (int*)(d_r + 1));
     unsigned long long h[2] = {0, 0};
     const hipError_t e1 = hipGetLastError(), e2 = hipMemcpy(h, d_r, sizeof(h), hipMemcpyDeviceToHost);
     (void)hipFree(d_r);
