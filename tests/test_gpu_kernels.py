@@ -49,6 +49,28 @@ def test_axhelm_f64_matches_oracle(nbx, nbo, dev, meshes, N, Nf, bk):
     assert np.array_equal(got, ref), f"not bit-exact: rel {relerr(got, ref):.3e}"
 
 
+@pytest.mark.parametrize("dims", [(5, 3, 3), (16, 16, 16)])
+def test_axhelm_f64_bk_n7_two_wave_kernel(nbx, nbo, dev, dims):
+    """The stand-alone FP64 BK one-field operator at N=7 splits each element
+    over two waves (slabs 0-3 and 4-7) and exchanges q and Gt through LDS:
+    bit-identical to the oracle on an odd element count and on the 16^3
+    (4096-element) mesh the axhelm benchmark runs, and twice in a row on the
+    same buffers (nothing carried between launches)."""
+    import torch
+    box = nbo.Box.make(7, *dims, perturb=0.1, zaspect=1.5, seed=31)
+    m = nbo.box_mesh(box)
+    E, off = box.E, box.E * box.Np
+    q = nbo.urand(7, 3, off)
+    ref = nbo.axhelm(7, 1, 1, E, off, m["ggeo"], m["D"], np.zeros(2 * off), q)
+    dg, dD, dq = T(dev, m["ggeo"]), T(dev, m["D"]), T(dev, q)
+    for _ in range(2):
+        dAq = torch.full((off,), float("nan"), dtype=torch.float64, device=dev)
+        nbx.axhelm(7, 1, 1, E, off, dg, dD, None, dq, dAq)
+        torch.cuda.synchronize()
+        got = H(dAq)
+        assert np.array_equal(got, ref), f"not bit-exact: rel {relerr(got, ref):.3e}"
+
+
 @pytest.mark.parametrize("N", [3, 7])
 def test_axhelm_partial_and_by_name(nbx, nbo, dev, meshes, N):
     import torch
