@@ -22,6 +22,12 @@
  *   nbo_jacobi_local        diagonal of the dense element matrix built from
  *                           nbo_axhelm applied to unit vectors
  *   nbo_tree_*              exact integer-valued sums, brute-force numpy sums
+ *   nbo_axhelm_bf16mma /    within operand rounding of nbo_axhelm (FP64) for
+ *   nbo_axhelm_f16mma       BK and non-BK, constant vector near the null space;
+ *                           the fp16 form several times closer than the bf16
+ *                           form (tests/test_oracle_axhelm.py)
+ *   nbo_f16_round_array     numpy's float16 conversion: random values, exact
+ *                           ties (normal and subnormal), overflow edge, zeros
  */
 #include "nbo.h"
 #include <limits.h>
