@@ -15,8 +15,10 @@ algorithmic bytes E*Np*(2+7)*8 per launch (axhelm/main.cpp:162-163) over the
 average launch time measured with HIP events on the library's stream;
 traffic = HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
 WRITE_SIZE runs (profiles/traffic.json, calibrated on a 1 GiB copy).
-cpu_baseline: the CPU oracle's restatement of the same solve on a bounded
-sample (setup + 5 PCG iterations of the same problem), OpenMP on the host.
+cpu_baseline: the CPU oracle's restatement of the same solve, timed in a
+child process (oracle/cpu_baseline.py, OpenMP with OMP_PLACES=cores and
+OMP_PROC_BIND=close on the host's CPU share): the per-iteration time as the
+difference of two fixed-count solves, so the setup of each solve cancels.
 """
 import argparse
 import json
@@ -30,34 +32,55 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "PCG time-to-1e-8 residual, learned/mixed vs FP64 Jacobi; axhelm GDOF/s"
 HBM_PEAK_GBS = 8000.0
+ROOF_SETS = 4  # rotating buffer sets of the roofline measurement
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(N, NX, NY, NZ, iters):
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
-    import nbo
-    box = nbo.Box.make(N, NX, NY, NZ, perturb=0.1, seed=1)
-    t0 = time.perf_counter()
-    ob = nbo.OracleBP(box, bpmode=True)
-    t_setup = time.perf_counter() - t0
-    r, x = ob.rhs.copy(), np.zeros(ob.offset)
-    t0 = time.perf_counter()
-    st, _, _ = ob.pcg(r, x, 0.0, iters, jacobi=True)
-    t = time.perf_counter() - t0
-    E = box.E
-    return {
-        "value": iters * E * N ** 3 / t / 1e9, "unit": "GDOF/s", "cores": int(nbo.lib().nbo_max_threads()),
-        "kind": "port",
-        "sample": f"CPU oracle (nbo_bp_pcg, OpenMP, {cpu_model()}): same 32^3 N=7 problem, Jacobi PCG fixed {iters} iterations "
-                  f"incl. r=b-Ax and the Jacobi setup, {t:.2f} s (+{t_setup:.1f} s mesh/ogs setup untimed)",
-    }
+def run_cpu_baseline(pcg_nx, axhelm_nx, log):
+    """The CPU baseline in a child process (no GPU there): OpenMP placement
+    binds that process's threads only.  Threads = OMP_NUM_THREADS when set,
+    else the CPUs of this process's affinity set."""
+    import subprocess
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
+    env["OMP_PLACES"] = "cores"
+    env["OMP_PROC_BIND"] = "close"
+    cmd = [sys.executable, str(ROOT / "oracle" / "cpu_baseline.py")]
+    if pcg_nx:
+        cmd += ["--pcg", str(pcg_nx)]
+    if axhelm_nx:
+        cmd += ["--axhelm", str(axhelm_nx)]
+    r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=900)
+    if r.returncode != 0:
+        log(f"[bench] cpu baseline failed ({r.returncode}): {r.stderr.decode(errors='replace')[-2000:]}")
+        return None
+    res = json.loads(r.stdout.decode().strip().splitlines()[-1])
+    log(f"[bench] cpu baseline: {res}")
+    return res
 
 
-def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None, mg_cfg=None):
+def cpu_baseline_record(res):
+    """The benchmark line's cpu_baseline object from the child's result."""
+    if not res or "pcg" not in res:
+        return None
+    p, c = res["pcg"], res["cores"]
+    rec = {"value": p["value"], "unit": "GDOF/s", "cores": c["threads"], "kind": "port",
+           "sample": f"CPU oracle restatement of the same {round(p['elements'] ** (1 / 3))}^3 N=7 Jacobi PCG "
+                     f"(nbo_bp_pcg, OpenMP, {c['threads']} threads on {c['physical_cores_in_affinity']} physical "
+                     f"cores of {c['cpu']}, nproc {c['nproc']}, OMP_PLACES=cores OMP_PROC_BIND=close): "
+                     f"(t[{p['k2']} it] - t[{p['k1']} it]) / {p['k2'] - p['k1']} = {p['s_per_iteration']:.4f} s "
+                     f"per iteration, setup cancelled; {res['build']}"}
+    ax = res.get("axhelm")
+    if ax and ax.get("kind") == "reference":
+        rec["calibration"] = {"axhelm_reference_us": ax["us_per_apply"], "axhelm_port_us": ax["oracle_port_us_per_apply"],
+                              "port_over_reference": ax["oracle_port_over_reference"]}
+    return rec
+
+
+def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None, mg_cfg=None, net=None):
     """Time to a 1e-8 relative FP64 residual on the same problem: FP64 Jacobi
     PCG without the iteration cap, and the mixed-precision refinement solvers.
     Each solver runs once untimed, then once timed; the error of each
@@ -96,9 +119,51 @@ def solver_comparison(nbx, bp, b, r, x, log, tol=1e-8, bp_mg=None, mg_cfg=None):
                            ("mixed_mg_bf16cycle", nbx.NBX_BF16), ("mixed_mg_bf16mfma_cycle", nbx.NBX_BF16_MFMA)):
             bp_mg.mg_setup(orders, degree=degree, coarse_iters=ci, prec=prec)
             run(name, lambda: bp_mg.pcg(r, x, tol, 500))
-    base = out["fp64_jacobi"]["time_s"]
-    for k, v in out.items():
-        v["speedup_vs_fp64_jacobi"] = round(base / v["time_s"], 3)
+        if net is not None:
+            # the network trained on the 16^3 mesh's multigrid pairs, attached
+            # to this mesh (its features are this mesh's geometry)
+            bp_mg.mg_setup(orders, degree=degree, coarse_iters=ci, prec=nbx.NBX_BF16)
+            net.attach(bp_mg)
+            bp_mg.mg_set_learned(True)
+            run("learned_mixed_mg_bf16cycle", lambda: bp_mg.pcg(r, x, tol, 500))
+            bp_mg.mg_set_learned(False)
+            out["learned_mixed_mg_bf16cycle"]["gain_vs_same_mg"] = round(
+                out["mixed_mg_bf16cycle"]["time_s"] / out["learned_mixed_mg_bf16cycle"]["time_s"], 3)
+            out["learned_mixed_mg_bf16cycle"]["inference"] = "bf16 matrix cores"
+    set_speedups(out)
+    return out
+
+
+def set_speedups(out):
+    """speedup against FP64 Jacobi PCG; null for a solve that stopped at its
+    iteration cap without reaching the tolerance (or when FP64 Jacobi did)."""
+    base = out["fp64_jacobi"]
+    for v in out.values():
+        ok = v["converged"] and base["converged"]
+        v["speedup_vs_fp64_jacobi"] = round(base["time_s"] / v["time_s"], 3) if ok else None
+
+
+def flat_time_to_tol(res, prefix):
+    """Flat scalar summary of a time-to-tolerance table (kept by parsers that
+    drop nested objects): FP64 Jacobi, the fastest converged solver, and the
+    learned ones when present."""
+    out = {}
+    base = res.get("fp64_jacobi")
+    if not base:
+        return out
+    out[f"{prefix}_fp64_jacobi_s"] = base["time_s"] if base["converged"] else None
+    done = {k: v for k, v in res.items() if v["converged"] and k != "fp64_jacobi"}
+    if done:
+        best = min(done, key=lambda k: done[k]["time_s"])
+        out[f"{prefix}_best_solver"] = best
+        out[f"{prefix}_best_s"] = done[best]["time_s"]
+        out[f"{prefix}_best_speedup"] = done[best]["speedup_vs_fp64_jacobi"]
+    for k, v in res.items():
+        if k.startswith("learned"):
+            out[f"{prefix}_{k}_s"] = v["time_s"] if v["converged"] else None
+            out[f"{prefix}_{k}_speedup"] = v["speedup_vs_fp64_jacobi"]
+            if "gain_vs_same_mg" in v:
+                out[f"{prefix}_{k}_gain_vs_same_mg"] = v["gain_vs_same_mg"]
     return out
 
 
@@ -159,8 +224,7 @@ def run_bp6_n9_24cube(nbx, torch, args):
             rec["err_vs_fp64_per_field"] = field_errors(x, x64, 3, bp.fieldOffset, bp.Nlocal)
         out[name] = rec
         log(f"[bp6] {name}: {rec}")
-    for v in out.values():
-        v["speedup_vs_fp64_jacobi"] = round(out["fp64_jacobi"]["time_s"] / v["time_s"], 3)
+    set_speedups(out)
     return {"config": "BP6 vector Helmholtz N=9, 24^3 elements, 3 fields, lambda1=0.1, tol 1e-10",
             "elements": bp.Nelements, "solvers": out}
 
@@ -312,54 +376,71 @@ def run_graded_64x64x32(nbx, torch, args):
     net.attach(bl)
     run("learned_mixed_f32", lambda: bl.pcg_mixed(r, x, 1e-10, args.maxit_graded, prec=nbx.NBX_F32, inner_tol=1e-5,
                                                   max_cycles=20))
-    for v in out.values():
-        v["speedup_vs_fp64_jacobi"] = round(out["fp64_jacobi"]["time_s"] / v["time_s"], 3)
+    set_speedups(out)
     return {"config": "64x64x32 elements N=7, z-graded aspect 1:10, perturbed, tol 1e-10",
             "elements": 64 * 64 * 32, "learned_training": info, "solvers": out}
 
 
-def cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
-
-
-def cpu_reference_axhelm(bj, E, n, g0, D, log):
-    """The reference's own CPU kernel axhelm_bk_v0 (axhelm/kernel/CPU/axhelm.c,
-    compiled by oracle/Makefile into oracle/_ref, OpenMP) on the same 16^3
-    mesh, with axhelm/main.cpp's protocol: one warm-up call, then the mean of
-    100 calls into a preallocated output; host cores as OMP_NUM_THREADS gives
-    them.  The oracle's restatement (nbo_axhelm) is timed the same way beside
-    it, which calibrates the oracle as a CPU baseline."""
-    sys.path.insert(0, str(ROOT / "oracle"))
+def run_learned_mg(nbx, torch, args):
+    """Learned multigrid: the element network as the last step of the fine
+    level of every V-cycle (bf16-geometry cycle, orders 7, 3, 1), trained on
+    the mesh's own multigrid pairs (nekbench_amd.learned), against the same
+    multigrid without the network, on a uniform 16^3 mesh (tol 1e-8) and a
+    z-graded 16x16x8 mesh (aspect 1:10, tol 1e-10): iterations and time to
+    tolerance on five unseen right-hand sides.  --mg-variants lists training
+    variants kfirst:kcount:mode:lr:alpha (pairs of iterates kfirst..,
+    loss mode 0 energy / 1 scale-invariant, Adam rate, damped-Jacobi start)."""
     import numpy as np
-    import nbo
-    if not nbo.ref_available(7):
-        return None
-    ggeo = g0.cpu().numpy()
-    Dh = D.cpu().numpy()
-    q = np.random.default_rng(1).uniform(-1, 1, n)
-    out = np.zeros(n)
+    from nekbench_amd.learned import train_mg_network
+    sync = torch.cuda.synchronize
+    variants = []
+    for v in args.mg_variants.split(","):
+        kf, kc, mode, lr, alpha = v.split(":")
+        variants.append((v, int(kf), int(kc), int(mode), float(lr), float(alpha)))
+    cases = [("uniform_16cube", nbx.Box.make(7, 16, 16, 16, perturb=0.1, seed=1), 1e-8, 8),
+             ("graded_16x16x8", nbx.Box.make(7, 16, 16, 8, perturb=0.1, zaspect=10.0, seed=2), 1e-10, 8)]
+    out = {}
+    for name, box, tol, ci in cases:
+        bm = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
+        bm.mg_setup([7, 3, 1], degree=args.mg_degree, coarse_iters=ci, prec=nbx.NBX_BF16)
+        n = bm.fieldOffset
+        b, r, x = (torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3))
+        seeds = list(range(9000, 9000 + args.mg_eval_seeds))
 
-    def mean_time(fn, reps=100):
-        fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        return (time.perf_counter() - t0) / reps
+        def evaluate():
+            its, ts = [], []
+            for seed in seeds:
+                bm.make_rhs(seed, b)
+                for _ in range(2):
+                    r.copy_(b)
+                    x.zero_()
+                    st, t = timed(lambda: bm.pcg(r, x, tol, 500), sync)
+                its.append(st.iterations if st.converged else None)
+                ts.append(round(t, 5))
+            ok = all(i is not None for i in its)
+            return {"iterations": its, "time_s": ts, "mean_iterations": float(np.mean(its)) if ok else None,
+                    "mean_time_s": float(np.mean(ts)), "converged": ok}
 
-    t = mean_time(lambda: nbo.ref_axhelm(7, "axhelm_bk_v0", E, n, ggeo, Dh, None, q, out=out))
-    t_port = mean_time(lambda: nbo.axhelm(7, 1, 1, E, n, ggeo, Dh, None, q, out=out))
-    res = {"kind": "reference", "kernel": "axhelm_bk_v0 of axhelm/kernel/CPU/axhelm.c (g++ -O2 -fopenmp)",
-           "us_per_apply": round(t * 1e6, 1), "GB_s": round(E * 512 * 72 / t / 1e9, 1),
-           "gdofs": round(E * 343 / t / 1e9, 3), "cores": int(nbo.lib().nbo_max_threads()), "cpu": cpu_model(),
-           "oracle_port_us_per_apply": round(t_port * 1e6, 1), "oracle_port_over_reference": round(t_port / t, 3)}
-    log(f"[bench] reference CPU axhelm: {res}")
-    return res
+        res = {"tol": tol, "coarse_iterations": ci}
+        res["mg"] = evaluate()
+        log(f"[learned-mg] {name} plain multigrid {res['mg']}")
+        for tag, kf, kc, mode, lr, alpha in variants:
+            net, info = train_mg_network(nbx, torch, bm, seeds=(1000, args.mg_pair_seeds), kfirst=kf, kcount=kc,
+                                         steps=args.train_steps_mg, lr=lr, alpha=alpha, mode=mode, log=log)
+            rec = {"training": info}
+            for inf, mfma in (("fp32_inference", False), ("mfma_inference", True)):
+                net.set_inference(mfma)
+                rec[inf] = evaluate()
+            base = res["mg"]["mean_time_s"]
+            rec["gain_vs_same_mg"] = round(base / rec["mfma_inference"]["mean_time_s"], 3)
+            log(f"[learned-mg] {name} variant {tag}: fp32 {rec['fp32_inference']} mfma {rec['mfma_inference']}")
+            res[f"learned_{tag}"] = rec
+            bm.mg_set_learned(False)
+            net.free()
+        out[name] = res
+        bm.free()
+    return {"config": "learned multigrid step vs the same multigrid, bf16-geometry cycle (7,3,1), degree "
+                      f"{args.mg_degree}", "cases": out}
 
 
 def mesh16_measurements(nbx, torch, args, log, axhelm_only=False):
@@ -393,12 +474,27 @@ def mesh16_measurements(nbx, torch, args, log, axhelm_only=False):
     out = {"axhelm_fp64_us": round(t_ax * 1e6, 2), "axhelm_GB_s": round(bytes_ / t_ax / 1e9, 1),
            "axhelm_frac_of_8TBs": round(bytes_ / t_ax / 8e12, 4), "axhelm_gdofs": round(E * 343 / t_ax / 1e9, 2),
            "buffer_sets": len(sets), "bytes_per_apply": bytes_}
-    if not args.no_cpu:
-        out["cpu_reference_axhelm"] = cpu_reference_axhelm(bj, E, n, g0, D, log)
     del sets
+    # the practical ceiling at this size: a streaming triad c = a + alpha b
+    # (nbx_triad, the bw benchmark's kernel) moving the same bytes per launch,
+    # rotated over the same number of buffer sets, timed the same way
+    nt = bytes_ // 24
+    tri = [tuple(torch.rand(nt, dtype=torch.float64, device="cuda") for _ in range(3)) for _ in range(5)]
+    for a, b_, c in tri:
+        nbx.lib().nbx_triad(nt, 0.5, a.data_ptr(), b_.data_ptr(), c.data_ptr(), s.cuda_stream)
+    ev0.record(s)
+    for _ in range(reps):
+        for a, b_, c in tri:
+            nbx.lib().nbx_triad(nt, 0.5, a.data_ptr(), b_.data_ptr(), c.data_ptr(), s.cuda_stream)
+    ev1.record(s)
+    ev1.synchronize()
+    t_tri = ev0.elapsed_time(ev1) / 1e3 / (reps * len(tri))
+    del tri
+    out.update({"triad_same_bytes_us": round(t_tri * 1e6, 2), "triad_same_bytes_GB_s": round(24 * nt / t_tri / 1e9, 1),
+                "axhelm_over_triad_time": round(t_ax / t_tri, 3)})
     log(f"[bench] 16^3 axhelm: {out}")
     if axhelm_only:
-        return out
+        return out, None
     b = bj.device_array("rhs", n).clone()
     r, x = torch.empty_like(b), torch.empty_like(b)
     sync = torch.cuda.synchronize
@@ -418,11 +514,24 @@ def mesh16_measurements(nbx, torch, args, log, axhelm_only=False):
         # the smaller coarse problem (15^3 unknowns) needs fewer coarse iterations than 32^3
         bm.mg_setup([7, 3, 1], degree=3, coarse_iters=8, prec=prec)
         run(name, bm)
-    for v in res.values():
-        v["speedup_vs_fp64_jacobi"] = round(res["fp64_jacobi"]["time_s"] / v["time_s"], 3)
+    # learned multigrid: the bf16-geometry cycle plus the network's step on
+    # the fine level, trained here on this mesh's multigrid pairs
+    net = None
+    if not args.no_learned:
+        from nekbench_amd.learned import train_mg_network
+        net, info = train_mg_network(nbx, torch, bm, seeds=(1000, args.mg_pair_seeds), kcount=8,
+                                     steps=args.train_steps_mg, log=log)
+        net.set_inference(True)
+        run("learned_mixed_mg_bf16cycle", bm)
+        bm.mg_set_learned(False)
+        res["learned_mixed_mg_bf16cycle"]["gain_vs_same_mg"] = round(
+            res["mixed_mg_bf16cycle"]["time_s"] / res["learned_mixed_mg_bf16cycle"]["time_s"], 3)
+        res["learned_mixed_mg_bf16cycle"]["inference"] = "bf16 matrix cores"
+        out["learned_mg_training"] = info
+    set_speedups(res)
     out["time_to_1e-8"] = res
     log(f"[bench] 16^3 time to 1e-8: {res}")
-    return out
+    return out, net
 
 
 def profile_mg_pass(nbx, torch, NX):
@@ -448,7 +557,22 @@ def profile_mg_pass(nbx, torch, NX):
     zl = torch.empty_like(b)
     for _ in range(5):
         net.apply(bl, b, zl)
+    net.set_inference(True)  # the matrix-core inference
+    for _ in range(5):
+        net.apply(bl, b, zl)
     torch.cuda.synchronize()
+    # the learned multigrid step (its damped-Jacobi start network, matrix-core inference)
+    from nekbench_amd.learned import damped_jacobi_params
+    net2 = nbx.Net(seed=4)
+    net2.set_params(damped_jacobi_params(net2.params(), 0.5))
+    net2.set_inference(True)
+    net2.attach(bp)
+    bp.mg_set_learned(True)
+    for _ in range(2):
+        r.copy_(b)
+        x.zero_()
+        st = bp.pcg(r, x, 1e-8, 500)
+    log(f"[bench] learned multigrid (damped-Jacobi start): {st.iterations} iterations, {st.elapsed_s * 1e3:.1f} ms")
     log("[bench] profile pass done")
 
 
@@ -461,25 +585,38 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxit", type=int, default=500)
     ap.add_argument("--ax-launches", type=int, default=20)
-    ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the time-to-1e-8 comparison of the FP64, mixed and learned solvers")
-    ap.add_argument("--study", default="pcg32", choices=("pcg32", "axhelm16", "bp6", "learned16", "graded"),
+    ap.add_argument("--study", default="pcg32",
+                    choices=("pcg32", "axhelm16", "bp6", "learned16", "graded", "learned_mg"),
                     help="pcg32: the PCG benchmark line on 32^3 (default); axhelm16: the 16^3 FP64 axhelm alone "
                          "(rotating buffer sets); bp6: BP6 N=9 24^3 mixed precision; learned16: learned "
                          "preconditioner on 16^3; graded: 64x64x32 graded stress case (all but pcg32 print "
                          "one JSON line of their own)")
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps-mg", type=int, default=1500,
+                    help="Adam steps of the learned multigrid step's network (trained on the 16^3 mesh)")
+    ap.add_argument("--mg-pair-seeds", type=int, default=100,
+                    help="right-hand sides of the learned multigrid's training pairs (8 pairs each)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned multigrid solver")
+    ap.add_argument("--mg-degree", type=int, default=3, help="Chebyshev degree of the learned-multigrid study")
+    ap.add_argument("--mg-variants", default="0:8:0:1e-3:0.5",
+                    help="learned-multigrid study: comma list of kfirst:kcount:mode:lr:alpha training variants")
+    ap.add_argument("--mg-eval-seeds", type=int, default=5, help="right-hand sides of the learned-multigrid study")
+    ap.add_argument("--mg-alpha", type=float, default=0.5,
+                    help="damping of the learned step's damped-Jacobi start (1 - alpha multiplies Jacobi)")
     ap.add_argument("--learn-mode", type=int, default=1, choices=(0, 1),
                     help="energy-norm loss: 0 fixed scale, 1 scale-invariant")
     ap.add_argument("--maxit-graded", type=int, default=30000)
     ap.add_argument("--profile-mg", action="store_true",
                     help="short sequence for a rocprofv3 kernel trace: two multigrid PCG solves (bf16-geometry "
-                         "V-cycle, degree 3, 20 coarse iterations) to 1e-8 and five network applies; prints no JSON")
+                         "V-cycle, degree 3, 20 coarse iterations) to 1e-8, five network applies with the FP32 and "
+                         "five with the matrix-core inference, two learned-multigrid solves; prints no JSON")
     ap.add_argument("--profile-pass", action="store_true",
                     help="short fixed sequence for rocprofv3 counter runs: a 1 GiB device copy (calibration), "
-                         "5 axhelm launches, one 10-iteration PCG solve; prints no JSON")
+                         "8 axhelm launches over the roofline's 4 rotating buffer sets, one 10-iteration PCG "
+                         "solve; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -495,10 +632,10 @@ def main():
     if args.study != "pcg32":
         torch.cuda.set_device(0)
         if args.study == "axhelm16":
-            res = mesh16_measurements(nbx, torch, args, log, axhelm_only=True)
+            res = mesh16_measurements(nbx, torch, args, log, axhelm_only=True)[0]
         else:
             res = {"bp6": run_bp6_n9_24cube, "learned16": run_learned_16cube,
-                   "graded": run_graded_64x64x32}[args.study](nbx, torch, args)
+                   "graded": run_graded_64x64x32, "learned_mg": run_learned_mg}[args.study](nbx, torch, args)
         print(json.dumps(res), flush=True)
         return
 
@@ -528,10 +665,15 @@ def main():
             dst.copy_(src)
         ggeo = bp.device_array("ggeo", bp.Nelements * 7 * bp.Np)
         D = bp.device_array("D", (N + 1) ** 2)
-        Ap = torch.empty_like(x)
         x.copy_(b)
-        for _ in range(5):
-            nbx.axhelm(N, 1, 1, bp.Nelements, n, ggeo, D, None, x, Ap)
+        # the roofline's rotating buffer sets: each launch reads a set the
+        # previous ROOF_SETS - 1 launches did not touch
+        sets = [(ggeo, x, torch.empty_like(x))] + [(ggeo.clone(), torch.rand_like(x) * 2 - 1, torch.empty_like(x))
+                                                   for _ in range(ROOF_SETS - 1)]
+        for k in range(2 * ROOF_SETS):
+            g_, q_, a_ = sets[k % ROOF_SETS]
+            nbx.axhelm(N, 1, 1, bp.Nelements, n, g_, D, None, q_, a_)
+        del sets
         r.copy_(b)
         x.zero_()
         bp.pcg(r, x, 0.0, 10)
@@ -553,23 +695,30 @@ def main():
     log(f"[bench] PCG: {its} iterations, converged={st.converged}, rel res "
         f"{(st.rdotr / st.rdotr0) ** 0.5:.3e}, {t * 1e3:.2f} ms/solve ({t / max(its, 1) * 1e6:.1f} us/iter)")
 
-    # axhelm launches on the solve's own arrays, HIP events on the launch stream
+    # axhelm launches on the solve's mesh with ROOF_SETS rotating (ggeo, q, Aq)
+    # buffer sets (4 x 1.2 GB, far beyond the 256 MB Infinity Cache, so no
+    # launch finds its inputs in cache), HIP events on the launch stream
     ggeo = bp.device_array("ggeo", E * 7 * bp.Np)
     D = bp.device_array("D", (N + 1) ** 2)
     Ap = torch.empty_like(x)
     s = torch.cuda.current_stream()
-    for _ in range(3):
-        nbx.axhelm(N, 1, 1, E, n, ggeo, D, None, x, Ap, stream=s)
+    roof_sets = [(ggeo, x, Ap)] + [(ggeo.clone(), torch.rand_like(x) * 2 - 1, torch.empty_like(x))
+                                   for _ in range(ROOF_SETS - 1)]
+    for g_, q_, a_ in roof_sets:
+        nbx.axhelm(N, 1, 1, E, n, g_, D, None, q_, a_, stream=s)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(1, args.ax_launches // ROOF_SETS)
     ev0.record(s)
-    for _ in range(args.ax_launches):
-        nbx.axhelm(N, 1, 1, E, n, ggeo, D, None, x, Ap, stream=s)
+    for _ in range(reps):
+        for g_, q_, a_ in roof_sets:
+            nbx.axhelm(N, 1, 1, E, n, g_, D, None, q_, a_, stream=s)
     ev1.record(s)
     ev1.synchronize()
-    ax_s = ev0.elapsed_time(ev1) / 1e3 / args.ax_launches
+    ax_s = ev0.elapsed_time(ev1) / 1e3 / (reps * ROOF_SETS)
+    del roof_sets
     ax_bytes = E * bp.Np * (2 + 7) * 8
     ax_gbs = ax_bytes / ax_s / 1e9
-    log(f"[bench] axhelm BK N=7 E={E}: {ax_s * 1e6:.1f} us/launch, {ax_gbs:.0f} GB/s, "
+    log(f"[bench] axhelm BK N=7 E={E}: {ax_s * 1e6:.1f} us/launch over {ROOF_SETS} buffer sets, {ax_gbs:.0f} GB/s, "
         f"{E * N ** 3 / ax_s / 1e9:.1f} GDOF/s")
 
     # the other precisions of the same operator on the same mesh (FP32 vectors)
@@ -592,7 +741,7 @@ def main():
     log(f"[bench] axhelm variants: {variants}")
     del g32, g16, q32, A32
 
-    mesh16 = None if args.no_compare else mesh16_measurements(nbx, torch, args, log)
+    mesh16, net = (None, None) if args.no_compare else mesh16_measurements(nbx, torch, args, log)
 
     compare = None
     if not args.no_compare:
@@ -601,13 +750,24 @@ def main():
         bp_mg = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
         bp_mg.mg_setup(mg_cfg[0], degree=mg_cfg[1], coarse_iters=mg_cfg[2])
         log(f"[bench] multigrid setup {time.perf_counter() - t1:.1f} s (orders 7,3,1)")
-        compare = solver_comparison(nbx, bp, b, r, x, log, bp_mg=bp_mg, mg_cfg=mg_cfg)
+        compare = solver_comparison(nbx, bp, b, r, x, log, bp_mg=bp_mg, mg_cfg=mg_cfg, net=net)
         bp_mg.free()
 
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists() and NX == 32:
         traffic = json.loads(tfile.read_text()).get("axhelm_bk_f64_n7_32cube", {}).get("traffic_bytes_per_launch")
+    cpu = None if args.no_cpu else run_cpu_baseline(NX, 0 if args.no_compare else 16, log)
+    if mesh16 is not None and cpu and "axhelm" in cpu:
+        mesh16["cpu_reference_axhelm"] = dict(cpu["axhelm"], cores=cpu["cores"]["threads"], cpu=cpu["cores"]["cpu"])
+    flat = {}
+    if compare:
+        flat.update(flat_time_to_tol(compare, "t1e8_32cube"))
+    if mesh16 and "time_to_1e-8" in mesh16:
+        flat.update(flat_time_to_tol(mesh16["time_to_1e-8"], "t1e8_16cube"))
+    if mesh16:
+        flat.update({"axhelm16_us": mesh16["axhelm_fp64_us"], "axhelm16_frac_of_8TBs": mesh16["axhelm_frac_of_8TBs"],
+                     "triad16_same_bytes_us": mesh16.get("triad_same_bytes_us")})
     out = {
         "metric": METRIC, "value": round(gdofs, 4), "unit": "GDOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
@@ -615,25 +775,23 @@ def main():
         "config": {"workload": f"nekBone FP64 Jacobi PCG, {NX}^3 elements N=7 Dirichlet perturbed box, "
                                f"tol {args.tol:g} or {args.maxit} iterations", "elements": E, "N": N,
                    "iterations": its, "converged": bool(st.converged),
-                   "time_to_tol_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
+                   # wall time of one solve; a time to tolerance only when converged is true
+                   "time_s": round(t, 6), "final_rel_residual": (st.rdotr / st.rdotr0) ** 0.5,
                    "axhelm_gdofs": round(E * N ** 3 / ax_s / 1e9, 3),
                    # axhelm/main.cpp:165-168: (12 Nq + 15) flop per node (BK); FP64 vector peak 78.6 TFLOP/s
                    "axhelm_fp64_tflops": round(E * bp.Np * (12 * (N + 1) + 15) / ax_s / 1e12, 2),
                    "axhelm_fp64_frac_of_78p6": round(E * bp.Np * (12 * (N + 1) + 15) / ax_s / 78.6e12, 4),
                    "axhelm_variants": variants,
                    "parallelism": "single GPU", "setup_s": round(setup_s, 3),
+                   **flat,
                    "time_to_1e-8": compare, "mesh16": mesh16},
         "roofline": {"bound": "hbm", "achieved": round(ax_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ax_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_axhelm_bk8_split", "bytes_per_launch": ax_bytes,
-                     "us_per_launch": round(ax_s * 1e6, 3)},
-        "cpu_baseline": None,
+                     "us_per_launch": round(ax_s * 1e6, 3), "buffer_sets": ROOF_SETS,
+                     "frac_by_counter_bytes": (round(traffic / ax_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None)},
+        "cpu_baseline": cpu_baseline_record(cpu),
     }
-    if not args.no_cpu:
-        try:
-            out["cpu_baseline"] = cpu_baseline(N, NX, NX, NX, args.cpu_iters)
-        except Exception as e:  # the baseline is reported, never required
-            log(f"[bench] cpu baseline failed: {e}")
     print(json.dumps(out), flush=True)
 
 

@@ -167,7 +167,6 @@ int nbx_box_mesh_device(const nbx_box* box, int mask_boundary, double* ggeo, nbx
 int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
                                const double* x, const double* y, double* result_host, void* stream);
 int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream);
-
 /* Bandwidth probes of the bw benchmark (bw/main.cpp:59-140, bw/kernel/bwKernels.okl)
  * plus the copy and read streams: the measured HBM and LDS ceilings.  Device
  * arrays, 16-byte aligned (hipMalloc'd); asynchronous on `stream`.
@@ -260,7 +259,8 @@ int nbx_bp_pcg(nbx_bp* bp, const double* lambda, double* r, double* x, double to
  * stopping test stay FP64; then x += d.  Stops when |r|_w <= tol |r_0|_w
  * (true residual), after max_cycles cycles or maxit inner iterations in
  * total.  Returns the total inner iterations; stats->rdotr is the final
- * true residual. */
+ * true residual.  The singular all-Neumann problem (nbx_bp_all_neumann) is
+ * refused with NBX_ERR_UNSUPPORTED: its true residual need not reach tol. */
 int nbx_bp_pcg_mixed(nbx_bp* bp, const double* lambda, double* r, double* x, double tol, int maxit,
                      nbx_prec op_prec, double inner_tol, int max_cycles, nbx_pcg_stats* stats);
 /* ---- p-multigrid preconditioner (NBX_PRECON_MG) ---------------------------
@@ -288,6 +288,14 @@ int nbx_bp_mg_use_graph(nbx_bp* bp, int on);
 int nbx_bp_mg_info(const nbx_bp* bp, int level, int* N, double* lmax);
 /* z = M r, one V-cycle (device vectors), synchronous */
 int nbx_bp_mg_apply(nbx_bp* bp, const double* r, double* z);
+/* Learned multigrid (no counterpart in the reference): with on = 1 the fine
+ * level ends every V-cycle with one application of the network attached by
+ * nbx_bp_attach_net to the residual the cycle leaves there:
+ * rf = res - A d after post-smoothing, x += mask(gs(w invDiag (rf + s f(rf))))
+ * (the network's element output, assembled).  N = 7, one field.  Networks
+ * for this step are trained on pairs from nbx_net_generate_mg_pairs. */
+int nbx_bp_mg_set_learned(nbx_bp* bp, int on);
+int nbx_bp_mg_get_learned(const nbx_bp* bp);
 
 /* ======================= learned preconditioner ==========================
  * A per-element network over the 8^3 GLL nodes (N = 7, one field): inputs
@@ -322,6 +330,16 @@ int nbx_net_apply(nbx_bp* bp, const double* r, double* zl, int weighted);
 int nbx_net_generate_pairs(nbx_net* net, nbx_bp* bp, int nseeds, uint64_t seed0, const int* snapshots, int nsnap,
                            double tol_exact);
 int nbx_net_set_pairs(nbx_net* net, nbx_bp* bp, int npairs, const float* R, const float* E);
+/* training pairs for the learned multigrid step: for nseeds seeded right-hand
+ * sides b, x* from the multigrid PCG to tol_exact and its iterates x_k,
+ * k = kfirst .. kfirst+kcount-1; one V-cycle z = M (b - A x_k) leaves the
+ * pair (r' = b - A x_k - A z, e' = x* - x_k - z).  The handle needs precon
+ * MG (nbx_bp_mg_setup); returns the count */
+int nbx_net_generate_mg_pairs(nbx_net* net, nbx_bp* bp, int nseeds, uint64_t seed0, int kfirst, int kcount,
+                              double tol_exact);
+/* host copies (R, E may be NULL) of the stored pairs first .. first+count-1,
+ * local-vector length each */
+int nbx_net_copy_pairs(const nbx_net* net, int first, int count, float* R, float* E);
 /* loss sum_samples sum_n (z - e)^2 / sum_pair e^2 and its parameter gradient
  * over (pair, element) samples (host int pairs), chunk samples per workgroup.
  * G (device, local vector, may be NULL): when given, the loss is external and
@@ -338,6 +356,18 @@ int nbx_net_train(nbx_net* net, nbx_bp* bp, int steps, int batch, float lr, uint
  *           left by the best multiple of z) */
 int nbx_net_train_energy(nbx_net* net, nbx_bp* bp, int steps, float lr, uint64_t seed, int mode,
                          double* loss_hist);
+
+/* Phase timers of nekBone's PROFILING breakdown (nekBone/main.cpp:181-224,
+ * timer tags of nekBone/solve.cpp:71-109).  With profiling on, every PCG
+ * solve records events at its phase boundaries and accumulates device time
+ * per phase: seconds[0] local Ax (with the fused p = z + beta p and the pAp
+ * shares), [1] gs (gather-scatter + mask), [2] updatePCG (x, r update, and
+ * with COPY/JACOBI the preconditioner and the dots' tile partials, fused into
+ * one pass), [3] dot (the reductions' finishing passes; with MG/LEARNED also
+ * the dots), [4] preco (the V-cycle or the network with its assembly).
+ * The events add a little time to each phase.  reset != 0 zeroes the sums. */
+int nbx_bp_set_profiling(nbx_bp* bp, int on);
+int nbx_bp_profile(nbx_bp* bp, double* seconds, int reset);
 
 /* per-iteration history of the last solve (host outputs, length >= iterations+1):
  * rdotr and alpha per iteration; returns the count written */

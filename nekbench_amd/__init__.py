@@ -19,6 +19,7 @@ LIB_PATH = PKG / "libnbx.so"
 NBX_F64, NBX_F32, NBX_BF16, NBX_F16, NBX_BF16_MFMA = 0, 1, 2, 3, 4
 NBX_DOUBLE, NBX_FLOAT, NBX_INT, NBX_LONG = 0, 1, 2, 3
 NBX_ADD, NBX_MUL, NBX_MIN, NBX_MAX = 0, 1, 2, 3
+NBX_SUCCESS, NBX_ERR_INVALID, NBX_ERR_UNSUPPORTED, NBX_ERR_HIP, NBX_ERR_ALLOC, NBX_ERR_DIVERGED = 0, -1, -2, -3, -4, -5
 PRECON_COPY, PRECON_JACOBI, PRECON_LEARNED, PRECON_MG = 0, 1, 2, 3
 # reference ggeo factor order (core/params.h:55-62)
 GWJID, G00ID, G01ID, G02ID, G11ID, G12ID, G22ID, NGGEO = 0, 1, 2, 3, 4, 5, 6, 7
@@ -337,6 +338,13 @@ class BP:
     def mg_apply(self, r, z):
         check(lib().nbx_bp_mg_apply(self.h, ptr(r), ptr(z)), "nbx_bp_mg_apply")
 
+    def mg_set_learned(self, on=True):
+        """The fine level ends each V-cycle with the attached network's correction."""
+        check(lib().nbx_bp_mg_set_learned(self.h, int(on)), "nbx_bp_mg_set_learned")
+
+    def mg_learned(self) -> bool:
+        return bool(lib().nbx_bp_mg_get_learned(self.h))
+
     def make_rhs(self, seed, out):
         check(lib().nbx_bp_make_rhs(self.h, seed, ptr(out)), "nbx_bp_make_rhs")
 
@@ -401,6 +409,8 @@ _SIGS.update({
     "nbx_bp_mg_info": (_I, [_P, _I, _P, _P]),
     "nbx_bp_mg_apply": (_I, [_P, _P, _P]),
     "nbx_bp_mg_use_graph": (_I, [_P, _I]),
+    "nbx_bp_mg_set_learned": (_I, [_P, _I]),
+    "nbx_bp_mg_get_learned": (_I, [_P]),
     "nbx_bp_all_neumann": (_I, [_P]),
     "nbx_net_create": (_P, [C.c_uint64, _P]),
     "nbx_net_free": (None, [_P]),
@@ -413,6 +423,8 @@ _SIGS.update({
     "nbx_bp_update_jacobi": (_I, [_P]),
     "nbx_net_apply": (_I, [_P, _P, _P, _I]),
     "nbx_net_generate_pairs": (_I, [_P, _P, _I, C.c_uint64, _P, _I, _D]),
+    "nbx_net_generate_mg_pairs": (_I, [_P, _P, _I, C.c_uint64, _I, _I, _D]),
+    "nbx_net_copy_pairs": (_I, [_P, _I, _I, _P, _P]),
     "nbx_net_set_pairs": (_I, [_P, _P, _I, _P, _P]),
     "nbx_net_gradient": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "nbx_net_train": (_I, [_P, _P, _I, _I, C.c_float, C.c_uint64, _P]),
@@ -434,6 +446,7 @@ class Net:
         if not self.h:
             raise NbxError(f"nbx_net_create failed: {last_error()}")
         self.nparams = l.nbx_net_num_params()
+        self.npairs = 0
 
     @classmethod
     def load(cls, path, stream=None):
@@ -471,12 +484,33 @@ class Net:
         k = lib().nbx_net_generate_pairs(self.h, bp.h, nseeds, seed0, ptr(snaps), snaps.size, tol_exact)
         if k < 0:
             raise NbxError(f"nbx_net_generate_pairs failed: {last_error()}")
+        self.npairs = k
         return k
+
+    def generate_mg_pairs(self, bp: "BP", nseeds, seed0, kcount, kfirst=0, tol_exact=1e-11) -> int:
+        """(residual, error) pairs left by one V-cycle on the iterates
+        kfirst .. kfirst+kcount-1 of the multigrid PCG (training data of the
+        learned multigrid step)."""
+        k = lib().nbx_net_generate_mg_pairs(self.h, bp.h, nseeds, seed0, kfirst, kcount, tol_exact)
+        if k < 0:
+            raise NbxError(f"nbx_net_generate_mg_pairs failed: {last_error()}")
+        self.npairs = k
+        return k
+
+    def pairs(self, bp: "BP", first=0, count=None):
+        """Host copies (float32 arrays, one row per pair) of the stored pairs."""
+        if count is None:
+            count = self.npairs - first
+        R = np.zeros((count, bp.Nlocal), dtype=np.float32)
+        E = np.zeros_like(R)
+        check(lib().nbx_net_copy_pairs(self.h, first, count, ptr(R), ptr(E)), "nbx_net_copy_pairs")
+        return R, E
 
     def set_pairs(self, bp: "BP", R, E):
         """R, E: device float32 tensors of shape (npairs, local nodes)."""
         assert R.dim() == 2 and R.shape == E.shape
         check(lib().nbx_net_set_pairs(self.h, bp.h, R.shape[0], ptr(R), ptr(E)), "nbx_net_set_pairs")
+        self.npairs = R.shape[0]
 
     def gradient(self, bp: "BP", samples, chunk=16, G=None):
         s = np.ascontiguousarray(samples, dtype=np.int32).reshape(-1, 2)

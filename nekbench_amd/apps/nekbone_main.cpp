@@ -8,9 +8,12 @@
 // (default 3), NFIELDS (1 or 3), BPMODE, LAMBDA (default 1.1), PRECONDITIONER
 // (COPY, JACOBI), KRYLOV SOLVER (PCG; "FLEXIBLE" anywhere in the value selects
 // the flexible beta), SOLVER TOLERANCE (default 1e-8), NREPETITIONS (default
-// 10), MAXIMUM ITERATIONS, FIXED ITERATION COUNT, VERBOSE.  THREAD MODEL,
-// ARCH, PLATFORM/DEVICE NUMBER, KERNEL ID, OVERLAP, PROFILING and TIMER SYNC
-// are accepted and have no effect (one GPU, one rank, one kernel per case).
+// 10), MAXIMUM ITERATIONS, FIXED ITERATION COUNT, VERBOSE, PROFILING (TRUE
+// prints the reference's "breakdown" block after the summary: device time
+// per phase over the timed solves, nbx_bp_profile).  THREAD MODEL, ARCH,
+// PLATFORM/DEVICE NUMBER, KERNEL ID, OVERLAP and TIMER SYNC are accepted and
+// have no effect (one GPU, one rank, one kernel per case; the phase timers
+// always wait for the device).
 // Additions with no reference counterpart: PRECONDITIONER MULTIGRID with
 // [MULTIGRID ORDERS] (comma list, default N, N/2, 1), [MULTIGRID DEGREE]
 // (3), [MULTIGRID COARSE ITERATIONS] (20), [MULTIGRID PRECISION]
@@ -66,7 +69,7 @@ struct Settings {
   int bpmode = 0, BPid = 0;
   double lambda1 = 1.1;
   nbx_precon precon = NBX_PRECON_COPY;
-  int flexible = 0, verbose = 0, fixed = 0, fused = 1;
+  int flexible = 0, verbose = 0, fixed = 0, fused = 1, profiling = 0;
   double tol = 1e-8;
   int Ntests = 10, maxIterFixed = 1000;
   std::vector<int> mgOrders;
@@ -95,6 +98,7 @@ Settings resolve(const nbx_app::SetupIni& o)
   if (o.has("KRYLOV SOLVER") && !o.compare("KRYLOV SOLVER", "PCG")) s.error = "only the PCG Krylov solver exists";
   s.flexible = o.compare("KRYLOV SOLVER", "FLEXIBLE") ? 1 : 0;
   s.verbose = o.compare("VERBOSE", "TRUE") ? 1 : 0;
+  s.profiling = o.compare("PROFILING", "TRUE") ? 1 : 0;
   o.get("SOLVER TOLERANCE", s.tol);
   o.get("NREPETITIONS", s.Ntests);
   o.get("MAXIMUM ITERATIONS", s.maxIterFixed);
@@ -259,6 +263,10 @@ int main(int argc, char** argv)
   }
 
   if (s.fixed) tol = 0;
+  if (s.profiling) {  // phase timers over the timed solves (nekBone/main.cpp:181-193)
+    int rc = nbx_bp_set_profiling(bp, 1);
+    if (rc < 0) die("nbx_bp_set_profiling", rc);
+  }
   std::printf("\nrunning solver ...");
   std::fflush(stdout);
   double elapsed = 0;
@@ -304,6 +312,17 @@ int main(int argc, char** argv)
   std::printf("  throughput   : %g GDOF/s/iter\n", gDOFs);
   std::printf("  bandwidth    : %g GB/s\n", bw);
   std::printf("  GFLOPS/s     : %g\n", gFlops);
+  if (s.profiling) {
+    double etime[5];
+    const int rc = nbx_bp_profile(bp, etime, 0);
+    if (rc < 0) die("nbx_bp_profile", rc);
+    std::printf("\nbreakdown\n");
+    std::printf("  local Ax  : %g s\n", etime[0]);
+    std::printf("  gs        : %g s\n", etime[1]);
+    std::printf("  updatePCG : %g s\n", etime[2]);
+    std::printf("  dot       : %g s\n", etime[3]);
+    std::printf("  preco     : %g s\n\n", etime[4]);
+  }
 
   hipFree(r);
   hipFree(x);

@@ -89,6 +89,7 @@ def build(verbose: bool = True) -> Path:
     if verbose:
         print(f"[nbx build] linked {LIB}", flush=True)
     build_apps(verbose)
+    build_native(verbose)
     return LIB
 
 
@@ -112,6 +113,32 @@ def build_apps(verbose: bool = True) -> None:
         os.replace(str(out) + ".tmp", out)
         if verbose:
             print(f"[nbx build] {out}", flush=True)
+
+
+NATIVE = PKG / "native"
+NATIVE_ORDERS = range(1, 16)
+
+
+def build_native(verbose: bool = True) -> None:
+    """native/libaxhelm_N<N>.so for N = 1..15: the reference's native axhelm
+    symbols (axhelm_bk_v0, ...) over libnbx.so, one library per order as the
+    reference compiles its CPU kernel per p_Nq."""
+    NATIVE.mkdir(exist_ok=True)
+    src = APPS / "nbx_native_axhelm.cpp"
+    procs = []
+    for n in NATIVE_ORDERS:
+        out = NATIVE / f"libaxhelm_N{n}.so"
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", f"-DNBX_NATIVE_N={n}", f"-I{INCLUDE}",
+               str(src), "-o", str(out) + ".tmp", f"-L{PKG}", "-lnbx", "-Wl,-rpath,$ORIGIN/..",
+               f"-Wl,-rpath,{ROCM}/lib"]
+        procs.append((out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    for out, p in procs:
+        text, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"{out.name} build failed:\n" + text.decode(errors="replace"))
+        os.replace(str(out) + ".tmp", out)
+    if verbose:
+        print(f"[nbx build] {NATIVE}/libaxhelm_N1..15.so", flush=True)
 
 
 def build_oracle(verbose: bool = True) -> None:

@@ -28,12 +28,7 @@ namespace nbx {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short f2bf(float x)
-{
-  uint32_t u = __float_as_uint(x);
-  u = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite values)
-  return (short)(u >> 16);
-}
+__device__ __forceinline__ short f2bf(float x) { return (short)bf16_bits(x); }
 
 // matrix-core operand types: storage element, 4-vector, rounding, product
 struct OpBf16 {

@@ -56,10 +56,19 @@ struct nbx_bp {
   nbx_net* net = nullptr;
   nbx_mg* mg = nullptr;     // p-multigrid preconditioner (levels own their handles)
   float* feat = nullptr;    // normalised geometric factors [E][7][Np]
+  uint16_t* feat16 = nullptr;  // the same as bf16 (matrix-core inference)
   double* zl = nullptr;     // element-local preconditioner output
   int stall_window = 20;    // iterations compared for the stagnation test
   double stall_ratio = 0.9; // rdotr(it) > ratio * rdotr(it - window) -> Jacobi from then on
   std::vector<void*> owned;
+  // phase timers of nekBone's PROFILING breakdown (nekBone/main.cpp:181-224):
+  // events recorded at phase boundaries, each interval credited to the phase
+  // that ends there; accumulated at every host wait of the solve
+  bool prof = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_tag;
+  int prof_used = 0;
+  double prof_t[5] = {0, 0, 0, 0, 0};  // Ax, gs, updatePCG, dot, preco
 
   int alloc(double** ptr, int64_t n)
   {
@@ -70,6 +79,15 @@ struct nbx_bp {
   }
 };
 
+
+namespace nbx {
+enum ProfPhase { kProfAx = 0, kProfGs = 1, kProfUpdate = 2, kProfDot = 3, kProfPreco = 4, kProfNone = -1 };
+// record a phase boundary (no-op unless profiling); the interval since the
+// previous boundary is credited to `phase`
+void prof_mark(nbx_bp* bp, int phase);
+// accumulate the recorded intervals (the stream must have reached the last mark)
+int prof_collect(nbx_bp* bp);
+}  // namespace nbx
 
 // FP64 PCG on the handle's operator with an explicit preconditioner choice
 // (used to generate training data); x is updated in place, r becomes the
@@ -94,7 +112,11 @@ int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallb
 // forget a handle that is being freed (the network keeps a list of the
 // handles it is attached to and clears their pointer when it is freed first)
 void net_detach(nbx_net* net, nbx_bp* bp);
+// inference kernel of a network (0 FP32, 1 matrix cores; -1 for none)
+int net_inference_mode(const nbx_net* net);
 // one multigrid V-cycle z = M r on the handle's stream (kernels skip when *skip)
 int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip);
+// the same V-cycle launched kernel by kernel (no graph capture or replay)
+int mg_apply_direct(nbx_bp* bp, const double* r, double* z);
 void mg_release(nbx_mg* mg);
 }  // namespace nbx

@@ -37,6 +37,23 @@ __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_loa
 template <typename T>
 __device__ __forceinline__ void stnt(T v, T* p) { __builtin_nontemporal_store(v, p); }
 
+// float -> bf16 bits (the upper half of the float) by round to nearest, ties
+// to even: adding 0x7FFF (+1 when the kept part is odd) carries into bit 16
+// exactly when the dropped half lies above the midpoint, or on it with an odd
+// kept part; past the largest finite bf16 this carries into infinity.  A NaN
+// stays a NaN (sign and leading payload kept, quiet bit set).
+__host__ __device__ __forceinline__ uint16_t bf16_bits(float x)
+{
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+// the same rounding, as a float value
+__host__ __device__ __forceinline__ float bf16_round(float x)
+{
+  return __builtin_bit_cast(float, (uint32_t)bf16_bits(x) << 16);
+}
+
 // Device scalar block of the PCG driver.  Kernels outside the driver only
 // read `done` (skip all work once the solve has ended) and `beta`.
 struct PcgFlags {

@@ -12,6 +12,7 @@
 struct nbx_net {
   float* master = nullptr;  // FP32 master weights
   float* wbf = nullptr;     // bf16-rounded copy used by every forward
+  void* frag = nullptr;     // matrix-core fragments of wbf (rebuilt after every weight change)
   float* m = nullptr;
   float* v = nullptr;
   float* grad = nullptr;
@@ -48,15 +49,6 @@ __host__ __device__ uint64_t mix64(uint64_t x)
   return x ^ (x >> 31);
 }
 
-float bf16_round(float x)
-{
-  uint32_t u;
-  std::memcpy(&u, &x, 4);
-  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-  float r;
-  std::memcpy(&r, &u, 4);
-  return r;
-}
 
 // Initial weights: uniform(-a, a) with a = 1/sqrt(fan_in) for Wr, Wg, W2
 // (seeded splitmix64), zero biases, and a zero output layer so that the
@@ -81,6 +73,9 @@ int upload_params(nbx_net* net, const float* host)
   for (int q = 0; q < kNetParams; ++q) bf[q] = bf16_round(host[q]);
   NBX_HIP(hipMemcpy(net->master, host, sizeof(float) * kNetParams, hipMemcpyHostToDevice));
   NBX_HIP(hipMemcpy(net->wbf, bf.data(), sizeof(float) * kNetParams, hipMemcpyHostToDevice));
+  int rc = net_frag_launch(net->wbf, net->frag, nullptr);
+  if (rc) return rc;
+  NBX_HIP(hipDeviceSynchronize());
   NBX_HIP(hipMemset(net->m, 0, sizeof(float) * kNetParams));
   NBX_HIP(hipMemset(net->v, 0, sizeof(float) * kNetParams));
   net->steps = 0;
@@ -92,7 +87,7 @@ void net_release(nbx_net* n)
   if (!n) return;
   (void)hipDeviceSynchronize();
   for (nbx_bp* bp : n->attached) bp->net = nullptr;  // LEARNED solves now fail with an error
-  for (void* p : {(void*)n->master, (void*)n->wbf, (void*)n->m, (void*)n->v, (void*)n->grad, (void*)n->loss,
+  for (void* p : {(void*)n->master, (void*)n->wbf, (void*)n->frag, (void*)n->m, (void*)n->v, (void*)n->grad, (void*)n->loss,
                   (void*)n->partials, (void*)n->samples, (void*)n->R, (void*)n->Ev, (void*)n->invNorm,
                   (void*)n->work, (void*)n->invNormE})
     if (p) (void)hipFree(p);
@@ -221,6 +216,8 @@ __global__ __launch_bounds__(256) void k_draw_samples(const int32_t batch, const
 
 namespace nbx {
 
+int net_inference_mode(const nbx_net* net) { return net ? net->infer_mfma : -1; }
+
 void net_detach(nbx_net* net, nbx_bp* bp)
 {
   auto& a = net->attached;
@@ -235,8 +232,8 @@ int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallb
 {
   if (!bp->net || !bp->feat) return fail(NBX_ERR_INVALID, "learned preconditioner: no network attached");
   if (bp->net->infer_mfma)
-    return net_apply_mma_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, nbx_ogs_inv_degree(bp->ogs), out,
-                                fallback, skip, bp->stream);
+    return net_apply_mma_launch(bp->E, bp->net->frag, bp->net->wbf, bp->feat16, r, bp->invDiagA,
+                                nbx_ogs_inv_degree(bp->ogs), ogs_degree(bp->ogs), out, fallback, skip, bp->stream);
   return net_apply_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, nbx_ogs_inv_degree(bp->ogs), out, fallback,
                           skip, bp->stream);
 }
@@ -254,6 +251,7 @@ extern "C" nbx_net* nbx_net_create(uint64_t seed, void* stream)
   n->stream = as_stream(stream);
   if (hipMalloc(&n->master, sizeof(float) * kNetParams) != hipSuccess ||
       hipMalloc(&n->wbf, sizeof(float) * kNetParams) != hipSuccess ||
+      hipMalloc(&n->frag, kNetFragBytes) != hipSuccess ||
       hipMalloc(&n->m, sizeof(float) * kNetParams) != hipSuccess ||
       hipMalloc(&n->v, sizeof(float) * kNetParams) != hipSuccess ||
       hipMalloc(&n->grad, sizeof(float) * kNetParams) != hipSuccess || hipMalloc(&n->loss, sizeof(double)) != hipSuccess) {
@@ -361,6 +359,8 @@ extern "C" int nbx_bp_attach_net(nbx_bp* bp, nbx_net* net)
   NBX_HIP(hipMemcpyAsync(d_s + 7, inv, sizeof(inv), hipMemcpyHostToDevice, bp->stream));
   int rc = net_features_launch(bp->E, bp->ggeo, d_s + 7, bp->feat, bp->stream);
   if (rc) return rc;
+  if (!bp->feat16) NBX_HIP(hipMalloc(&bp->feat16, sizeof(uint16_t) * (size_t)bp->E * 7 * bp->Np));
+  if ((rc = net_features16_launch((int64_t)bp->E * 7 * bp->Np, bp->feat, bp->feat16, bp->stream))) return rc;
   NBX_HIP(hipStreamSynchronize(bp->stream));
   (void)hipFree(d_s);
   if (bp->net && bp->net != net) nbx::net_detach(bp->net, bp);
@@ -384,7 +384,8 @@ extern "C" int nbx_net_apply(nbx_bp* bp, const double* r, double* zl, int weight
   if (!bp || !r || !zl || !bp->net) return fail(NBX_ERR_INVALID, "net_apply: bad arguments / no network");
   const double* w = weighted ? nbx_ogs_inv_degree(bp->ogs) : nullptr;
   int rc = bp->net->infer_mfma
-               ? net_apply_mma_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, w, zl, nullptr, nullptr, bp->stream)
+               ? net_apply_mma_launch(bp->E, bp->net->frag, bp->net->wbf, bp->feat16, r, bp->invDiagA, w,
+                                      weighted ? ogs_degree(bp->ogs) : nullptr, zl, nullptr, nullptr, bp->stream)
                : net_apply_launch(bp->E, bp->net->wbf, bp->feat, r, bp->invDiagA, w, zl, nullptr, nullptr, bp->stream);
   if (rc) return rc;
   NBX_HIP(hipStreamSynchronize(bp->stream));
@@ -466,6 +467,80 @@ extern "C" int nbx_net_generate_pairs(nbx_net* n, nbx_bp* bp, int nseeds, uint64
   n->npairs = k;
   n->energy_pairs = 0;
   return k;
+}
+
+// Training pairs for the learned multigrid step (nbx_bp_mg_set_learned): the
+// residual and the error a V-cycle leaves behind.  For each seed, x* solves
+// A x = b_seed by the multigrid PCG to tol_exact; for k = kfirst ..
+// kfirst+kcount-1 the k-th iterate x_k of the same PCG gives r_k = b - A x_k
+// and e_k = x* - x_k; one V-cycle without the learned step, z = M r_k, then
+// leaves
+//   r' = r_k - A z,   e' = e_k - z     (A e' = r'),
+// stored as a float pair.  Returns the number of pairs.
+extern "C" int nbx_net_generate_mg_pairs(nbx_net* n, nbx_bp* bp, int nseeds, uint64_t seed0, int kfirst,
+                                         int kcount, double tol_exact)
+{
+  if (!n || !bp || nseeds < 1 || kfirst < 0 || kcount < 1 || bp->Nf != 1 || !(tol_exact > 0))
+    return fail(NBX_ERR_INVALID, "net_generate_mg_pairs: bad arguments");
+  const int maxk = kcount;
+  if (bp->opt.precon != NBX_PRECON_MG || !bp->mg)
+    return fail(NBX_ERR_INVALID, "net_generate_mg_pairs: the handle needs precon MG and nbx_bp_mg_setup");
+  const int total = nseeds * maxk;
+  int rc = ensure_pairs(n, total, bp->Nlocal);
+  if (rc) return rc;
+  const int was_learned = nbx_bp_mg_get_learned(bp);
+  if (was_learned && (rc = nbx_bp_mg_set_learned(bp, 0))) return rc;
+  double *b = nullptr, *xs = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *Az = nullptr;
+  const size_t bytes = sizeof(double) * bp->Nall;
+  NBX_HIP(hipMalloc(&b, bytes));
+  NBX_HIP(hipMalloc(&xs, bytes));
+  NBX_HIP(hipMalloc(&r, bytes));
+  NBX_HIP(hipMalloc(&x, bytes));
+  NBX_HIP(hipMalloc(&z, bytes));
+  NBX_HIP(hipMalloc(&Az, bytes));
+  const dim3 g((unsigned)((bp->Nlocal + 255) / 256));
+  int k = 0;
+  for (int sd = 0; sd < nseeds && rc == NBX_SUCCESS; ++sd) {
+    if ((rc = bp_make_rhs(bp, seed0 + (uint64_t)sd, b))) break;
+    NBX_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, bp->stream));
+    NBX_HIP(hipMemsetAsync(xs, 0, bytes, bp->stream));
+    if ((rc = bp_solve_plain(bp, NBX_PRECON_MG, r, xs, tol_exact, 1000, nullptr))) break;
+    for (int j = kfirst; j < kfirst + maxk && rc == NBX_SUCCESS; ++j) {
+      NBX_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, bp->stream));
+      NBX_HIP(hipMemsetAsync(x, 0, bytes, bp->stream));
+      if (j > 0 && (rc = bp_solve_plain(bp, NBX_PRECON_MG, r, x, 0.0, j, nullptr))) break;
+      // one V-cycle on r_k, then the residual and error it leaves
+      if ((rc = mg_apply_direct(bp, r, z)) || (rc = bp_apply(bp, bp->lambda, z, Az, nullptr)) ||
+          (rc = nbx_scaled_add((nbx_dlong)bp->Nall, -1.0, Az, 1.0, r, bp->stream)) ||
+          (rc = nbx_scaled_add((nbx_dlong)bp->Nall, 1.0, z, 1.0, x, bp->stream)))
+        break;
+      float* Rk = n->R + (int64_t)k * n->pairStride;
+      float* Ek = n->Ev + (int64_t)k * n->pairStride;
+      hipLaunchKernelGGL(k_to_float_diff, g, dim3(256), 0, bp->stream, (int64_t)bp->Nlocal, r, nullptr, Rk);
+      hipLaunchKernelGGL(k_to_float_diff, g, dim3(256), 0, bp->stream, (int64_t)bp->Nlocal, xs, x, Ek);
+      hipLaunchKernelGGL(k_sumsq_f, dim3(1), dim3(256), 0, bp->stream, (int64_t)bp->Nlocal, Ek, n->invNorm + k);
+      NBX_HIP_LAUNCH();
+      ++k;
+    }
+  }
+  (void)hipStreamSynchronize(bp->stream);
+  for (void* p : {(void*)b, (void*)xs, (void*)r, (void*)x, (void*)z, (void*)Az}) (void)hipFree(p);
+  if (was_learned) (void)nbx_bp_mg_set_learned(bp, 1);
+  if (rc) return rc;
+  n->npairs = k;
+  n->energy_pairs = 0;
+  return k;
+}
+
+// Host copies of stored pairs first .. first+count-1 (Nlocal floats each).
+extern "C" int nbx_net_copy_pairs(const nbx_net* n, int first, int count, float* R, float* Ev)
+{
+  if (!n || first < 0 || count < 0 || first + count > n->npairs)
+    return fail(NBX_ERR_INVALID, "net_copy_pairs: pairs %d..%d of %d", first, first + count - 1, n ? n->npairs : 0);
+  const size_t off = (size_t)first * n->pairStride, bytes = sizeof(float) * (size_t)count * n->pairStride;
+  if (R) NBX_HIP(hipMemcpy(R, n->R + off, bytes, hipMemcpyDeviceToHost));
+  if (Ev) NBX_HIP(hipMemcpy(Ev, n->Ev + off, bytes, hipMemcpyDeviceToHost));
+  return NBX_SUCCESS;
 }
 
 // Store caller-provided pairs (device float vectors of length Nlocal each).
@@ -552,6 +627,7 @@ extern "C" int nbx_net_train(nbx_net* n, nbx_bp* bp, int steps, int batch, float
     if ((rc = run_gradient(n, bp, batch, chunk))) return rc;
     n->steps += 1;
     rc = net_adam_launch(n->master, n->m, n->v, n->grad, n->wbf, lr, n->steps, bp->stream);
+    if (rc == NBX_SUCCESS) rc = net_frag_launch(n->wbf, n->frag, bp->stream);
     if (rc) return rc;
     if (loss_hist) {
       double l;
@@ -645,6 +721,7 @@ extern "C" int nbx_net_train_energy(nbx_net* n, nbx_bp* bp, int steps, float lr,
       NBX_HIP(hipMemcpyAsync(n->loss, bp->red.result + 4, sizeof(double), hipMemcpyDeviceToDevice, bp->stream));
     n->steps += 1;
     if ((rc = net_adam_launch(n->master, n->m, n->v, n->grad, n->wbf, lr, n->steps, bp->stream))) return rc;
+    if ((rc = net_frag_launch(n->wbf, n->frag, bp->stream))) return rc;
     if (loss_hist) {
       double l;
       NBX_HIP(hipMemcpyAsync(&l, n->loss, sizeof(double), hipMemcpyDeviceToHost, bp->stream));

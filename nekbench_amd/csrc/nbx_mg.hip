@@ -26,6 +26,13 @@
 // outer solver's done flag so the iterations enqueued past convergence cost
 // nothing.
 //
+// Learned step (nbx_bp_mg_set_learned, a network attached with
+// nbx_bp_attach_net): the fine level ends its cycle with one application of
+// the element network to the residual the cycle leaves there,
+//   rf = res - A d  (the fine residual after post-smoothing),
+//   x += mask(gs(w invDiag (rf + s f(rf)))),
+// a learned local correction trained on the error V-cycles leave behind.
+//
 // Precision: NBX_F64 runs the whole cycle in FP64.  NBX_F32 and NBX_BF16 run
 // the smoothers and transfers of every level but the last in FP32 (vectors,
 // Jacobi diagonal, interpolation and arithmetic), with the operator's ggeo and
@@ -34,6 +41,7 @@
 #include "nbx_bp_internal.h"
 #include "nbx_blas.h"
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 struct nbx_mg_level {
@@ -56,6 +64,9 @@ struct nbx_mg {
   double lo = 0.1, hi = 1.1;
   nbx_prec prec = NBX_F64;
   nbx::PcgScalars* coarse_template = nullptr;  // device copy of the coarse solve's initial scalars
+  bool learned = false;        // fine level ends with the attached network's correction
+  double* lrf = nullptr;       // learned step: the fine residual after the cycle (FP64 copy)
+  double* lzl = nullptr;       // learned step: element-local network output
   std::vector<void*> owned;
   // the cycle as a HIP graph, captured for one (r, z, skip) and replayed
   bool use_graph = true;
@@ -65,6 +76,8 @@ struct nbx_mg {
   const void* g_r = nullptr;
   void* g_z = nullptr;
   const int* g_skip = nullptr;
+  const void* g_net = nullptr;  // the learned step's network and inference mode at capture:
+  int g_infer = -1;             // the captured kernels depend on both
 };
 
 namespace nbx {
@@ -197,6 +210,25 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc
     d[id] = acc;
     x[id] = x[id] + (X)acc;
   }
+}
+
+// learned step: rf = res - Ad as FP64 (the network's input)
+template <typename V>
+__global__ __launch_bounds__(256) void k_mg_resid(const int64_t n, const V* __restrict__ res, const V* __restrict__ Ad,
+                                                  double* __restrict__ rf, const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rf[i] = (double)(res[i] - Ad[i]);
+}
+
+// learned step: x += z (assembled network output)
+__global__ __launch_bounds__(256) void k_mg_add(const int64_t n, const double* __restrict__ z, double* __restrict__ x,
+                                                const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + z[i];
 }
 
 __global__ __launch_bounds__(256) void k_mg_to_f32(const int64_t n, const double* __restrict__ a,
@@ -414,6 +446,19 @@ int vcycle_level(nbx_mg* mg, int l, const R* r, X* x, const int* skip)
                        (const R*)nullptr, res, Ad, d, x, skip);
   }
   NBX_HIP_LAUNCH();
+  if constexpr (std::is_same<X, double>::value) {  // the fine level (its correction is FP64)
+  if (l == 0 && mg->learned) {
+    // learned step: rf = res - A d; x += mask(gs(w invDiag (rf + s f(rf))))
+    if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
+    const dim3 gn((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL((k_mg_resid<V>), gn, b, 0, bp->stream, (int64_t)n, res, Ad, mg->lrf, skip);
+    NBX_HIP_LAUNCH();
+    if ((rc = net_apply_learned(bp, mg->lrf, mg->lzl, nullptr, skip))) return rc;
+    if ((rc = gs_add_mask_launch(bp->ogs, mg->lzl, 1, off, bp->Nmasked, bp->maskIds, skip, bp->stream))) return rc;
+    hipLaunchKernelGGL(k_mg_add, gn, b, 0, bp->stream, (int64_t)n, mg->lzl, x, skip);
+    NBX_HIP_LAUNCH();
+  }
+  }
   return NBX_SUCCESS;
 }
 
@@ -448,6 +493,8 @@ static void mg_drop_graph(nbx_mg* mg)
   mg->g_r = nullptr;
   mg->g_z = nullptr;
   mg->g_skip = nullptr;
+  mg->g_net = nullptr;
+  mg->g_infer = -1;
 }
 
 void mg_release(nbx_mg* mg)
@@ -500,8 +547,15 @@ static int mg_capture(nbx_mg* mg, nbx_bp* bp, const double* r, double* z, const 
   mg->g_r = r;
   mg->g_z = z;
   mg->g_skip = skip;
-  (void)bp;
+  mg->g_net = mg->learned ? bp->net : nullptr;
+  mg->g_infer = mg->learned ? net_inference_mode(bp->net) : -1;
   return NBX_SUCCESS;
+}
+
+int mg_apply_direct(nbx_bp* bp, const double* r, double* z)
+{
+  if (!bp->mg) return fail(NBX_ERR_INVALID, "multigrid preconditioner: not set up (nbx_bp_mg_setup)");
+  return mg_enqueue(bp->mg, r, z, nullptr);
 }
 
 int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip)
@@ -509,7 +563,8 @@ int mg_apply(nbx_bp* bp, const double* r, double* z, const int* skip)
   nbx_mg* mg = bp->mg;
   if (!mg) return fail(NBX_ERR_INVALID, "multigrid preconditioner: not set up (nbx_bp_mg_setup)");
   if (!mg->use_graph) return mg_enqueue(mg, r, z, skip);
-  if (!mg->exec || mg->g_r != r || mg->g_z != z || mg->g_skip != skip) {
+  const bool net_changed = mg->learned && (mg->g_net != bp->net || mg->g_infer != net_inference_mode(bp->net));
+  if (!mg->exec || mg->g_r != r || mg->g_z != z || mg->g_skip != skip || net_changed) {
     int rc = mg_capture(mg, bp, r, z, skip);
     if (rc) return rc;
   }
@@ -646,6 +701,28 @@ extern "C" int nbx_bp_mg_use_graph(nbx_bp* bp, int on)
   if (!on) mg_drop_graph(bp->mg);
   return NBX_SUCCESS;
 }
+
+extern "C" int nbx_bp_mg_set_learned(nbx_bp* bp, int on)
+{
+  if (!bp || !bp->mg) return fail(NBX_ERR_INVALID, "bp_mg_set_learned: no multigrid set up");
+  nbx_mg* mg = bp->mg;
+  (void)hipStreamSynchronize(bp->stream);
+  if (on) {
+    if (!bp->net || !bp->feat)
+      return fail(NBX_ERR_INVALID, "bp_mg_set_learned: no network attached (nbx_bp_attach_net)");
+    if (bp->N != 7 || bp->Nf != 1)
+      return fail(NBX_ERR_UNSUPPORTED, "bp_mg_set_learned: the network is built for N = 7, one field");
+    if (!mg->lrf) {
+      int rc;
+      if ((rc = mg_alloc(mg, &mg->lrf, bp->Nall)) || (rc = mg_alloc(mg, &mg->lzl, bp->Nall))) return rc;
+    }
+  }
+  if (mg->learned != (on != 0)) mg_drop_graph(mg);  // the cycle changes
+  mg->learned = on != 0;
+  return NBX_SUCCESS;
+}
+
+extern "C" int nbx_bp_mg_get_learned(const nbx_bp* bp) { return bp && bp->mg && bp->mg->learned ? 1 : 0; }
 
 extern "C" int nbx_bp_mg_info(const nbx_bp* bp, int level, int* N, double* lmax)
 {

@@ -14,9 +14,16 @@ constexpr int kNetPartial = 3044;       // floats per workgroup partial
 
 int net_apply_launch(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
                      const double* w, double* zl, const int* fallback, const int* skip, hipStream_t s);
-// the same on the matrix cores (bf16 operands, FP32 accumulation; nbx_net_mma.hip)
-int net_apply_mma_launch(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
-                         const double* w, double* zl, const int* fallback, const int* skip, hipStream_t s);
+// the same on the matrix cores (bf16 operands, FP32 accumulation;
+// nbx_net_mma.hip): frag = the weight fragments of net_frag_launch, W the
+// bf16-valued weights (for b3), feat16 the bf16 features; the assembly
+// weight is 1/deg (when deg is given, equal to w bit for bit) or w
+constexpr int kNetFragBytes = 64 * 176;
+int net_frag_launch(const float* W, void* frag, hipStream_t s);
+int net_features16_launch(int64_t n, const float* feat, uint16_t* feat16, hipStream_t s);
+int net_apply_mma_launch(int32_t E, const void* frag, const float* W, const uint16_t* feat16, const double* r,
+                         const double* invDiag, const double* w, const uint8_t* deg, double* zl,
+                         const int* fallback, const int* skip, hipStream_t s);
 size_t net_train_lds_bytes();
 int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64_t pairStride, const float* R,
                      const float* Ev, const double* invNorm, const float* W, const float* feat, const double* invDiag,

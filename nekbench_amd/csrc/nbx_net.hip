@@ -366,12 +366,7 @@ __global__ __launch_bounds__(256) void k_net_grad_sum(const int32_t nblocks, con
   }
 }
 
-__device__ __forceinline__ float round_bf16(float x)
-{
-  uint32_t u = __float_as_uint(x);
-  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-  return __uint_as_float(u);
-}
+__device__ __forceinline__ float round_bf16(float x) { return bf16_round(x); }
 
 // Adam on FP32 master weights; the forward weights are their bf16 rounding
 __global__ __launch_bounds__(256) void k_net_adam(const int32_t np, float* __restrict__ master,

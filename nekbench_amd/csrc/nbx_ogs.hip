@@ -136,11 +136,18 @@ __global__ __launch_bounds__(256) void k_gs_fixed(const int32_t Ngroups, const i
 // launch.  Every class list (pairs, quads, octets, CSR groups, masked nodes)
 // is cut into 8 equal chunks, chunk x of every class runs on XCD x (blocks
 // b with b % 8 == x), so one L2 sees the same element range for every class.
-// Masked nodes belong to no group: mask writes never touch a gathered value.
+// Each chunk is cut again into S equal slices, and an XCD runs slice s of
+// every class before slice s + 1 of any: the class lists are ordered by
+// their first member, so equal fractions cover about the same elements, and
+// the lines a slice's pairs touch are still in L2 when its edges and corners
+// touch them.  Masked nodes belong to no group: mask writes never touch a
+// gathered value.  The order of groups does not change any group's sum.
 struct GsClasses {
   int32_t n[5];          // pairs, quads, octets, CSR groups, masked nodes
   int32_t per[5];        // items per XCD chunk
-  int32_t bstart[6];     // cumulative blocks per XCD
+  int32_t ps[5];         // items per slice of a chunk
+  int32_t bstart[6];     // cumulative blocks of one slice, by class
+  int32_t nslice;        // slices per chunk
   const int32_t* ids[5]; // members (pairs/quads/octets), CSR ids, mask ids
   const int32_t* off;    // CSR offsets
 };
@@ -151,11 +158,13 @@ __global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const in
 {
   if (skip && *skip) return;
   const int32_t xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+  const int32_t slice = local / C.bstart[5], lb = local - slice * C.bstart[5];
   int c = 0;
-  while (c < 5 && local >= C.bstart[c + 1]) ++c;
-  if (c == 5) return;
-  const int32_t item = xcd * C.per[c] + (local - C.bstart[c]) * 256 + (int32_t)threadIdx.x;
-  if (item >= min(C.n[c], (xcd + 1) * C.per[c])) return;
+  while (c < 5 && lb >= C.bstart[c + 1]) ++c;
+  if (c == 5 || slice >= C.nslice) return;
+  const int32_t s0 = slice * C.ps[c];
+  const int32_t item = xcd * C.per[c] + s0 + (lb - C.bstart[c]) * 256 + (int32_t)threadIdx.x;
+  if (item >= min(C.n[c], xcd * C.per[c] + min(C.per[c], s0 + C.ps[c]))) return;
   if (c == 4) {
     const int32_t id = C.ids[4][item];
     for (int f = 0; f < k; ++f) q[id + (int64_t)f * stride] = T(0);
@@ -257,16 +266,26 @@ static int gs_add_mask_launch_t(const nbx_ogs* o, T* v, int k, int32_t stride, i
   GsClasses C{};
   const int32_t n[5] = {o->Nfix[0], o->Nfix[1], o->Nfix[2], o->Ngen, Nmasked};
   const int32_t* ids[5] = {o->d_fix[0], o->d_fix[1], o->d_fix[2], o->d_ids, maskIds};
-  C.bstart[0] = 0;
+  int32_t items = 0;
   for (int c = 0; c < 5; ++c) {
     C.n[c] = n[c];
     C.per[c] = (n[c] + 7) / 8;
     C.ids[c] = ids[c];
-    C.bstart[c + 1] = C.bstart[c] + (C.per[c] + 255) / 256;
+    items += C.per[c];
   }
-  C.off = o->d_offsets;
-  if (C.bstart[5] == 0) return NBX_SUCCESS;
-  hipLaunchKernelGGL(k_gs_add_mask<T>, dim3((unsigned)(8 * C.bstart[5])), dim3(256), 0, s, C, k, stride, v, skip);
+  if (items == 0) return NBX_SUCCESS;
+  // slices of about kGsSliceItems items of an XCD's chunk: with pairs of
+  // one-field N = 7 elements about 128 elements, whose 512 KB of values the
+  // XCD's 4 MB L2 holds while the slice's blocks run
+  constexpr int32_t kGsSliceItems = 16384;
+  C.nslice = std::max<int32_t>(1, items / kGsSliceItems);
+  C.bstart[0] = 0;
+  for (int c = 0; c < 5; ++c) {
+    C.ps[c] = (C.per[c] + C.nslice - 1) / C.nslice;
+    C.bstart[c + 1] = C.bstart[c] + (C.ps[c] + 255) / 256;
+  }
+  const int64_t blocks = 8 * (int64_t)C.nslice * C.bstart[5];
+  hipLaunchKernelGGL(k_gs_add_mask<T>, dim3((unsigned)blocks), dim3(256), 0, s, C, k, stride, v, skip);
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
 }

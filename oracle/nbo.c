@@ -705,13 +705,26 @@ int nbo_max_threads(void) { return omp_get_max_threads(); }
  * specified, so agreement is to FP32 rounding, not bit for bit); geometric
  * factors applied in FP32 with the FP32 kernel's fma order; direction
  * results summed r, then s, then t. */
-static float bf16r(float x)
+float nbo_bf16r(float x)
 {
   union { float f; uint32_t u; } a;
   a.f = x;
+  if ((a.u & 0x7FFFFFFFu) > 0x7F800000u) { /* NaN: keep sign and leading payload, set the quiet bit */
+    a.u = (a.u | 0x00400000u) & 0xFFFF0000u;
+    return a.f;
+  }
+  /* adding 0x7FFF (+1 when the kept part is odd) carries into bit 16 exactly
+   * when the dropped half is above the midpoint, or at it with an odd kept part */
   a.u = (a.u + 0x7FFFu + ((a.u >> 16) & 1u)) & 0xFFFF0000u;
   return a.f;
 }
+
+void nbo_bf16_round_array(int64_t n, const float* x, float* y)
+{
+  for (int64_t i = 0; i < n; ++i) y[i] = nbo_bf16r(x[i]);
+}
+
+static float bf16r(float x) { return nbo_bf16r(x); }
 
 /* IEEE binary16 rounding (nearest even) of a float, returned as a float:
  * 10 fraction bits, normal range from 2^-14, subnormal quantum 2^-24,

@@ -88,6 +88,11 @@ void nbo_axhelm_f16mma(int bk, int32_t E, int32_t offset, const float* ggeo, con
                        const float* q, float* Aq);
 /* IEEE binary16 rounding (nearest even, subnormals kept) of each value */
 void nbo_f16_round_array(int64_t n, const float* x, float* y);
+/* bf16 rounding of a float (the value of its upper 16 bits after round to
+ * nearest, ties to even; past the largest finite bf16 -> infinity; a NaN stays
+ * a NaN, quieted).  Every bf16 operand of the oracle goes through it. */
+float nbo_bf16r(float x);
+void nbo_bf16_round_array(int64_t n, const float* x, float* y);
 /* single-precision restatement (same loop nest, float arithmetic) */
 void nbo_axhelm_f32(int N, int Nf, int bk, int32_t Nelements, int32_t offset,
                     const float* ggeo, const float* D, const float* lambda,
@@ -164,6 +169,11 @@ typedef struct nbo_mg {
   const float* const* D32;
   const float* const* invD32;
   const float* const* J32;
+  /* learned step (library: nbx_bp_mg_set_learned): when W != NULL the fine
+   * level ends the cycle with x += mask(gs(net(res - A d))), the element
+   * network of nbo_net_apply with weights W and features feat */
+  const float* W;
+  const float* feat;
 } nbo_mg;
 void nbo_mg_interp(int Nf, int Nc, double* J);
 void nbo_mg_cheb(double lmax, double lo, double hi, int degree, double* c1, double* c2);
@@ -185,6 +195,10 @@ void nbo_net_gradient(int32_t nsamples, int32_t chunk, const int32_t* samples, i
                       const double* invDiag, const double* Gz, float* grad, double* loss);
 int nbo_bp_pcg_learned(const nbo_bp* bp, const float* W, const float* feat, int window, double ratio, double* r,
                        double* x, double tol, int maxit, double* hist_rdotr, double* hist_alpha, nbo_pcg_stats* st);
+/* nbo_bp_pcg_mixed with the learned preconditioner in every inner solve */
+int nbo_bp_pcg_mixed_learned(const nbo_bp* bp, const float* W, const float* feat, int window, double ratio, int prec,
+                             double* r, double* x, double tol, int maxit, double inner_tol, int max_cycles,
+                             nbo_pcg_stats* st);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@ the smoke check and bench.py's CPU-baseline leg, never by the product."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -43,7 +44,9 @@ class Ogs(C.Structure):
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        path = HERE / "libnbo.so"
+        # NBO_LIB selects another build of the same sources (the AddressSanitizer
+        # one of `make -C oracle asan`, run by tests/test_asan.py)
+        path = Path(os.environ["NBO_LIB"]) if os.environ.get("NBO_LIB") else HERE / "libnbo.so"
         if not path.exists():
             raise RuntimeError(f"{path} not built (make -C oracle)")
         _lib = C.CDLL(str(path))
@@ -275,6 +278,15 @@ class OracleBP:
                                C.c_double(tol), maxit, C.c_double(inner_tol), max_cycles, C.byref(st))
         return st
 
+    def pcg_mixed_learned(self, r, x, tol, maxit, W, feat, prec=1, inner_tol=1e-5, max_cycles=20, window=20,
+                          ratio=0.9):
+        """Refinement with the learned preconditioner in the inner solves."""
+        st = PCGStats()
+        lib().nbo_bp_pcg_mixed_learned(C.byref(self.struct), p(np.ascontiguousarray(W, dtype=np.float32)), p(feat),
+                                       window, C.c_double(ratio), int(prec), p(r), p(x), C.c_double(tol), maxit,
+                                       C.c_double(inner_tol), max_cycles, C.byref(st))
+        return st
+
     def invdiag(self):
         d = jacobi_local(self.N, self.E, self.offset, self.mesh["ggeo"], self.mesh["D"], self.lam)
         gs_many(self.ogs, d, 1, self.offset)
@@ -356,17 +368,19 @@ def net_gradient(samples, chunk, pairStride, R, Ev, invNorm, W, feat, invDiag, G
 
 
 def bf16_round(x):
-    """float32 -> nearest-even bf16 value (as float32)."""
-    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
-    return u.view(np.float32)
+    """float32 -> nearest-even bf16 value (as float32), by the C restatement
+    (nbo_bf16r: the one bf16 rounding every oracle path uses)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib().nbo_bf16_round_array(C.c_int64(x.size), p(x), p(y))
+    return y
 
 
 class MGStruct(C.Structure):
     _fields_ = [("nlev", C.c_int), ("bps", C.c_void_p), ("invD", C.c_void_p), ("J", C.c_void_p),
                 ("degree", C.c_int), ("c1", C.c_void_p), ("c2", C.c_void_p), ("coarse_iters", C.c_int),
                 ("prec", C.c_int), ("ggeo32", C.c_void_p), ("lam32", C.c_void_p), ("D32", C.c_void_p),
-                ("invD32", C.c_void_p), ("J32", C.c_void_p)]
+                ("invD32", C.c_void_p), ("J32", C.c_void_p), ("W", C.c_void_p), ("feat", C.c_void_p)]
 
 
 class OracleMG:
@@ -420,7 +434,23 @@ class OracleMG:
         self.f32_ptrs = [arr(i) for i in range(5)]
         self.struct = MGStruct(n, C.addressof(self.bps), C.addressof(self.invD_ptrs), C.addressof(self.J_ptrs),
                                degree, self.c1.ctypes.data, self.c2.ctypes.data, coarse_iters, prec,
-                               *[C.addressof(a) for a in self.f32_ptrs])
+                               *[C.addressof(a) for a in self.f32_ptrs], None, None)
+        self._net = None
+
+    def set_net(self, W=None):
+        """Learned step on the fine level (library: nbx_bp_mg_set_learned) with
+        network weights W (bf16 values as float32; None turns it off); the
+        features are the fine mesh's normalised geometric factors."""
+        if W is None:
+            self.struct.W = self.struct.feat = None
+            self._net = None
+            return
+        fine = self.fine
+        feat = net_features(fine.E, fine.Np, fine.mesh["ggeo"])
+        W = np.ascontiguousarray(W, dtype=np.float32)
+        self._net = (W, feat)
+        self.struct.W = W.ctypes.data
+        self.struct.feat = feat.ctypes.data
 
     def vcycle(self, r):
         x = np.zeros_like(r)

@@ -280,13 +280,7 @@ double nbo_sumsq_inv_f(int64_t n, const float* a)
  * weights rounded to bf16 (nearest even), products accumulated in double and
  * rounded to float (the products are exact in FP32; the matrix cores'
  * summation order is not specified, so agreement is to FP32 rounding). */
-static float bfr(float x)
-{
-  union { float f; uint32_t u; } a;
-  a.f = x;
-  a.u = (a.u + 0x7FFFu + ((a.u >> 16) & 1u)) & 0xFFFF0000u;
-  return a.f;
-}
+static float bfr(float x) { return nbo_bf16r(x); }
 
 void nbo_net_apply_mma(int32_t E, const float* W, const float* feat, const double* r, const double* invDiag,
                        const double* w, double* zl)

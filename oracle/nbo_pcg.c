@@ -115,13 +115,7 @@ static double fused_pap(const nbo_bp* bp, const double* p, const double* Ap, int
  * values held in float (prec 2, round-to-nearest-even of the float) */
 typedef struct { float *g, *D, *lam; int bf16; } lo_ops;
 
-static float to_bf16f(double v)
-{
-  union { float f; uint32_t u; } a;
-  a.f = (float)v;
-  a.u = (a.u + 0x7FFFu + ((a.u >> 16) & 1u)) & 0xFFFF0000u;
-  return a.f;
-}
+static float to_bf16f(double v) { return nbo_bf16r((float)v); }
 
 static void lo_make(const nbo_bp* bp, int prec, lo_ops* lo)
 {
@@ -306,11 +300,32 @@ int nbo_bp_pcg(const nbo_bp* bp, int jacobi, int flexible, int fused, double* r,
   return its;
 }
 
+static int pcg_mixed_impl(const nbo_bp* bp, int jacobi, int flexible, int prec, const learned_ctx* ln, double* r,
+                          double* x, double tol, int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st);
+
 /* Iterative refinement around PCG with a low-precision operator (library's
  * nbx_bp_pcg_mixed): r = b - A64 x; each cycle solves A_lo d = r from d = 0
  * to inner_tol (no absolute floor), x += d; stop on the true residual. */
 int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, double* r, double* x, double tol,
                      int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st)
+{
+  return pcg_mixed_impl(bp, jacobi, flexible, prec, NULL, r, x, tol, maxit, inner_tol, max_cycles, st);
+}
+
+/* The same refinement with the learned preconditioner in the inner solves
+ * (library: nbx_bp_pcg_mixed on a handle with precon LEARNED): each inner
+ * solve is the learned flexible PCG of nbo_bp_pcg_learned with the
+ * low-precision operator, its stall test starting afresh. */
+int nbo_bp_pcg_mixed_learned(const nbo_bp* bp, const float* W, const float* feat, int window, double ratio, int prec,
+                             double* r, double* x, double tol, int maxit, double inner_tol, int max_cycles,
+                             nbo_pcg_stats* st)
+{
+  const learned_ctx ln = {W, feat, window, ratio, NULL};
+  return pcg_mixed_impl(bp, 1, 1, prec, &ln, r, x, tol, maxit, inner_tol, max_cycles, st);
+}
+
+static int pcg_mixed_impl(const nbo_bp* bp, int jacobi, int flexible, int prec, const learned_ctx* ln, double* r,
+                          double* x, double tol, int maxit, double inner_tol, int max_cycles, nbo_pcg_stats* st)
 {
   const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
   const int64_t Nall = (int64_t)bp->Nf * bp->offset;
@@ -333,8 +348,8 @@ int nbo_bp_pcg_mixed(const nbo_bp* bp, int jacobi, int flexible, int prec, doubl
     if (jacobi && !invDiag) { invDiag = (double*)malloc(sizeof(double) * N); local_jacobi(bp, invDiag); }
     memset(d, 0, sizeof(double) * Nall);
     nbo_pcg_stats s;
-    pcg_core(bp, jacobi, flexible, 1, prec ? &lo : NULL, invDiag, r, d, inner_tol, 0.0, maxit - total, 0,
-             NULL, NULL, &s);
+    pcg_core_ext(bp, jacobi, flexible, 1, prec ? &lo : NULL, invDiag, ln, r, d, inner_tol, 0.0, maxit - total, 0,
+                 NULL, NULL, &s);
     total += s.iterations;
     breakdown = s.breakdown;
     for (int64_t n = 0; n < Nall; ++n) x[n] = x[n] + d[n];
@@ -444,6 +459,21 @@ double nbo_mg_power(const nbo_bp* bp_in, const double* invD, uint64_t seed, uint
 }
 
 static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x);
+
+/* Learned step of the fine level (library: the end of vcycle_level in
+ * nbx_mg.hip): rf = the fine residual the cycle leaves (FP64 copy),
+ * zl = w invDiag (rf + s f(rf)) element by element, x += mask(gs(zl)). */
+static void mg_learned_step(const nbo_mg* mg, const double* rf, double* x)
+{
+  const nbo_bp* bp = &mg->bps[0];
+  const int64_t n = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
+  double* zl = (double*)calloc(bp->offset, sizeof(double));
+  nbo_net_apply(bp->E, mg->W, mg->feat, rf, mg->invD[0], bp->w, zl);
+  nbo_gs_many(bp->ogs, zl, 1, bp->offset, NBO_DOUBLE, NBO_ADD);
+  for (int32_t m = 0; m < bp->Nmasked; ++m) zl[bp->maskIds[m]] = 0.0;
+  for (int64_t i = 0; i < n; ++i) x[i] = x[i] + zl[i];
+  free(zl);
+}
 
 /* V-cycle at level l: x = M_l r (x overwritten) */
 void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
@@ -589,6 +619,11 @@ void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
         d[id] = di;
         x[id] = x[id] + di;
       }
+  }
+  if (l == 0 && mg->W) {  /* learned step: rf = res - A d */
+    nbo_bp_apply(bp, d, Ad);
+    for (int64_t i = 0; i < n; ++i) Ad[i] = res[i] - Ad[i];
+    mg_learned_step(mg, Ad, x);
   }
   free(res); free(d); free(Ad); free(rc); free(xc);
 }
@@ -767,6 +802,13 @@ static void mg_vcycle_f(const nbo_mg* mg, int l, const void* r, void* x)
       d[id] = di;
       XADD(id, di);
     }
+  }
+  if (rdbl && mg->W) {  /* learned step on the fine level: rf = res - A d, as double */
+    mg_apply_f(mg, l, d, Ad);
+    double* rf = (double*)malloc(sizeof(double) * n);
+    for (int64_t i = 0; i < n; ++i) rf[i] = (double)(res[i] - Ad[i]);
+    mg_learned_step(mg, rf, xD);
+    free(rf);
   }
 #undef FIELDS
 #undef XADD

@@ -67,3 +67,20 @@ def test_host_errors_are_reported(nbx):
     bad = nbx.Box.make(0, 1, 1, 1)
     with pytest.raises(nbx.NbxError, match="bad sizes"):
         nbx.box_mesh(bad)
+
+
+def test_native_symbol_libraries_export_the_reference_names(nbx):
+    """nekbench_amd/native/libaxhelm_N<N>.so (N = 1..15) export every symbol
+    include/nbx_native_axhelm.h declares -- the reference's native kernel
+    names (axhelm/kernel/CPU/axhelm.c:27,113,229,307 and the Partial forms) --
+    and report their order."""
+    import ctypes
+    text = (ROOT / "include" / "nbx_native_axhelm.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(axhelm[A-Za-z0-9_]*)\s*\(", text)))
+    assert {"axhelm_v0", "axhelm_bk_v0", "axhelm_n3_v0", "axhelm_bk_n3_v0", "axhelmPartial_bk_v0"} <= set(names)
+    for n in range(1, 16):
+        lib = ctypes.CDLL(str(ROOT / "nekbench_amd" / "native" / f"libaxhelm_N{n}.so"))
+        missing = [s for s in names if not hasattr(lib, s)]
+        assert not missing, (n, missing)
+        assert lib.axhelm_native_order() == n and lib.axhelm_native_status() == 0

@@ -122,3 +122,27 @@ def test_nekbone_bp6_and_multigrid(tmp_path):
         res[pre] = (float(m.group(1)), int(m.group(2)))
     assert res["MULTIGRID"][1] * 4 < res["JACOBI"][1], res
     assert res["MULTIGRID"][0] < 1e-5 and res["JACOBI"][0] < 1e-5, res
+
+
+@pytest.mark.parametrize("sequence,precon", [("FUSED", "JACOBI"), ("REFERENCE", "JACOBI"), ("FUSED", "MULTIGRID")])
+def test_nekbone_profiling_breakdown(tmp_path, sequence, precon):
+    """[PROFILING] TRUE prints the reference's breakdown block after the
+    summary (nekBone/main.cpp:214-224): device seconds of local Ax, gs,
+    updatePCG, dot and preco over the timed solves; every phase the schedule
+    runs is timed, and together they stay within the solves' wall time."""
+    ini = tmp_path / "prof.ini"
+    ini.write_text("[POLYNOMIAL DEGREE]\n7\n[BOX NX]\n8\n[BOX NY]\n8\n[BOX NZ]\n8\n[BPMODE]\nFALSE\n[LAMBDA]\n0.5\n"
+                   f"[PRECONDITIONER]\n{precon}\n[SOLVER TOLERANCE]\n1e-8\n[NREPETITIONS]\n3\n"
+                   f"[FIXED ITERATION COUNT]\nFALSE\n[KERNEL SEQUENCE]\n{sequence}\n[PROFILING]\nTRUE\n")
+    out = run(BIN / "nekBone", ini)
+    assert "\nbreakdown\n" in out, out
+    rows = dict(re.findall(r"^  (local Ax|gs|updatePCG|dot|preco)\s*: (\S+) s$", out, re.M))
+    assert set(rows) == {"local Ax", "gs", "updatePCG", "dot", "preco"}, out
+    t = {k: float(v) for k, v in rows.items()}
+    assert all(t[k] > 0 for k in ("local Ax", "gs", "updatePCG", "dot", "preco")), t
+    summ = dict(re.findall(r"^  (\w[\w ]*?)\s*: (\S+)", out, re.M))
+    total = float(summ["elapsed time"].split()[0])
+    assert sum(t.values()) <= 1.05 * total + 1e-3, (t, total)
+    # without the key there is no breakdown
+    ini.write_text(ini.read_text().replace("[PROFILING]\nTRUE\n", ""))
+    assert "breakdown" not in run(BIN / "nekBone", ini)

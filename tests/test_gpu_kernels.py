@@ -3,6 +3,8 @@ the CPU oracle on the same seeded inputs.  FP64 results are expected to be
 bit-identical (same operation order, explicit fma on both sides); the tests
 assert the documented tolerance (1e-12 relative) and, separately, record
 bit-exactness where the design promises it."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -328,3 +330,42 @@ def test_axhelm_f16_mfma_limits(nbx, dev):
         nbx.axhelm(7, 3, 1, E, off, g, D, None, q, A, prec=nbx.NBX_F16)
     with pytest.raises(RuntimeError, match="fp16-MFMA"):
         nbx.axhelm(7, 1, 1, E, off, g, D, None, q, A, elementList=el, prec=nbx.NBX_F16)
+
+
+@pytest.mark.parametrize("N", [3, 7, 9])
+def test_native_symbols_match_nbx_axhelm(nbx, dev, N):
+    """The reference's native kernel symbols (nekbench_amd/native/
+    libaxhelm_N<N>.so, called as kernelHelper.cpp would call axhelm.c's:
+    dlong arguments by reference, device arrays) give nbx_axhelm's bits for
+    the four full and four Partial forms."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(str(Path(nbx.__file__).resolve().parent / "native" / f"libaxhelm_N{N}.so"))
+    box = nbx.Box.make(N, 3, 2, 2, perturb=0.1, seed=20 + N)
+    ggeo = nbx.box_mesh(box, ids=False)["ggeo"]
+    z, w, D = nbx.gll(N)
+    E, Np = box.Nelements, box.Np
+    off = E * Np
+    rng = np.random.default_rng(N)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    g, Dd = t(ggeo), t(D)
+    lam = t(np.concatenate([rng.uniform(0.5, 1.5, off), rng.uniform(0.1, 0.5, off)]))
+    q = t(rng.uniform(-1, 1, 3 * off))
+    el = torch.tensor([5, 0, 7, 2], dtype=torch.int32, device=dev)
+    P = lambda x: ctypes.c_void_p(x.data_ptr())
+    for name, nf, bk, partial in (("axhelm_v0", 1, 0, False), ("axhelm_bk_v0", 1, 1, False),
+                                  ("axhelm_n3_v0", 3, 0, False), ("axhelm_bk_n3_v0", 3, 1, False),
+                                  ("axhelmPartial_v0", 1, 0, True), ("axhelmPartial_bk_v0", 1, 1, True),
+                                  ("axhelmPartial_n3_v0", 3, 0, True), ("axhelmPartial_bk_n3_v0", 3, 1, True)):
+        ne = el.numel() if partial else E
+        got = torch.full((nf * off,), 3.0, dtype=torch.float64, device=dev)
+        exp = got.clone()
+        args = [ctypes.byref(ctypes.c_int32(ne)), ctypes.byref(ctypes.c_int32(off))]
+        if partial:
+            args.append(P(el))
+        getattr(lib, name)(*args, P(g), P(Dd), P(lam), P(q), P(got))
+        torch.cuda.synchronize()
+        assert lib.axhelm_native_status() == 0, nbx.last_error()
+        nbx.axhelm(N, nf, bk, ne, off, g, Dd, lam, q, exp, elementList=el if partial else None)
+        torch.cuda.synchronize()
+        assert torch.equal(got, exp), name

@@ -28,8 +28,11 @@ def make(nbx, nbo, N, dims, Nf=1, bpmode=True, l1=0.0, precon=1, flexible=False)
 @pytest.mark.parametrize("case", [(3, (3, 3, 2), 1, True, 0.0, 1, False),
                                   (7, (2, 2, 2), 1, False, 0.5, 1, False),
                                   (4, (2, 2, 2), 3, False, 0.1, 0, True),
-                                  (7, (2, 2, 2), 3, False, 0.2, 1, False)],
-                         ids=["poisson", "helmholtz", "bp6flex", "vector_n7"])
+                                  (7, (2, 2, 2), 3, False, 0.2, 1, False),
+                                  # the BP6 vector Helmholtz form of the N=9 three-field workload
+                                  # (lambda0 = 1, lambda1 = 0.1, Jacobi, flexible), on 2x2x1 elements
+                                  (9, (2, 2, 1), 3, False, 0.1, 1, True)],
+                         ids=["poisson", "helmholtz", "bp6flex", "vector_n7", "bp6_n9"])
 def test_mixed_matches_oracle(nbx, nbo, dev, prec, case):
     import torch
     N, dims, Nf, bpmode, l1, precon, flex = case
@@ -65,3 +68,54 @@ def test_mixed_bf16_needs_more_cycles_than_f32(nbx, dev):
         assert st.converged and (st.rdotr / st.rdotr0) ** 0.5 <= 1e-10
         out[name] = st.cycles
     assert out["bf16"] > out["f32"] >= 2
+
+
+def test_mixed_refuses_all_neumann(nbx, dev):
+    """The singular all-Neumann problem (no Dirichlet mask, lambda1 = 0;
+    nekBone/setup.cpp:210-214) has a true residual that need not reach the
+    tolerance, so the refinement solver refuses it instead of running every
+    cycle; nbx_bp_pcg (with BPZeroMean around it) still solves it."""
+    import torch
+    bp = nbx.BP(nbx.Box.make(3, 2, 2, 2, periodic=1, perturb=0.0, seed=5), bpmode=True, dirichlet=False,
+                precon=nbx.PRECON_JACOBI)
+    assert bp.all_neumann
+    r = bp.device_array("rhs", bp.fieldOffset).clone()
+    x = torch.zeros_like(r)
+    with pytest.raises(nbx.NbxError, match="all-Neumann"):
+        bp.pcg_mixed(r, x, 1e-8, 100, prec=nbx.NBX_F32)
+    st = bp.pcg(r, x, 1e-8, 500)
+    assert st.converged
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_mixed_learned_matches_oracle_on_graded_mesh(nbx, nbo, dev, prec):
+    """The stress-case solver (learned preconditioner with the FP32 or
+    bf16-geometry operator inside FP64 refinement) on a small z-graded mesh
+    (aspect 1:10, perturbed) against the oracle's restatement
+    (nbo_bp_pcg_mixed_learned): iterations, cycles, final true residual and
+    solution bit for bit, with the FP32 network inference."""
+    import torch
+    kw = dict(perturb=0.1, seed=44, zaspect=10.0)
+    dims = (2, 2, 3)
+    bp = nbx.BP(nbx.Box.make(7, *dims, **kw), bpmode=True, precon=nbx.PRECON_LEARNED)
+    ob = nbo.OracleBP(nbo.Box.make(7, *dims, **kw), bpmode=True)
+    net = nbx.Net(seed=3)
+    W = net.params()
+    W[2176:3041] = np.random.default_rng(11).uniform(-0.05, 0.05, 865).astype(np.float32)
+    net.set_params(W)
+    net.attach(bp)
+    Wb = nbo.bf16_round(net.params())
+    feat = nbo.net_features(ob.E, 512, ob.mesh["ggeo"])
+    n = bp.fieldOffset
+    inner = {"f32": 1e-4, "bf16": 1e-2}[prec]
+    r = bp.device_array("rhs", n).clone()
+    x = torch.zeros_like(r)
+    st = bp.pcg_mixed(r, x, 1e-10, 3000, prec=PRECS[prec], inner_tol=inner, max_cycles=30)
+    xr = np.zeros(n)
+    ost = ob.pcg_mixed_learned(ob.rhs.copy(), xr, 1e-10, 3000, Wb, feat, prec=PRECS[prec], inner_tol=inner,
+                               max_cycles=30)
+    print(f"learned mixed {prec}: {st.iterations} iterations, {st.cycles} cycles", flush=True)
+    assert st.iterations == ost.iterations and st.cycles == ost.iter
+    assert st.converged == ost.converged == 1
+    assert st.rdotr == ost.rdotr
+    assert np.array_equal(H(x), xr)

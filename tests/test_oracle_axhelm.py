@@ -182,6 +182,67 @@ def test_f16_rounding_matches_numpy(nbo):
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
 
 
+def _bf16_nearest_even(a):
+    """Independent bf16 rounding of float32 values: take the two bf16
+    neighbours of each magnitude (truncation toward zero and the next bf16
+    away from zero; above the largest finite bf16 that neighbour is 2^128, as
+    if the exponent were unbounded, and stands for infinity), compare the
+    distances exactly in float64 and break ties on the even last bit."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.int64)
+    mag = u & 0x7FFFFFFF
+    lo = mag & 0xFFFF0000                                   # |x| truncated
+    hi = lo + 0x10000                                       # next bf16 magnitude
+    val = lambda m: m.astype(np.uint32).view(np.float32).astype(np.float64)
+    ax = np.abs(a.astype(np.float64))
+    with np.errstate(invalid="ignore", over="ignore"):
+        hi_val = np.where(hi >= 0x7F800000, 2.0 ** 128, val(np.minimum(hi, 0x7F7F0000)))
+        dlo, dhi = ax - val(lo), hi_val - ax
+    pick_hi = (dhi < dlo) | ((dhi == dlo) & ((lo >> 16) & 1).astype(bool))
+    out = np.where(pick_hi, np.where(hi >= 0x7F800000, np.inf, hi_val), val(lo))
+    out = np.where(mag == 0x7F800000, np.inf, out)          # infinities stay
+    return np.copysign(out, a.astype(np.float64)).astype(np.float32)
+
+
+def test_bf16_rounding_is_nearest_even(nbo):
+    """The oracle's bf16 rounding (nbo_bf16r, which nbo.bf16_round and every
+    bf16 path of the oracle call) against an independent nearest-even
+    rounding and against torch's bfloat16 conversion: random values over the
+    whole float range, every exact tie between neighbouring bf16 values (all
+    normal and subnormal binades, both parities of the kept part), the values
+    one float ulp either side of each tie, the largest-finite -> infinity edge,
+    signed zeros, infinities and NaN.  A truncating or ties-away rounding fails
+    the tie cases; a rounding without the overflow rule fails the edge."""
+    import torch
+    rng = np.random.default_rng(23)
+    rand = rng.integers(0, 2 ** 32, 400000, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    rand = rand[np.isfinite(rand)]
+    # every bf16 value below infinity (positive), and the float halfway to the next one
+    kept = np.arange(0, 0x7F80, dtype=np.uint32)
+    ties = ((kept << 16) | 0x8000).view(np.float32)         # exact midpoints, even and odd kept parts
+    below, above = ((kept << 16) | 0x7FFF).view(np.float32), ((kept << 16) | 0x8001).view(np.float32)
+    edge = np.array([0.0, -0.0, np.inf, -np.inf, 3.3895314e38, 3.3961776e38, 3.4028235e38,
+                     np.float32(2.0 ** -126), np.float32(2.0 ** -133), np.float32(2.0 ** -149), 1.0, -1.0],
+                    dtype=np.float32)
+    for a in (rand, ties, -ties, below, -below, above, -above, edge):
+        got = nbo.bf16_round(a)
+        exp = _bf16_nearest_even(a)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+        tb = torch.from_numpy(a.copy()).to(torch.bfloat16).to(torch.float32).numpy()
+        assert np.array_equal(got.view(np.uint32), tb.view(np.uint32))
+    # ties go to the even neighbour: up exactly when the kept part is odd
+    assert np.array_equal(nbo.bf16_round(ties).view(np.uint32) >> 16, kept + (kept & 1))
+    # largest finite bf16 is 0x7F7F; its half-ulp midpoint and above overflow
+    assert nbo.bf16_round(np.array([3.3961776e38], np.float32)).view(np.uint32)[0] == 0x7F800000
+    assert nbo.bf16_round(np.array([np.nextafter(np.float32(3.3961776e38), np.float32(0))], np.float32)
+                          ).view(np.uint32)[0] == 0x7F7F0000
+    # signed zeros keep their sign; NaNs (any payload, either sign) stay NaN
+    z = nbo.bf16_round(np.array([0.0, -0.0], np.float32)).view(np.uint32)
+    assert z[0] == 0 and z[1] == 0x80000000
+    nans = np.array([0x7FC00000, 0x7F800001, 0x7F807FFF, 0xFF800001, 0x7FFFFFFF], np.uint32).view(np.float32)
+    assert np.isnan(nbo.bf16_round(nans)).all()
+
+
 def test_f16_mfma_oracle_tracks_fp64(nbo):
     """Matrix-core form with fp16 operands (FP32 geometry): within fp16
     rounding of the FP64 operator for BK and non-BK, several times closer

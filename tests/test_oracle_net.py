@@ -248,3 +248,31 @@ def test_matrix_core_forward_tracks_fp32(nbo, data):
     W0 = W.copy()
     W0[OW3:NPAR] = 0.0
     assert np.array_equal(nbo.net_apply_mma(E, W0, data["feat"], r, data["invDiag"], w), jac)
+
+
+def test_damped_jacobi_start_of_the_learned_multigrid_step(nbo):
+    """The learned multigrid step starts from damped Jacobi
+    (nekbench_amd.learned.damped_jacobi_params): with those parameters the
+    network's output is y = -alpha r^, so the element output is
+    (1 - alpha) w invDiag r up to the float rounding of r^ = r / s, for any
+    values of the other parameters (here random), for the FP32 form; the
+    matrix-core restatement rounds r^ and the hidden values to bf16, so there
+    the output is that up to alpha times half a bf16 ulp of r."""
+    from nekbench_amd.learned import damped_jacobi_params
+    rng = np.random.default_rng(4)
+    base = rng.uniform(-0.3, 0.3, NPAR).astype(np.float32)
+    E = 3
+    ob = nbo.OracleBP(nbo.Box.make(7, 3, 1, 1, perturb=0.1, seed=8), bpmode=True)
+    feat = nbo.net_features(ob.E, ob.Np, ob.mesh["ggeo"])
+    r = rng.uniform(-1, 1, E * NP_)
+    invd = rng.uniform(0.5, 2.0, E * NP_)
+    w = ob.invd
+    for alpha in (0.5, 0.25, 0.75):
+        W = nbo.bf16_round(damped_jacobi_params(base, alpha))
+        exp = (1 - alpha) * w * (invd * r)
+        for f, tol in ((nbo.net_apply, 1e-6), (nbo.net_apply_mma, alpha * 2.0 ** -8 / (1 - alpha))):
+            z = f(E, W, feat, r, invd, w)
+            assert np.abs(z - exp).max() <= tol * np.abs(exp).max(), (f.__name__, alpha)
+    # the parameters are valid bf16 values where the construction sets them
+    P = damped_jacobi_params(base, 0.5)
+    assert np.array_equal(nbo.bf16_round(P[OW3:OB3]), P[OW3:OB3])
