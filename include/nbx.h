@@ -167,6 +167,20 @@ int nbx_box_mesh_device(const nbx_box* box, int mask_boundary, double* ggeo, nbx
 int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
                                const double* x, const double* y, double* result_host, void* stream);
 int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream);
+/* No counterpart in the reference.  nbx_weighted_inner_product for callers
+ * that cannot allocate device memory (work under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same value, bit for
+ * bit, written to result_dev (device memory, one double) on `stream` without
+ * a host wait, its tile partials in a fixed device array of the library
+ * (512 KiB per GPU, kept until the process ends) instead of a library
+ * allocation.  N <= NBX_WDOT_NOALLOC_MAX_N (larger N: NBX_ERR_INVALID).  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * (concurrent streams, host threads) race and must be serialised by the
+ * caller. */
+#define NBX_WDOT_NOALLOC_MAX_N 134217728
+int nbx_weighted_inner_product_noalloc(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
+                                       const double* x, const double* y, double* result_dev, void* stream);
+
 /* Bandwidth probes of the bw benchmark (bw/main.cpp:59-140, bw/kernel/bwKernels.okl)
  * plus the copy and read streams: the measured HBM and LDS ceilings.  Device
  * arrays, 16-byte aligned (hipMalloc'd); asynchronous on `stream`.

@@ -20,6 +20,7 @@ NBX_F64, NBX_F32, NBX_BF16, NBX_F16, NBX_BF16_MFMA = 0, 1, 2, 3, 4
 NBX_DOUBLE, NBX_FLOAT, NBX_INT, NBX_LONG = 0, 1, 2, 3
 NBX_ADD, NBX_MUL, NBX_MIN, NBX_MAX = 0, 1, 2, 3
 NBX_SUCCESS, NBX_ERR_INVALID, NBX_ERR_UNSUPPORTED, NBX_ERR_HIP, NBX_ERR_ALLOC, NBX_ERR_DIVERGED = 0, -1, -2, -3, -4, -5
+NBX_WDOT_NOALLOC_MAX_N = 134217728  # include/nbx.h
 PRECON_COPY, PRECON_JACOBI, PRECON_LEARNED, PRECON_MG = 0, 1, 2, 3
 # reference ggeo factor order (core/params.h:55-62)
 GWJID, G00ID, G01ID, G02ID, G11ID, G12ID, G22ID, NGGEO = 0, 1, 2, 3, 4, 5, 6, 7
@@ -88,6 +89,7 @@ _SIGS = {
     "nbx_urand_fill": (_I, [C.c_uint64, C.c_uint64, C.c_int64, _P]),
     "nbx_box_mesh": (_I, [C.POINTER(Box), _I, _P, _P, _P, _P, _P, _P]),
     "nbx_weighted_inner_product": (_I, [C.c_int32, C.c_int32, _I, _P, _P, _P, _P, _P]),
+    "nbx_weighted_inner_product_noalloc": (_I, [C.c_int32, C.c_int32, _I, _P, _P, _P, _P, _P]),
     "nbx_scaled_add": (_I, [C.c_int32, _D, _P, _D, _P, _P]),
     "nbx_box_mesh_device": (_I, [C.POINTER(Box), _I, _P, _P, _P]),
     "nbx_ogs_setup_device": (_P, [C.c_int32, _P, _I]),
@@ -214,6 +216,13 @@ def weighted_inner_product(N, offset, Nfields, w, x, y, stream=None) -> float:
     check(lib().nbx_weighted_inner_product(N, offset, Nfields, ptr(w), ptr(x), ptr(y), C.byref(out),
                                            stream_ptr(stream)), "nbx_weighted_inner_product")
     return out.value
+
+
+def weighted_inner_product_noalloc(N, offset, Nfields, w, x, y, out, stream=None) -> None:
+    """<x,y>_w into out (device tensor, one double) on `stream`, no host wait;
+    the library's fixed device array holds the partials (no allocation)."""
+    check(lib().nbx_weighted_inner_product_noalloc(N, offset, Nfields, ptr(w), ptr(x), ptr(y), ptr(out),
+                                                   stream_ptr(stream)), "nbx_weighted_inner_product_noalloc")
 
 
 def scaled_add(N, alpha, x, beta, y, stream=None):

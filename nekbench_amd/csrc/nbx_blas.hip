@@ -180,3 +180,58 @@ extern "C" int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
 }
+
+// ---- allocation-free weighted inner product ------------------------------
+// The tile partials of nbx_weighted_inner_product_noalloc live in this fixed
+// device array (one copy per GPU, held until the process ends): one double
+// per tile of 2048 nodes, so N <= kNoAllocMaxTiles * 2048 = 134,217,728.  In
+// host code `scratch` names a host-side stand-in, not device memory: kernels
+// get the device address that hipGetSymbolAddress returns for the current
+// device, looked up once per device on the first call there.
+static constexpr unsigned int kNoAllocMaxTiles = 65536;  // 512 KiB of partials
+static __device__ __align__(256) double scratch[kNoAllocMaxTiles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out)
+{
+  int dev = -1;
+  NBX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(NBX_ERR_UNSUPPORTED, "weighted_inner_product_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(NBX_ERR_HIP, "weighted_inner_product_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return NBX_SUCCESS;
+}
+}  // namespace
+
+extern "C" int nbx_weighted_inner_product_noalloc(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
+                                                  const double* x, const double* y, double* result_dev,
+                                                  void* stream)
+{
+  if (N <= 0 || Nfields < 1 || !w || !x || !y || !result_dev)
+    return fail(NBX_ERR_INVALID, "weighted_inner_product_noalloc: bad arguments");
+  if (Nfields > 1 && offset < N) return fail(NBX_ERR_INVALID, "weighted_inner_product_noalloc: offset < N");
+  const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
+  if (nb > kNoAllocMaxTiles)
+    return fail(NBX_ERR_INVALID, "weighted_inner_product_noalloc: N = %d exceeds the maximum %u", N,
+                kNoAllocMaxTiles * (unsigned int)kRedTile);
+  double* partials = nullptr;
+  int rc = scratch_for_current_device(&partials);
+  if (rc) return rc;
+  // every partial the finish pass reads is written by the tile pass first,
+  // so nothing needs clearing between calls
+  ReduceWork ws;
+  ws.partials = partials;
+  ws.capacity = kNoAllocMaxTiles;
+  return launch_wdot(N, offset, Nfields, w, x, y, false, ws, result_dev, as_stream(stream));
+}
