@@ -28,7 +28,8 @@ namespace nbx {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short f2bf(float x) { return (short)bf16_bits(x); }
+// v_cvt_pk_bf16_f32: nearest even, the bits of bf16_bits for every non-NaN input
+__device__ __forceinline__ short f2bf(float x) { return (short)bf16_bits_hw(x); }
 
 // matrix-core operand types: storage element, 4-vector, rounding, product
 struct OpBf16 {

@@ -286,6 +286,8 @@ static void bp_release(nbx_bp* bp)
   if (bp->maskIds) (void)hipFree(bp->maskIds);
   if (bp->S) (void)hipFree(bp->S);
   if (bp->hS) (void)hipHostFree(bp->hS);
+  if (bp->hist_rdotr) (void)hipFree(bp->hist_rdotr);
+  if (bp->hist_alpha) (void)hipFree(bp->hist_alpha);
   for (hipEvent_t ev : bp->prof_ev) (void)hipEventDestroy(ev);
   if (bp->ev0) (void)hipEventDestroy(bp->ev0);
   if (bp->ev1) (void)hipEventDestroy(bp->ev1);

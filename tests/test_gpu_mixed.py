@@ -76,14 +76,15 @@ def test_mixed_refuses_all_neumann(nbx, dev):
     tolerance, so the refinement solver refuses it instead of running every
     cycle; nbx_bp_pcg (with BPZeroMean around it) still solves it."""
     import torch
-    bp = nbx.BP(nbx.Box.make(3, 2, 2, 2, periodic=1, perturb=0.0, seed=5), bpmode=True, dirichlet=False,
-                precon=nbx.PRECON_JACOBI)
+    # nekBone's own problem: periodic box [-1,1]^3, BP5, the manufactured cosine right-hand side
+    box = nbx.Box.make(5, 3, 3, 3, periodic=1, perturb=0.0, seed=5, extent=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0))
+    bp = nbx.BP(box, bpmode=True, dirichlet=False, precon=nbx.PRECON_COPY, rhs_kind=1)
     assert bp.all_neumann
     r = bp.device_array("rhs", bp.fieldOffset).clone()
     x = torch.zeros_like(r)
     with pytest.raises(nbx.NbxError, match="all-Neumann"):
         bp.pcg_mixed(r, x, 1e-8, 100, prec=nbx.NBX_F32)
-    st = bp.pcg(r, x, 1e-8, 500)
+    st = bp.pcg(r, x, 1e-8, 1000)
     assert st.converged
 
 
