@@ -123,6 +123,32 @@ __device__ __forceinline__ s16x8 relu_bf16(f32x4 c0, f32x4 c1)
   return __builtin_elementwise_max(__builtin_bit_cast(s16x8, p), s16x8{0, 0, 0, 0, 0, 0, 0, 0});
 }
 
+// one step of a wave maximum over 64-bit values: each lane takes the larger of
+// its value and the value the DPP pattern brings it (0 where it brings none)
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint64_t dpp_max_step(uint64_t v)
+{
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, kCtrl, kRowMask, 0xf, true);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), kCtrl, kRowMask, 0xf, true);
+  const uint64_t d = ((uint64_t)hi << 32) | lo;
+  return d > v ? d : v;
+}
+
+// maximum over the wave's 64 lanes, in every lane: row shifts by 1, 2, 4 and
+// 8 leave each row's maximum in its lane 15, the row broadcasts 15 and 31
+// carry it on to lane 63, which is read back (no LDS round trips)
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
+{
+  v = dpp_max_step<0x111, 0xf>(v);  // row_shr:1
+  v = dpp_max_step<0x112, 0xf>(v);  // row_shr:2
+  v = dpp_max_step<0x114, 0xf>(v);  // row_shr:4
+  v = dpp_max_step<0x118, 0xf>(v);  // row_shr:8
+  v = dpp_max_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_max_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, 63), hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // 1 / d in double for a node degree d (the quotient the division gives,
 // rounded at compile time): a table read instead of a division per node
 struct InvDegreeTable {
@@ -186,20 +212,14 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
   for (int q = t; q < kPadN; q += kP) s_in[q] = 0;  // the tile's border, once: elements rewrite the interior only
   // element scale s = max |r| (exact), as the largest bit pattern of |r|:
   // the bits of non-negative doubles order as their values
-  auto wave_max = [](double v) {
-    uint64_t m = __builtin_bit_cast(uint64_t, fabs(v));
-#pragma unroll
-    for (int h = 32; h >= 1; h >>= 1) {
-      const uint64_t o = __shfl_xor(m, h, 64);
-      m = o > m ? o : m;
-    }
-    return m;
-  };
-  auto block_max = [&]() {
-    uint64_t m = lds.red[0];
-#pragma unroll
-    for (int q = 1; q < 8; ++q) m = lds.red[q] > m ? lds.red[q] : m;
-    return __builtin_bit_cast(double, m);
+  auto wave_max = [](double v) { return wave_max_u64(__builtin_bit_cast(uint64_t, fabs(v))); };
+  auto block_max = [&]() {  // lanes 0..7 of each wave hold the 8 wave maxima, the rest 0
+    uint64_t m = l < 8 ? lds.red[l] : 0u;
+    m = dpp_max_step<0x111, 0xf>(m);  // row_shr:1
+    m = dpp_max_step<0x112, 0xf>(m);  // row_shr:2
+    m = dpp_max_step<0x114, 0xf>(m);  // row_shr:4: lane 7 has all 8
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)m, 7), hi = __builtin_amdgcn_readlane((uint32_t)(m >> 32), 7);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
   };
   // r^ = r / s rounded to bf16 once per node, and the factors, into the operand tile
   auto stage = [&](double rv, double sv, const uint16_t* fv) {
@@ -225,17 +245,17 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
     const double wn = deg ? kInvDegree.v[dg] : wraw;
     const int64_t en = e + gridDim.x;
     const bool more = en < E;
-    double rn_n = 0.0, di_n = 0.0, wraw_n = 1.0;
-    uint32_t dg_n = 0u;
-    uint16_t fe_n[7] = {};
-    if (more) {
-      rn_n = r[en * kP + t];
-      di_n = invDiag[en * kP + t];
-      if (deg) dg_n = deg[en * kP + t];
-      else if (w) wraw_n = w[en * kP + t];
+    // (past the last element the loads repeat this one: no branch, no use)
+    const int64_t ef = more ? en : e;
+    const double* const rp = r + ef * kP;  // uniform bases, lane offsets fixed
+    const double* const dp = invDiag + ef * kP;
+    const uint16_t* const fp = feat16 + ef * 7 * kP;
+    const double rn_n = rp[t], di_n = dp[t];
+    const uint32_t dg_n = deg ? deg[ef * kP + t] : 0u;
+    const double wraw_n = w && !deg ? w[ef * kP + t] : 1.0;
+    uint16_t fe_n[7];
 #pragma unroll
-      for (int q = 0; q < 7; ++q) fe_n[q] = feat16[(en * 7 + q) * kP + t];
-    }
+    for (int q = 0; q < 7; ++q) fe_n[q] = fp[q * kP + t];
     __syncthreads();  // the operand tile holds this element; the previous element's gather is done
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) {
@@ -284,12 +304,15 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
     // y_n = b3 + sum over the taps whose node n + tap is in the element of T[tap][n + tap]: every read
     // is at node t's address plus a constant and stays inside s_T (the guards cover the reach 64 + 8 + 1
     // of row 0 and row 26); the reads of taps that leave the element are discarded
+    // (the zero padding is a bit mask on the value, not a select: a select of
+    // a load can become a branch around the load, one LDS round trip per tap)
     float y = b3;
 #pragma unroll
     for (int tap = 0; tap < kTaps; ++tap) {
       const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
       const float v = s_T[kTGuard + tap * kTRow + t + (dx - 1) + 8 * (dy - 1) + 64 * (dz - 1)];
-      y = y + ((ox[dx] && oy[dy] && oz[dz]) ? v : 0.f);  // zero padding
+      const uint32_t keep = (ox[dx] && oy[dy] && oz[dz]) ? 0xFFFFFFFFu : 0u;
+      y = y + __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) & keep);
     }
     zl[id] = wn * (di * (rn + s * (double)y));
     if (!more) break;
