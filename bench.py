@@ -422,6 +422,12 @@ def run_learned_mg(nbx, torch, args):
                     "mean_time_s": float(np.mean(ts)), "converged": ok}
 
         res = {"tol": tol, "coarse_iterations": ci}
+        # the plain cycle with one more Chebyshev step: the learned step costs
+        # about one fine-level smoothing step, so it must beat that too
+        bm.mg_setup([7, 3, 1], degree=args.mg_degree + 1, coarse_iters=ci, prec=nbx.NBX_BF16)
+        res[f"mg_degree{args.mg_degree + 1}"] = evaluate()
+        log(f"[learned-mg] {name} plain multigrid, degree {args.mg_degree + 1}: {res[f'mg_degree{args.mg_degree + 1}']}")
+        bm.mg_setup([7, 3, 1], degree=args.mg_degree, coarse_iters=ci, prec=nbx.NBX_BF16)
         res["mg"] = evaluate()
         log(f"[learned-mg] {name} plain multigrid {res['mg']}")
         for tag, kf, kc, mode, lr, alpha in variants:
@@ -433,6 +439,8 @@ def run_learned_mg(nbx, torch, args):
                 rec[inf] = evaluate()
             base = res["mg"]["mean_time_s"]
             rec["gain_vs_same_mg"] = round(base / rec["mfma_inference"]["mean_time_s"], 3)
+            rec[f"gain_vs_mg_degree{args.mg_degree + 1}"] = round(
+                res[f"mg_degree{args.mg_degree + 1}"]["mean_time_s"] / rec["mfma_inference"]["mean_time_s"], 3)
             log(f"[learned-mg] {name} variant {tag}: fp32 {rec['fp32_inference']} mfma {rec['mfma_inference']}")
             res[f"learned_{tag}"] = rec
             bm.mg_set_learned(False)
@@ -520,7 +528,7 @@ def mesh16_measurements(nbx, torch, args, log, axhelm_only=False):
     if not args.no_learned:
         from nekbench_amd.learned import train_mg_network
         net, info = train_mg_network(nbx, torch, bm, seeds=(1000, args.mg_pair_seeds), kcount=8,
-                                     steps=args.train_steps_mg, log=log)
+                                     steps=args.train_steps_mg, alpha=args.mg_alpha, log=log)
         net.set_inference(True)
         run("learned_mixed_mg_bf16cycle", bm)
         bm.mg_set_learned(False)
@@ -595,16 +603,16 @@ def main():
                          "preconditioner on 16^3; graded: 64x64x32 graded stress case (all but pcg32 print "
                          "one JSON line of their own)")
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--train-steps-mg", type=int, default=1500,
+    ap.add_argument("--train-steps-mg", type=int, default=3000,
                     help="Adam steps of the learned multigrid step's network (trained on the 16^3 mesh)")
     ap.add_argument("--mg-pair-seeds", type=int, default=100,
                     help="right-hand sides of the learned multigrid's training pairs (8 pairs each)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned multigrid solver")
     ap.add_argument("--mg-degree", type=int, default=3, help="Chebyshev degree of the learned-multigrid study")
-    ap.add_argument("--mg-variants", default="0:8:0:1e-3:0.5",
+    ap.add_argument("--mg-variants", default="0:8:1:3e-3:0.3,0:8:0:1e-3:0.5",
                     help="learned-multigrid study: comma list of kfirst:kcount:mode:lr:alpha training variants")
     ap.add_argument("--mg-eval-seeds", type=int, default=5, help="right-hand sides of the learned-multigrid study")
-    ap.add_argument("--mg-alpha", type=float, default=0.5,
+    ap.add_argument("--mg-alpha", type=float, default=0.3,
                     help="damping of the learned step's damped-Jacobi start (1 - alpha multiplies Jacobi)")
     ap.add_argument("--learn-mode", type=int, default=1, choices=(0, 1),
                     help="energy-norm loss: 0 fixed scale, 1 scale-invariant")

@@ -7,10 +7,11 @@ end of every V-cycle to the fine-level residual the cycle leaves behind
   pairs      nbx_net_generate_mg_pairs: seeded right-hand sides, the
              multigrid PCG's iterates x_k, and the residual / error one
              V-cycle leaves on them, (r' = r_k - A M r_k, e' = e_k - M r_k);
-  loss       nbx_net_train_energy, mode 0: (z - e')^T A (z - e') / e'^T A e'
+  loss       nbx_net_train_energy: mode 0, (z - e')^T A (z - e') / e'^T A e'
              with z = mask(gs(w invDiag (r' + s f(r')))), the A-norm of the
              error left after the learned correction relative to before it
-             (below 1 exactly when the step reduces the error);
+             (below 1 exactly when the step reduces the error); mode 1 (the
+             default), its scale-invariant form;
   start      damped Jacobi, z = (1 - alpha) invDiag r' (two hidden channels
              carry relu(r^) and relu(-r^) through the network, the output
              layer scales them), so training starts from a step that already
@@ -66,8 +67,8 @@ def validation_residual(bp, b, r, x, iters, torch):
     return (st.rdotr / st.rdotr0) ** 0.5 if st.rdotr0 > 0 else 0.0
 
 
-def train_mg_network(nbx, torch, bp, *, seeds=(1000, 100), kfirst=0, kcount=8, steps=2000, chunk=250, lr=1e-3,
-                     alpha=0.5, mode=0, val_seed=8999, val_iters=6, net_seed=1, log=print):
+def train_mg_network(nbx, torch, bp, *, seeds=(1000, 100), kfirst=0, kcount=8, steps=3000, chunk=250, lr=3e-3,
+                     alpha=0.3, mode=1, val_seed=8999, val_iters=6, net_seed=1, log=print):
     """Train a network for the learned step of `bp` (precon MG, multigrid set
     up, N = 7, one field) on the pairs of iterates kfirst .. kfirst+kcount-1
     with the energy loss of nbx_net_train_energy (mode 0: the A-norm error
