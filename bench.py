@@ -225,8 +225,52 @@ def run_bp6_n9_24cube(nbx, torch, args):
         out[name] = rec
         log(f"[bp6] {name}: {rec}")
     set_speedups(out)
+    ops = bp6_operator_times(nbx, torch, bp)
     return {"config": "BP6 vector Helmholtz N=9, 24^3 elements, 3 fields, lambda1=0.1, tol 1e-10",
-            "elements": bp.Nelements, "solvers": out}
+            "elements": bp.Nelements, "solvers": out, "operator": ops}
+
+
+def bp6_operator_times(nbx, torch, bp, sets=2, reps=10):
+    """The three-field N=9 Helmholtz operator (axhelm_n3, non-BK) alone on the
+    BP6 mesh, per precision: FP64 and FP32 on the vector ALUs, bf16 and fp16
+    operands on the matrix cores (k_axhelm_mma16). Each precision gets its own
+    `sets` copies of every array (beyond the 256 MB Infinity Cache for every
+    precision but bf16 geometry) and is timed over reps x sets launches with
+    HIP events on the launch stream; GB/s by axhelm/main.cpp:162-163's count
+    (FP64 words: (3*2 + 7 + 2) * Np per element), whatever the precision."""
+    E, Np, off = bp.Nelements, bp.Np, bp.fieldOffset
+    g64 = bp.device_array("ggeo", E * 7 * Np)
+    D64 = bp.device_array("D", 100)
+    lam64 = bp.device_array("lambda", 2 * off)
+    q64 = torch.rand(3 * off, dtype=torch.float64, device="cuda") * 2 - 1
+    ref_bytes = E * Np * (3 * 2 + 7 + 2) * 8
+    bf = lambda a: a.float().to(torch.bfloat16).view(torch.int16)
+    forms = {"fp64": (nbx.NBX_F64, lambda: (g64.clone(), D64, lam64.clone(), q64.clone())),
+             "fp32": (nbx.NBX_F32, lambda: (g64.float(), D64.float(), lam64.float(), q64.float())),
+             "bf16_mfma": (nbx.NBX_BF16_MFMA, lambda: (bf(g64), D64.float(), bf(lam64), q64.float())),
+             "fp16_mfma": (nbx.NBX_F16, lambda: (g64.float(), D64.float(), lam64.float(), q64.float()))}
+    s = torch.cuda.current_stream()
+    out = {}
+    for name, (prec, make) in forms.items():
+        arrs = []
+        for _ in range(sets):
+            g, D, lam, q = make()
+            arrs.append((g, D, lam, q, torch.empty_like(q)))
+        for g, D, lam, q, A in arrs:
+            nbx.axhelm(9, 3, 0, E, off, g, D, lam, q, A, prec=prec, stream=s)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(s)
+        for _ in range(reps):
+            for g, D, lam, q, A in arrs:
+                nbx.axhelm(9, 3, 0, E, off, g, D, lam, q, A, prec=prec, stream=s)
+        ev1.record(s)
+        ev1.synchronize()
+        t = ev0.elapsed_time(ev1) / 1e3 / (reps * sets)
+        out[name] = {"us_per_launch": round(t * 1e6, 1), "GB_s_reference_count": round(ref_bytes / t / 1e9, 1)}
+        log(f"[bp6] operator {name}: {out[name]}")
+        del arrs
+    out["reference_bytes_per_launch"] = ref_bytes
+    return out
 
 
 def train_net(nbx, torch, bp, nseeds, snapshots, steps, lr, log, seed0=1000, mode=1, select_seed=None,
@@ -599,7 +643,8 @@ def main():
     ap.add_argument("--study", default="pcg32",
                     choices=("pcg32", "axhelm16", "bp6", "learned16", "graded", "learned_mg"),
                     help="pcg32: the PCG benchmark line on 32^3 (default); axhelm16: the 16^3 FP64 axhelm alone "
-                         "(rotating buffer sets); bp6: BP6 N=9 24^3 mixed precision; learned16: learned "
+                         "(rotating buffer sets); bp6: BP6 N=9 24^3 mixed precision and its operator per "
+                         "precision; learned16: learned "
                          "preconditioner on 16^3; graded: 64x64x32 graded stress case (all but pcg32 print "
                          "one JSON line of their own)")
     ap.add_argument("--train-steps", type=int, default=3000)

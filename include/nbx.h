@@ -41,9 +41,9 @@ typedef int64_t nbx_hlong;
 #define NBX_ERR_DIVERGED (-5)    /* PCG breakdown (non-finite or non-positive curvature) */
 
 /* NBX_BF16_MFMA: bf16 geometry and bf16 MFMA tensor contractions with FP32
- * accumulation, FP32 vectors (nbx_axhelm: N = 7, one field, no elementList).
- * NBX_F16: FP32 geometry and fp16 MFMA tensor contractions with FP32
- * accumulation, FP32 vectors (nbx_axhelm only, same limits). */
+ * accumulation, FP32 vectors (nbx_axhelm: N = 7 or 9, one or three fields, no
+ * elementList).  NBX_F16: FP32 geometry and fp16 MFMA tensor contractions with
+ * FP32 accumulation, FP32 vectors (nbx_axhelm only, same limits). */
 typedef enum { NBX_F64 = 0, NBX_F32 = 1, NBX_BF16 = 2, NBX_F16 = 3, NBX_BF16_MFMA = 4 } nbx_prec;
 /* ogs types / ops (3rdParty/ogs/ogs.hpp:128-140: "double", "float", "int",
  * "long long int"; "add", "mul", "min", "max") */
@@ -72,10 +72,11 @@ int nbx_device_synchronize(void);
  * arithmetic; ggeo and lambda as bf16, i.e. the upper 16 bits of the float,
  * uint16_t; one field, no elementList), NBX_BF16_MFMA (as NBX_BF16, with q,
  * D and the G-scaled gradients rounded to bf16 for the matrix-core
- * contractions, FP32 accumulation; N = 7), NBX_F16 (float ggeo, D, lambda, q,
- * Aq; q, D and the G-scaled gradients rounded to fp16, nearest even, for the
- * matrix-core contractions, FP32 accumulation; N = 7, one field, no
- * elementList; fp16's range suits q of order one).  N in [1, 15]. */
+ * contractions, FP32 accumulation; N = 7 or 9, one or three fields, no
+ * elementList), NBX_F16 (float ggeo, D, lambda, q, Aq; q, D and the G-scaled
+ * gradients rounded to fp16, nearest even, for the matrix-core contractions,
+ * FP32 accumulation; N = 7 or 9, one or three fields, no elementList; fp16's
+ * range suits q of order one).  N in [1, 15]. */
 int nbx_axhelm(int N, int Nfields, int bk, nbx_prec prec,
                nbx_dlong Nelements, nbx_dlong offset, const nbx_dlong* elementList,
                const void* ggeo, const void* D, const void* lambda,

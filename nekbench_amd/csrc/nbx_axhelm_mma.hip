@@ -1,8 +1,9 @@
 // axhelm with the tensor contractions on the matrix cores: bf16 or fp16
-// operands, FP32 accumulation (v_mfma_f32_16x16x16_bf16 / _f16), N = 7, one
-// field.
+// operands, FP32 accumulation (v_mfma_f32_16x16x16_bf16 / _f16).  N = 7 with
+// one field uses k_axhelm_mma; N = 9, and N = 7 with three fields, use
+// k_axhelm_mma16 (below).
 //
-// One wave64 workgroup holds two elements.  Each 1-D contraction of an
+// k_axhelm_mma: one wave64 workgroup holds two elements.  Each 1-D contraction of an
 // element, (D q)(a, c) = sum_m D[a][m] q(m, c) with a the index along the
 // direction and c = 0..63 the other two, is a 8x8 by 8x64 product; the two
 // elements are stacked into one 16x16x16 MFMA with the block-diagonal
@@ -195,14 +196,168 @@ __global__ __launch_bounds__(64) void k_axhelm_mma(const int32_t Nelements, cons
   }
 }
 
+// General form for 8 <= Nq <= 16 and one or three fields (the N = 9 BP6
+// operator among them): one element per 256-lane workgroup, each 1-D
+// contraction of a field a (16 x 16) by (16 x Nq^2) product with D
+// zero-padded to 16 x 16, columns in tiles of 16 (the padding rows and
+// columns of D are zero, so the padded rows of the operand contribute
+// nothing and the padded output rows are never read).  Operands sit in LDS
+// in the MFMA's B order: for direction d the column c runs over the other two
+// indices and the 16 values of the contracted index are contiguous, so a
+// lane's four operands (rows 4g .. 4g + 3 of its column) are one 8-byte
+// read and its four products one 16-byte write.  The geometric factors
+// and lambda are read once per element into registers and serve every field;
+// each field runs: stage h(q) in the three direction orders; forward
+// products; G-scaling in FP32 (the FP32 kernel's fma order) rounded to the
+// operand type in the order of its own direction; transposed products;
+// Aq = (r + s) + t [+ GWJ lambda1 q].  The arithmetic is that of the N = 7
+// kernel above, so one restatement (nbo_axhelm_*mma) covers both.
+template <int NQ, int NF, bool BK, typename G, typename OP>
+__global__ __launch_bounds__(256) void k_axhelm_mma16(const int32_t Nelements, const int32_t offset,
+                                                      const G* __restrict__ ggeo, const float* __restrict__ D,
+                                                      const G* __restrict__ lambda, const float* __restrict__ q,
+                                                      float* __restrict__ Aq, const int* __restrict__ skip)
+{
+  static_assert(NQ >= 2 && NQ <= 16, "Nq in [2, 16]");
+  constexpr int P = NQ * NQ * NQ, NQ2 = NQ * NQ, CT = (NQ2 + 15) / 16, NC = CT * 16;
+  constexpr int BS = 20;  // operand column stride (elements): 40 bytes, 8-byte aligned reads
+  constexpr int FS = 20;  // product column stride (floats): 80 bytes, 16-byte aligned writes
+  constexpr int PT = (P + 255) / 256;
+  typedef typename OP::elem H;
+  typedef typename OP::vec4 V4;
+  if (skip && *skip) return;
+  __shared__ __align__(16) H s_b[3][NC * BS];      // operands per direction: h(q), then h(G)
+  __shared__ __align__(16) float s_c[3][NC * FS];  // products per direction
+  const int t = threadIdx.x, l = t & 63, w = t >> 6, kb = 4 * (l >> 4), lc = l & 15;
+  const int64_t e = blockIdx.x;
+  // zero operands once: the padding rows (contracted index >= Nq) stay zero
+  for (int u = t; u < 3 * NC * BS; u += 256) (&s_b[0][0])[u] = (H)0;
+  V4 Af, At;
+  {
+    const int row = lc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = kb + i;
+      const bool in = row < NQ && k < NQ;
+      Af[i] = in ? OP::round(D[row * NQ + k]) : (H)0;
+      At[i] = in ? OP::round(D[k * NQ + row]) : (H)0;
+    }
+  }
+  // this lane's nodes: their places in the three direction orders and geometry
+  int pb[PT][3], pc[PT][3];
+  float gf[PT][6], l0[PT], wl1[PT];
+#pragma unroll
+  for (int u = 0; u < PT; ++u) {
+    const int n = t + 256 * u, nn = n < P ? n : P - 1;
+    const int i = nn % NQ, j = (nn / NQ) % NQ, k = nn / NQ2;
+    pb[u][0] = (j + NQ * k) * BS + i;
+    pb[u][1] = (i + NQ * k) * BS + j;
+    pb[u][2] = (i + NQ * j) * BS + k;
+    pc[u][0] = (j + NQ * k) * FS + i;
+    pc[u][1] = (i + NQ * k) * FS + j;
+    pc[u][2] = (i + NQ * j) * FS + k;
+    const G* g = ggeo + e * (int64_t)(kNggeo * P) + nn;
+    gf[u][0] = ld_geo(&g[kG00 * P]);
+    gf[u][1] = ld_geo(&g[kG01 * P]);
+    gf[u][2] = ld_geo(&g[kG02 * P]);
+    gf[u][3] = ld_geo(&g[kG11 * P]);
+    gf[u][4] = ld_geo(&g[kG12 * P]);
+    gf[u][5] = ld_geo(&g[kG22 * P]);
+    l0[u] = BK ? 1.f : ld_geo(&lambda[e * P + nn]);
+    wl1[u] = BK ? 0.f : ld_geo(&g[kGWJ * P]) * ld_geo(&lambda[e * P + offset + nn]);
+  }
+  // a (direction, column tile) product per wave turn
+  auto products = [&](const V4& A) {
+    for (int tile = w; tile < 3 * CT; tile += 4) {
+      const int d = tile / CT, col = (tile - d * CT) * 16 + lc;
+      const V4 B = *reinterpret_cast<const V4*>(&s_b[d][col * BS + kb]);
+      const f32x4 C = OP::mma(A, B, f32x4{0.f, 0.f, 0.f, 0.f});
+      *reinterpret_cast<f32x4*>(&s_c[d][col * FS + kb]) = C;
+    }
+  };
+  __syncthreads();
+#pragma unroll 1
+  for (int f = 0; f < NF; ++f) {
+    const float* qf = q + (int64_t)f * offset + e * P;
+    float qv[PT];
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int n = t + 256 * u;
+      qv[u] = n < P ? ldnt(&qf[n]) : 0.f;
+      if (n < P) {
+        const H h = OP::round(qv[u]);
+        s_b[0][pb[u][0]] = h;
+        s_b[1][pb[u][1]] = h;
+        s_b[2][pb[u][2]] = h;
+      }
+    }
+    __syncthreads();
+    products(Af);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      if (t + 256 * u >= P) continue;
+      const float qr = s_c[0][pc[u][0]], qs = s_c[1][pc[u][1]], qt = s_c[2][pc[u][2]];
+      float Gr = __builtin_fmaf(gf[u][2], qt, __builtin_fmaf(gf[u][1], qs, gf[u][0] * qr));
+      float Gs = __builtin_fmaf(gf[u][4], qt, __builtin_fmaf(gf[u][3], qs, gf[u][1] * qr));
+      float Gt = __builtin_fmaf(gf[u][5], qt, __builtin_fmaf(gf[u][4], qs, gf[u][2] * qr));
+      if (!BK) {
+        Gr = l0[u] * Gr;
+        Gs = l0[u] * Gs;
+        Gt = l0[u] * Gt;
+      }
+      s_b[0][pb[u][0]] = OP::round(Gr);
+      s_b[1][pb[u][1]] = OP::round(Gs);
+      s_b[2][pb[u][2]] = OP::round(Gt);
+    }
+    __syncthreads();
+    products(At);
+    __syncthreads();
+    float* af = Aq + (int64_t)f * offset + e * P;
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int n = t + 256 * u;
+      if (n >= P) continue;
+      float res = (s_c[0][pc[u][0]] + s_c[1][pc[u][1]]) + s_c[2][pc[u][2]];
+      if (!BK) res = __builtin_fmaf(wl1[u], qv[u], res);
+      stnt(res, &af[n]);
+    }
+    // the next field's staging writes s_b only; its first barrier orders
+    // these reads of s_c before its products overwrite them
+  }
+}
+
 int axhelm_mma_launch(int N, int Nf, int bk, bool bf16_geometry, bool f16_operands, int32_t Nelements,
                       int32_t offset, const void* ggeo, const void* D, const void* lambda, const void* q, void* Aq,
                       const int* skip, hipStream_t s)
 {
   const char* name = f16_operands ? "axhelm fp16-MFMA" : "axhelm bf16-MFMA";
-  if (N != 7 || Nf != 1) return fail(NBX_ERR_UNSUPPORTED, "%s: N=7 and one field only", name);
+  if (!((N == 7 || N == 9) && (Nf == 1 || Nf == 3)))
+    return fail(NBX_ERR_UNSUPPORTED, "%s: N = 7 or 9, one or three fields", name);
   if (f16_operands && bf16_geometry) return fail(NBX_ERR_UNSUPPORTED, "%s: FP32 geometry only", name);
   if (Nelements <= 0) return Nelements == 0 ? NBX_SUCCESS : fail(NBX_ERR_INVALID, "axhelm: Nelements < 0");
+  if (Nf == 3 && offset < Nelements * (N + 1) * (N + 1) * (N + 1))
+    return fail(NBX_ERR_INVALID, "axhelm: offset < Nelements * Np");
+  if (!(N == 7 && Nf == 1)) {
+#define NBX_MMA16(NQV, NFV, BKV, GT, OPT)                                                                     \
+  hipLaunchKernelGGL((k_axhelm_mma16<NQV, NFV, BKV, GT, OPT>), dim3((unsigned)Nelements), dim3(256), 0, s,   \
+                     Nelements, offset, (const GT*)ggeo, (const float*)D, (const GT*)lambda, (const float*)q, \
+                     (float*)Aq, skip)
+#define NBX_MMA16_BK(NQV, NFV, GT, OPT) \
+  if (bk) NBX_MMA16(NQV, NFV, true, GT, OPT); else NBX_MMA16(NQV, NFV, false, GT, OPT);
+#define NBX_MMA16_T(NQV, NFV)                                   \
+  if (f16_operands) { NBX_MMA16_BK(NQV, NFV, float, OpF16) }     \
+  else if (bf16_geometry) { NBX_MMA16_BK(NQV, NFV, uint16_t, OpBf16) } \
+  else { NBX_MMA16_BK(NQV, NFV, float, OpBf16) }
+    if (N == 7) { NBX_MMA16_T(8, 3) }
+    else if (Nf == 1) { NBX_MMA16_T(10, 1) }
+    else { NBX_MMA16_T(10, 3) }
+#undef NBX_MMA16_T
+#undef NBX_MMA16_BK
+#undef NBX_MMA16
+    NBX_HIP_LAUNCH();
+    return NBX_SUCCESS;
+  }
   const dim3 grid((unsigned)((Nelements + 1) / 2)), block(64);
 #define NBX_MMA(BKV, GT, OPT)                                                                                \
   hipLaunchKernelGGL((k_axhelm_mma<BKV, GT, OPT>), grid, block, 0, s, Nelements, offset, (const GT*)ggeo,    \

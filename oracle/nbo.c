@@ -697,8 +697,8 @@ double nbo_tree_dot(int64_t n, const double* x, const double* y)
 #include <omp.h>
 int nbo_max_threads(void) { return omp_get_max_threads(); }
 
-/* axhelm with matrix-core contractions (library: k_axhelm_mma,
- * nbx_axhelm_mma.hip), N = 7, one field: q, D and the G-scaled gradients are
+/* axhelm with matrix-core contractions (library: k_axhelm_mma and
+ * k_axhelm_mma16, nbx_axhelm_mma.hip), any N, Nf fields: q, D and the G-scaled gradients are
  * rounded to the operand type (bf16 or fp16, nearest even); the 8-term
  * contractions are accumulated in double and rounded to float (the products
  * are exact in FP32; the matrix cores' internal summation order is not
@@ -746,33 +746,44 @@ void nbo_f16_round_array(int64_t n, const float* x, float* y)
   for (int64_t i = 0; i < n; ++i) y[i] = f16r(x[i]);
 }
 
-static void axhelm_mma_impl(float (*h)(float), int bk, int32_t E, int32_t offset, const float* ggeo, const float* D,
-                            const float* lambda, const float* q, float* Aq);
+static void axhelm_mma_impl(float (*h)(float), int N, int Nf, int bk, int32_t E, int32_t offset, const float* ggeo,
+                            const float* D, const float* lambda, const float* q, float* Aq);
 
 void nbo_axhelm_f16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
                        const float* q, float* Aq)
 {
-  axhelm_mma_impl(f16r, bk, E, offset, ggeo, D, lambda, q, Aq);
+  axhelm_mma_impl(f16r, 7, 1, bk, E, offset, ggeo, D, lambda, q, Aq);
 }
 
 void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
                         const float* q, float* Aq)
 {
-  axhelm_mma_impl(bf16r, bk, E, offset, ggeo, D, lambda, q, Aq);
+  axhelm_mma_impl(bf16r, 7, 1, bk, E, offset, ggeo, D, lambda, q, Aq);
 }
 
-static void axhelm_mma_impl(float (*h)(float), int bk, int32_t E, int32_t offset, const float* ggeo, const float* D,
-                            const float* lambda, const float* q, float* Aq)
+/* any order N <= 15 and Nf fields (q, Aq: field f at f*offset; the same
+ * ggeo and lambda for every field, as the three-field kernels) */
+void nbo_axhelm_mma_n(int f16, int N, int Nf, int bk, int32_t E, int32_t offset, const float* ggeo, const float* D,
+                      const float* lambda, const float* q, float* Aq)
 {
-  enum { Q = 8, P = 512 };
-  float Db[Q * Q];
+  axhelm_mma_impl(f16 ? f16r : bf16r, N, Nf, bk, E, offset, ggeo, D, lambda, q, Aq);
+}
+
+static void axhelm_mma_impl(float (*h)(float), int N, int Nf, int bk, int32_t E, int32_t offset, const float* ggeo,
+                            const float* D, const float* lambda, const float* q, float* Aq)
+{
+  enum { QM = 16, PM = QM * QM * QM };
+  const int Q = N + 1, P = Q * Q * Q;
+  float Db[QM * QM];
   for (int i = 0; i < Q * Q; ++i) Db[i] = h(D[i]);
 #pragma omp parallel
   {
-    float qb[P], qr[P], qs[P], qt[P], gr[P], gs[P], gt[P];
-#pragma omp for schedule(static)
-    for (int32_t e = 0; e < E; ++e) {
-      const float* qe = q + (int64_t)e * P;
+    float qb[PM], qr[PM], qs[PM], qt[PM], gr[PM], gs[PM], gt[PM];
+#pragma omp for schedule(static) collapse(2)
+    for (int32_t e = 0; e < E; ++e)
+    for (int f = 0; f < Nf; ++f) {
+      const float* qe = q + (int64_t)f * offset + (int64_t)e * P;
+      float* ae = Aq + (int64_t)f * offset + (int64_t)e * P;
       const float* g = ggeo + (int64_t)e * 7 * P;
       for (int n = 0; n < P; ++n) qb[n] = h(qe[n]);
       for (int n = 0; n < P; ++n) {
@@ -804,7 +815,7 @@ static void axhelm_mma_impl(float (*h)(float), int bk, int32_t E, int32_t offset
         }
         float res = ((float)ar + (float)as) + (float)at;
         if (!bk) res = fmaf(g[0 * P + n] * lambda[(int64_t)e * P + offset + n], qe[n], res);
-        Aq[(int64_t)e * P + n] = res;
+        ae[n] = res;
       }
     }
   }

@@ -86,6 +86,8 @@ void nbo_axhelm_bf16mma(int bk, int32_t E, int32_t offset, const float* ggeo, co
                         const float* q, float* Aq);
 void nbo_axhelm_f16mma(int bk, int32_t E, int32_t offset, const float* ggeo, const float* D, const float* lambda,
                        const float* q, float* Aq);
+void nbo_axhelm_mma_n(int f16, int N, int Nf, int bk, int32_t E, int32_t offset, const float* ggeo, const float* D,
+                      const float* lambda, const float* q, float* Aq);
 /* IEEE binary16 rounding (nearest even, subnormals kept) of each value */
 void nbo_f16_round_array(int64_t n, const float* x, float* y);
 /* bf16 rounding of a float (the value of its upper 16 bits after round to

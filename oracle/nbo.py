@@ -321,6 +321,16 @@ def axhelm_f16mma(bk, E, offset, ggeo, D, lam, q):
     return Aq
 
 
+def axhelm_mma_n(f16, N, Nf, bk, E, offset, ggeo, D, lam, q):
+    """float32 arrays, any order N and Nf fields (field f of q and Aq at
+    f*offset); bf16 (f16=False: ggeo and lam hold bf16 values) or fp16
+    matrix-core operands"""
+    Aq = np.zeros_like(q)
+    lib().nbo_axhelm_mma_n(int(bool(f16)), int(N), int(Nf), int(bk), C.c_int32(E), C.c_int32(offset), p(ggeo), p(D),
+                           p(lam), p(q), p(Aq))
+    return Aq
+
+
 def f16_round(x):
     """float32 -> nearest-even IEEE binary16 value (as float32), by the C restatement."""
     x = np.ascontiguousarray(x, dtype=np.float32)

@@ -327,9 +327,67 @@ def test_axhelm_f16_mfma_limits(nbx, dev):
     with pytest.raises(RuntimeError, match="fp16-MFMA"):
         nbx.axhelm(5, 1, 1, E, off, g, D, None, q, A, prec=nbx.NBX_F16)
     with pytest.raises(RuntimeError, match="fp16-MFMA"):
-        nbx.axhelm(7, 3, 1, E, off, g, D, None, q, A, prec=nbx.NBX_F16)
+        nbx.axhelm(7, 2, 1, E, off, g, D, None, q, A, prec=nbx.NBX_F16)
     with pytest.raises(RuntimeError, match="fp16-MFMA"):
         nbx.axhelm(7, 1, 1, E, off, g, D, None, q, A, elementList=el, prec=nbx.NBX_F16)
+
+
+@pytest.mark.parametrize("N,Nf", [(9, 1), (9, 3), (7, 3)])
+@pytest.mark.parametrize("bk", [1, 0])
+@pytest.mark.parametrize("f16", [False, True], ids=["bf16", "fp16"])
+def test_axhelm_mfma_general_matches_oracle(nbx, nbo, dev, N, Nf, bk, f16):
+    """The general matrix-core operator (k_axhelm_mma16: Nq up to 16, one or
+    three fields, D zero-padded to 16 x 16) against the oracle's restatement
+    (nbo_axhelm_mma_n): the matrix cores' summation order is not specified,
+    so a forward product can differ from the oracle's in the last FP32 bit,
+    and where that happens next to a rounding boundary of the operand type
+    the G value rounds to the neighbouring bf16 / fp16 value and moves the Nq
+    outputs along that line by up to D times one operand quantum.  So: nearly
+    every output (99 %) equal to FP32 rounding, every output within an
+    operand quantum; every field within operand rounding of the FP64
+    operator; three fields at a padded field offset, the padding untouched."""
+    import torch
+    Q = N + 1
+    Np = Q ** 3
+    box = nbo.Box.make(N, 3, 2, 1, perturb=0.1, seed=30 + N + Nf)
+    m = nbo.box_mesh(box)
+    E = box.E
+    off = E * Np + (64 if Nf > 1 else 0)
+    rng = np.random.default_rng(7 * N + Nf + 2 * bk + int(f16))
+    q = np.zeros(Nf * off)
+    for f in range(Nf):
+        q[f * off:f * off + E * Np] = rng.uniform(-1, 1, E * Np)
+    lam = np.zeros(2 * off)
+    lam[:E * Np] = rng.uniform(0.5, 1.5, E * Np)
+    lam[off:off + E * Np] = rng.uniform(0.1, 0.5, E * Np)
+    D32, q32 = m["D"].astype(np.float32), q.astype(np.float32)
+    if f16:
+        g32, l32 = m["ggeo"].astype(np.float32), lam.astype(np.float32)
+        dg, dl = torch.tensor(g32, device=dev), torch.tensor(l32, device=dev)
+        prec, tol64 = nbx.NBX_F16, 4e-3
+    else:
+        g32 = nbo.bf16_round(m["ggeo"].astype(np.float32))
+        l32 = nbo.bf16_round(lam.astype(np.float32))
+        bits = lambda a: torch.from_numpy((a.view(np.uint32) >> 16).astype(np.uint16).view(np.int16).copy())
+        dg, dl = bits(g32).to(dev), bits(l32).to(dev)
+        prec, tol64 = nbx.NBX_BF16_MFMA, 3e-2
+    dq = torch.tensor(q32, device=dev)
+    dA = torch.full_like(dq, 7.0)
+    nbx.axhelm(N, Nf, bk, E, off, dg, torch.tensor(D32, device=dev), dl, dq, dA, prec=prec)
+    torch.cuda.synchronize()
+    out = dA.cpu().numpy()
+    exp = nbo.axhelm_mma_n(f16, N, Nf, bk, E, off, g32, D32, l32, q32)
+    ref = nbo.axhelm(N, Nf, bk, E, off, m["ggeo"], m["D"], lam, q)
+    quantum = 2.0 ** -10 if f16 else 2.0 ** -7
+    for f in range(Nf):
+        sl = slice(f * off, f * off + E * Np)
+        scale = np.abs(exp[sl]).max()
+        d = np.abs(out[sl] - exp[sl])
+        assert np.mean(d <= 1e-5 * scale) >= 0.99, (f, np.mean(d <= 1e-5 * scale))
+        assert d.max() <= quantum * scale, (f, d.max() / scale)
+        assert np.abs(out[sl] - ref[sl]).max() <= tol64 * np.abs(ref[sl]).max(), f
+        if Nf > 1:  # the padding between fields is not written
+            assert np.all(out[f * off + E * Np:(f + 1) * off] == 7.0)
 
 
 @pytest.mark.parametrize("N", [3, 7, 9])

@@ -164,6 +164,52 @@ def test_bf16_mfma_oracle_tracks_fp64(nbo):
     assert np.abs(one).max() < 3e-2 * np.abs(ref1).max()
 
 
+@pytest.mark.parametrize("f16", [False, True])
+def test_mfma_oracle_general_form(nbo, f16):
+    """nbo_axhelm_mma_n: at N = 7 with one field it is the N = 7 form bit for
+    bit; at N = 9 with three fields every field is the one-field result of
+    that field (shared geometry and lambda) and tracks the FP64 operator
+    within operand rounding."""
+    rng = np.random.default_rng(5)
+    box = nbo.Box.make(7, 2, 2, 1, perturb=0.1, seed=13)
+    m = nbo.box_mesh(box)
+    E, off = box.E, box.E * 512
+    q = rng.uniform(-1, 1, off).astype(np.float32)
+    lam = np.concatenate([rng.uniform(0.5, 1.5, off), rng.uniform(0.1, 0.5, off)]).astype(np.float32)
+    g32, D32 = m["ggeo"].astype(np.float32), m["D"].astype(np.float32)
+    if not f16:
+        g32, lam = nbo.bf16_round(g32), nbo.bf16_round(lam)
+    for bk in (1, 0):
+        old = (nbo.axhelm_f16mma if f16 else nbo.axhelm_bf16mma)(bk, E, off, g32, D32, lam, q)
+        assert np.array_equal(nbo.axhelm_mma_n(f16, 7, 1, bk, E, off, g32, D32, lam, q), old)
+    box = nbo.Box.make(9, 2, 1, 1, perturb=0.1, seed=14)
+    m = nbo.box_mesh(box)
+    E, Np = box.E, 1000
+    off = E * Np + 40
+    q = np.zeros(3 * off)
+    for f in range(3):
+        q[f * off:f * off + E * Np] = rng.uniform(-1, 1, E * Np)
+    lam = np.zeros(2 * off)
+    lam[:E * Np] = rng.uniform(0.5, 1.5, E * Np)
+    lam[off:off + E * Np] = rng.uniform(0.1, 0.5, E * Np)
+    g32, l32, D32, q32 = (a.astype(np.float32) for a in (m["ggeo"], lam, m["D"], q))
+    if not f16:
+        g32, l32 = nbo.bf16_round(g32), nbo.bf16_round(l32)
+    for bk in (1, 0):
+        out = nbo.axhelm_mma_n(f16, 9, 3, bk, E, off, g32, D32, l32, q32)
+        ref = nbo.axhelm(9, 3, bk, E, off, m["ggeo"], m["D"], lam, q)
+        for f in range(3):
+            sl = slice(f * off, f * off + E * Np)
+            one = np.zeros(E * Np + 40, np.float32)
+            lone = np.zeros(2 * (E * Np + 40), np.float32)
+            lone[:E * Np], lone[E * Np + 40:2 * E * Np + 40] = l32[:E * Np], l32[off:off + E * Np]
+            single = nbo.axhelm_mma_n(f16, 9, 1, bk, E, E * Np + 40, g32, D32, lone, np.ascontiguousarray(
+                np.concatenate([q32[sl], np.zeros(40, np.float32)])))
+            assert np.array_equal(out[sl], single[:E * Np]), (bk, f)
+            err = np.abs(out[sl] - ref[sl]).max() / np.abs(ref[sl]).max()
+            assert err < (4e-3 if f16 else 3e-2), (bk, f, err)
+
+
 def test_f16_rounding_matches_numpy(nbo):
     """The oracle's IEEE binary16 rounding against numpy's float16 conversion:
     random values over the whole range, exact ties between neighbouring fp16
