@@ -39,6 +39,7 @@
 // lambda in FP32 or bf16; the cycle reads the FP64 residual and returns an
 // FP64 correction, and the coarse PCG stays FP64.
 #include "nbx_bp_internal.h"
+#include "nbx_axhelm_impl.h"
 #include "nbx_blas.h"
 #include <cmath>
 #include <type_traits>
@@ -64,6 +65,13 @@ struct nbx_mg {
   double lo = 0.1, hi = 1.1;
   nbx_prec prec = NBX_F64;
   nbx::PcgScalars* coarse_template = nullptr;  // device copy of the coarse solve's initial scalars
+  // coarse CG in two kernels per iteration (coarse level of order 1, every
+  // assembled node with at most 8 members): the member table of the
+  // assembled nodes, the two passes' block partials and the rz slots
+  bool cg_fused = false;
+  int32_t cg_items = 0;
+  int32_t* cg_members = nullptr;  // [items][8], -1 past the node's members
+  double *cg_pa = nullptr, *cg_pb = nullptr, *cg_scal = nullptr;
   bool learned = false;        // fine level ends with the attached network's correction
   double* lrf = nullptr;       // learned step: the fine residual after the cycle (FP64 copy)
   double* lzl = nullptr;       // learned step: element-local network output
@@ -348,8 +356,315 @@ int power_lmax(nbx_bp* bp, uint64_t seed, uint64_t stream_id, int iters, double*
   return NBX_SUCCESS;
 }
 
+// ---- coarse CG in two kernels per iteration ------------------------------
+// The coarse level (order 1) runs K Jacobi-PCG iterations from x = 0:
+//   z = D^-1 r, rz_0 = <r, z>;  for k = 0..K-1:
+//   beta = k ? rz_k / rz_{k-1} : 0;  p = z + beta p;  Ap_L = A_L p (element-local);
+//   pAp = sum_e p_e . (A_L p)_e;  alpha = rz_k / pAp;
+//   per assembled node g: Ap_g = sum of its members' Ap_L (member order of the
+//   gather-scatter); x += alpha p, r -= alpha Ap_g, z = D^-1 r at every member;
+//   rz_{k+1} = sum_g sum_fields r_g z_g;
+//   a non-finite beta or alpha (the residual reached zero) is taken as 0.
+// Pass A (k_cg_pass_a) does the p update, the element operator and pAp's block
+// partials; pass B (k_cg_pass_b) the gather, the updates and rz's block
+// partials.  Each pass combines the other pass's partials itself, every
+// workgroup in the same fixed order, so no separate reduction kernel and no
+// device-scope fence is needed; rz_k goes to slot k % 2 (written by pass A's
+// workgroup 0, read by the next pass B and pass A).
+//   Pass A: a quad of lanes per element (lane t = i + 2j of the 2 x 2 face,
+//   columns k = 0, 1), 64 elements per 256-lane workgroup; the element
+//   operator is axhelm_column's arithmetic for Nq = 2 with the other lanes'
+//   values brought by DPP quad permutes instead of LDS, so no barrier; the
+//   element share is the fused PCG's lane chain (fields, then k) and lane tree
+//   (a0 + a2) + (a1 + a3); the partial is a halving tree over each wave's 16
+//   shares (zero past the last element), then (S0 + S2) + (S1 + S3).
+//   Pass B: 8 lanes per assembled node (lane n = member n), 32 nodes per
+//   workgroup; Ap_g summed from 0 in member order, every member updated; the
+//   node's value fma(r_g, z_g, .) over fields in its lane 0, the others 0,
+//   summed by the 256-lane tree.
+// Partials are combined as tile partials.  The oracle's coarse CG restates
+// all of this order.
+constexpr int kCgEpb = 64;   // elements per pass-A workgroup
+constexpr int kCgNpb = 32;   // assembled nodes per pass-B workgroup
+
+// the combined value of nb partials in the tile-partial order, in every lane
+// of a 256-lane workgroup
+__device__ __forceinline__ double cg_combine(const double* __restrict__ part, unsigned nb, double* s4)
+{
+  const int tid = (int)threadIdx.x;
+  double acc = 0.0;
+  unsigned i = (unsigned)tid;
+  for (; i + 7 * 256 < nb; i += 8 * 256) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = part[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = acc + t[u];
+  }
+  for (; i < nb; i += 256) acc = acc + part[i];
+  const double w = wave_sum(acc);
+  if ((tid & 63) == 0) s4[tid >> 6] = w;
+  __syncthreads();
+  const double tot = (s4[0] + s4[2]) + (s4[1] + s4[3]);
+  __syncthreads();  // s4 is reused
+  return tot;
+}
+
+// the value of the quad lane that pattern Q (quad_perm) names for this lane
+template <int Q>
+__device__ __forceinline__ double quad(double v)
+{
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, Q, 0xf, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), Q, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// quad_perm patterns: lane t = i + 2j reads lane 2j (m = 0 along r), 2j + 1
+// (m = 1 along r), i (m = 0 along s), i + 2 (m = 1 along s)
+constexpr int kQr0 = 0 | 0 << 2 | 2 << 4 | 2 << 6, kQr1 = 1 | 1 << 2 | 3 << 4 | 3 << 6;
+constexpr int kQs0 = 0 | 1 << 2 | 0 << 4 | 1 << 6, kQs1 = 2 | 3 << 2 | 2 << 4 | 3 << 6;
+
+template <int NF, bool BK>
+__global__ __launch_bounds__(256) void k_cg_pass_a(const int32_t E, const int32_t offset,
+                                                   const double* __restrict__ ggeo, const double* __restrict__ D,
+                                                   const double* __restrict__ lambda, const double* __restrict__ z,
+                                                   double* __restrict__ p, double* __restrict__ Ap,
+                                                   double* __restrict__ pa, const double* __restrict__ pb,
+                                                   const unsigned nbB, double* __restrict__ scal, const int k,
+                                                   const int* __restrict__ skip)
+{
+  constexpr int NP = 8;
+  if (skip && *skip) return;
+  __shared__ double s4[4];
+  const int tid = (int)threadIdx.x, lane = tid & 63, t = tid & 3, i = t & 1, j = t >> 1;
+  // workgroup w runs on XCD w % 8: XCD x takes the x-th eighth of the element
+  // blocks, so its L2 holds the same part of the mesh in both passes
+  const unsigned per = gridDim.x / 8, xb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int64_t e = (int64_t)xb * kCgEpb + (tid >> 2);
+  if (xb * kCgEpb >= (unsigned)E) return;
+  const bool active = e < E;  // an idle quad computes on zeros and stores nothing
+  const double D00 = D[0], D01 = D[1], D10 = D[2], D11 = D[3];
+  const double Di0 = i ? D10 : D00, Di1 = i ? D11 : D01, Dj0 = j ? D10 : D00, Dj1 = j ? D11 : D01;
+  const double DTi0 = i ? D01 : D00, DTi1 = i ? D11 : D10, DTj0 = j ? D01 : D00, DTj1 = j ? D11 : D10;
+  const double Dk[2][2] = {{D00, D01}, {D10, D11}};  // Dk[k][m] = D[k][m]
+  double G[2][6], lam0[2], wl1[2];
+  const double* g = ggeo + e * (int64_t)(kNggeo * NP) + t;
+#pragma unroll
+  for (int kk = 0; kk < 2; ++kk) {
+    const int n = kk * 4;
+    G[kk][0] = active ? ldnt(&g[kG00 * NP + n]) : 0.0;
+    G[kk][1] = active ? ldnt(&g[kG01 * NP + n]) : 0.0;
+    G[kk][2] = active ? ldnt(&g[kG02 * NP + n]) : 0.0;
+    G[kk][3] = active ? ldnt(&g[kG11 * NP + n]) : 0.0;
+    G[kk][4] = active ? ldnt(&g[kG12 * NP + n]) : 0.0;
+    G[kk][5] = active ? ldnt(&g[kG22 * NP + n]) : 0.0;
+    lam0[kk] = (!BK && active) ? ldnt(&lambda[e * NP + n + t]) : 0.0;
+    wl1[kk] = (!BK && active) ? ldnt(&g[kGWJ * NP + n]) * ldnt(&lambda[e * NP + offset + n + t]) : 0.0;
+  }
+  // this element's p and z (every field) are loaded before the partials are
+  // combined, so their latency overlaps the combination's
+  double pv[NF][2], zv[NF][2];
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int64_t id = e * NP + (int64_t)f * offset + t + kk * 4;
+      pv[f][kk] = active ? p[id] : 0.0;
+      zv[f][kk] = active ? ldnt(&z[id]) : 0.0;
+    }
+  const double rz = cg_combine(pb, nbB, s4);
+  double beta = k == 0 ? 0.0 : rz / scal[(k - 1) & 1];
+  if (!isfinite(beta)) beta = 0.0;  // r reached zero: the iterate stays
+  if (xb == 0 && tid == 0) scal[k & 1] = rz;
+  double acc = 0.0;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int64_t base = e * NP + (int64_t)f * offset + t;
+    double pk[2], out[2];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      pk[kk] = active ? __builtin_fma(beta, pv[f][kk], zv[f][kk]) : 0.0;
+      if (active) p[base + kk * 4] = pk[kk];
+    }
+    // axhelm_column for Nq = 2 (same operations in the same order)
+    double rAt[2] = {0.0, 0.0}, rArs[2];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const double q = pk[kk];
+      double qr = 0.0, qs = 0.0, qt = 0.0;
+      qr = __builtin_fma(Di0, quad<kQr0>(q), qr);
+      qs = __builtin_fma(Dj0, quad<kQs0>(q), qs);
+      qt = __builtin_fma(Dk[kk][0], pk[0], qt);
+      qr = __builtin_fma(Di1, quad<kQr1>(q), qr);
+      qs = __builtin_fma(Dj1, quad<kQs1>(q), qs);
+      qt = __builtin_fma(Dk[kk][1], pk[1], qt);
+      const double G00 = G[kk][0], G01 = G[kk][1], G02 = G[kk][2], G11 = G[kk][3], G12 = G[kk][4],
+                   G22 = G[kk][5];
+      double Gr = __builtin_fma(G02, qt, __builtin_fma(G01, qs, G00 * qr));
+      double Gs = __builtin_fma(G12, qt, __builtin_fma(G11, qs, G01 * qr));
+      double Gt = __builtin_fma(G22, qt, __builtin_fma(G12, qs, G02 * qr));
+      if (!BK) {
+        Gr = lam0[kk] * Gr;
+        Gs = lam0[kk] * Gs;
+        Gt = lam0[kk] * Gt;
+      }
+      rAt[0] = __builtin_fma(Dk[kk][0], Gt, rAt[0]);
+      rAt[1] = __builtin_fma(Dk[kk][1], Gt, rAt[1]);
+      double ar = 0.0, as = 0.0;
+      ar = __builtin_fma(DTi0, quad<kQr0>(Gr), ar);
+      as = __builtin_fma(DTj0, quad<kQs0>(Gs), as);
+      ar = __builtin_fma(DTi1, quad<kQr1>(Gr), ar);
+      as = __builtin_fma(DTj1, quad<kQs1>(Gs), as);
+      rArs[kk] = ar + as;
+    }
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      double res = rArs[kk] + rAt[kk];
+      if (!BK) res = __builtin_fma(wl1[kk], pk[kk], res);
+      out[kk] = res;
+      if (active) stnt(res, &Ap[base + kk * 4]);
+      acc = __builtin_fma(pk[kk], out[kk], acc);
+    }
+  }
+  // element share (a0 + a2) + (a1 + a3); zero for an idle quad
+  const double s2 = acc + quad<(2 | 3 << 2 | 0 << 4 | 1 << 6)>(acc);
+  double sh = s2 + quad<(1 | 0 << 2 | 3 << 4 | 2 << 6)>(s2);
+  sh = active ? sh : 0.0;
+  // the wave's 16 shares (lanes 4s): halving tree s += s + h, h = 8, 4, 2, 1
+#pragma unroll
+  for (int h = 8; h >= 1; h >>= 1) sh = sh + __shfl(sh, (lane + 4 * h) & 63, 64);
+  if (lane == 0) s4[tid >> 6] = sh;
+  __syncthreads();
+  if (tid == 0) pa[xb] = (s4[0] + s4[2]) + (s4[1] + s4[3]);
+}
+
+// pass B (UPD = false: the initial pass, z = D^-1 r and rz_0 only)
+template <int NF, bool UPD>
+__global__ __launch_bounds__(256) void k_cg_pass_b(const int32_t items, const int32_t* __restrict__ members,
+                                                   const int32_t offset, const double* __restrict__ invD,
+                                                   const double* __restrict__ p, const double* __restrict__ Ap,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ z, const double* __restrict__ pa,
+                                                   const unsigned nbA, const double* __restrict__ scal, const int k,
+                                                   double* __restrict__ pb, const int* __restrict__ skip)
+{
+  if (skip && *skip) return;
+  __shared__ double s4[4];
+  const int tid = (int)threadIdx.x, lane = tid & 63, g0 = lane & ~7, n = tid & 7;
+  const unsigned per = gridDim.x / 8, xb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);  // as pass A
+  if (xb * kCgNpb >= (unsigned)items) return;
+  const int32_t item = (int32_t)xb * kCgNpb + (tid >> 3);
+  const int32_t m = item < items ? members[(int64_t)item * 8 + n] : -1;
+  const double di = m >= 0 ? invD[m] : 0.0;
+  // the member's values (every field) before the partials are combined
+  double rv[NF], av[NF], pv[NF], xv[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int64_t id = (int64_t)f * offset + (m >= 0 ? m : 0);
+    rv[f] = m >= 0 ? r[id] : 0.0;
+    av[f] = (UPD && m >= 0) ? Ap[id] : 0.0;
+    pv[f] = (UPD && m >= 0) ? p[id] : 0.0;
+    xv[f] = (UPD && m >= 0) ? x[id] : 0.0;
+  }
+  double alpha = 0.0;
+  if (UPD) {
+    alpha = scal[k & 1] / cg_combine(pa, nbA, s4);
+    if (!isfinite(alpha)) alpha = 0.0;
+  }
+  double v[1] = {0.0};
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const int64_t id = (int64_t)f * offset + (m >= 0 ? m : 0);
+    double rr = rv[f];
+    if (UPD) {
+      const double a = av[f];
+      double ag = 0.0;  // the gather-scatter's sum, in its member order
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double aq = __shfl(a, g0 + q, 64);
+        const int32_t mq = __shfl(m, g0 + q, 64);
+        if (mq >= 0) ag = ag + aq;
+      }
+      rr = __builtin_fma(-alpha, ag, rr);
+      if (m >= 0) {
+        x[id] = __builtin_fma(alpha, pv[f], xv[f]);
+        r[id] = rr;
+      }
+    }
+    const double zz = di * rr;
+    if (m >= 0) z[id] = zz;
+    if (n == 0) v[0] = __builtin_fma(rr, zz, v[0]);
+  }
+  block_sum256<1>(v, s4);
+  if (tid == 0) pb[xb] = v[0];
+}
+
+// the member table of the coarse level's assembled nodes (gather-scatter
+// group order; nodes with more than 8 members leave the fused CG off)
+static int cg_setup(nbx_mg* mg, nbx_bp* b)
+{
+  nbx_dlong N = 0, Nl = 0, G = 0;
+  if (nbx_ogs_sizes(b->ogs, &N, &Nl, &G, nullptr, nullptr) != NBX_SUCCESS)
+    return fail(NBX_ERR_HIP, "mg: coarse ogs sizes");
+  std::vector<nbx_dlong> off((size_t)G + 1), ids((size_t)std::max<nbx_dlong>(Nl, 1));
+  if (nbx_ogs_copy_csr(b->ogs, off.data(), ids.data()) != NBX_SUCCESS) return fail(NBX_ERR_HIP, "mg: coarse csr");
+  std::vector<int32_t> tab((size_t)std::max<nbx_dlong>(G, 1) * 8, -1);
+  for (nbx_dlong g = 0; g < G; ++g) {
+    const nbx_dlong n = off[g + 1] - off[g];
+    if (n > 8) return NBX_SUCCESS;  // stays on the generic PCG
+    for (nbx_dlong q = 0; q < n; ++q) tab[(size_t)g * 8 + q] = ids[off[g] + q];
+  }
+  int rc;
+  const unsigned nbA = (unsigned)((b->E + kCgEpb - 1) / kCgEpb), nbB = (unsigned)((G + kCgNpb - 1) / kCgNpb);
+  if ((rc = mg_alloc(mg, &mg->cg_members, (int64_t)tab.size())) || (rc = mg_alloc(mg, &mg->cg_pa, nbA)) ||
+      (rc = mg_alloc(mg, &mg->cg_pb, nbB)) || (rc = mg_alloc(mg, &mg->cg_scal, 2)))
+    return rc;
+  NBX_HIP(hipMemcpy(mg->cg_members, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice));
+  // masked nodes belong to no assembled node: their z and p stay zero
+  NBX_HIP(hipMemset(b->z, 0, sizeof(double) * (size_t)b->Nall));
+  NBX_HIP(hipMemset(b->p, 0, sizeof(double) * (size_t)b->Nall));
+  mg->cg_items = (int32_t)G;
+  mg->cg_fused = true;
+  return NBX_SUCCESS;
+}
+
+static int coarse_cg(nbx_mg* mg, nbx_mg_level& L, const int* skip)
+{
+  nbx_bp* bp = L.bp;
+  double* x = static_cast<double*>(L.x);
+  double* r = static_cast<double*>(L.r);
+  const unsigned nbA = (unsigned)((bp->E + kCgEpb - 1) / kCgEpb),
+                 nbB = (unsigned)((mg->cg_items + kCgNpb - 1) / kCgNpb);
+  hipLaunchKernelGGL(k_mg_fill0, mg_grid(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, x, skip);
+  const dim3 gA(8 * ((nbA + 7) / 8)), gB(8 * ((nbB + 7) / 8));  // eight XCD chunks of blocks
+#define NBX_CGB(NFV, UPD, KK)                                                                                   \
+  hipLaunchKernelGGL((k_cg_pass_b<NFV, UPD>), gB, dim3(256), 0, bp->stream, mg->cg_items, mg->cg_members,          \
+                     bp->fieldOffset, bp->invDiagA, bp->p, bp->Ap, x, r, bp->z, mg->cg_pa, nbA, mg->cg_scal, KK,   \
+                     mg->cg_pb, skip)
+#define NBX_CGA(NFV, BKV, KK)                                                                                  \
+  hipLaunchKernelGGL((k_cg_pass_a<NFV, BKV>), gA, dim3(256), 0, bp->stream, bp->E, bp->fieldOffset,                \
+                     bp->ggeo, bp->D, bp->lambda, bp->z, bp->p, bp->Ap, mg->cg_pa, mg->cg_pb, nbB, mg->cg_scal, KK, \
+                     skip)
+  const bool bk = bp->opt.bpmode != 0;
+  if (bp->Nf == 1) NBX_CGB(1, false, 0); else NBX_CGB(3, false, 0);
+  for (int k = 0; k < mg->coarse_iters; ++k) {
+    if (bp->Nf == 1) {
+      if (bk) NBX_CGA(1, true, k); else NBX_CGA(1, false, k);
+      NBX_CGB(1, true, k);
+    } else {
+      if (bk) NBX_CGA(3, true, k); else NBX_CGA(3, false, k);
+      NBX_CGB(3, true, k);
+    }
+  }
+#undef NBX_CGA
+#undef NBX_CGB
+  NBX_HIP_LAUNCH();
+  return NBX_SUCCESS;
+}
+
 int coarse_solve(nbx_mg* mg, nbx_mg_level& L, const int* skip)
 {
+  if (mg->cg_fused) return coarse_cg(mg, L, skip);
   nbx_bp* bp = L.bp;
   double* x = static_cast<double*>(L.x);
   hipLaunchKernelGGL(k_mg_fill0, mg_grid(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, x, skip);
@@ -681,6 +996,7 @@ extern "C" int nbx_bp_mg_setup(nbx_bp* bp, int nlevels, const int* orders, int d
         rc = fail(NBX_ERR_ALLOC, "bp_mg_setup: coarse scalars");
         break;
       }
+      if (b->N == 1 && (rc = cg_setup(mg, b))) break;  // order 1: the two-pass coarse CG
     }
   }
   if (rc == NBX_SUCCESS && hipStreamSynchronize(bp->stream) != hipSuccess)

@@ -475,6 +475,124 @@ static void mg_learned_step(const nbo_mg* mg, const double* rf, double* x)
   free(zl);
 }
 
+/* Coarse solve of order 1 in the library's two-pass form (nbx_mg.hip,
+ * k_cg_pass_a / k_cg_pass_b): K Jacobi-PCG iterations from x = 0 with
+ *   rz_k = sum over assembled nodes g (gather-scatter group order) of
+ *          sum over fields fma(r_g, z_g, .); 32 nodes per partial, node j at
+ *          lane 8j of the 256-lane tree (the other lanes zero), partials
+ *          combined as tile partials;
+ *   pAp  = element shares in the fused PCG's order (lane chains over fields
+ *          then k, lane tree (a0 + a2) + (a1 + a3)); 64 shares per partial:
+ *          a halving tree over each group of 16 (zero past the last element),
+ *          then (S0 + S2) + (S1 + S3); partials combined as tile partials;
+ *   the gathered Ap_g summed from 0 in member order, x += alpha p,
+ *   r -= alpha Ap_g, z = D^-1 r at every member; a non-finite beta or alpha
+ *   (the residual reached zero) is taken as 0, so the iterate stays.
+ * Returns 0, or -1 when an assembled node has more than 8 members (the
+ * library then runs the generic PCG, restated by pcg_core). */
+static int coarse_cg(const nbo_bp* bp, const double* invD, double* r, double* x, int K)
+{
+  enum { EPB = 64 };
+  const nbo_ogs* o = bp->ogs;
+  const int32_t G = o->NlocalGather;
+  for (int32_t g = 0; g < G; ++g)
+    if (o->offsets[g + 1] - o->offsets[g] > 8) return -1;
+  const int Nq2 = 4, Np = 8, Nf = bp->Nf;
+  const int64_t n = (int64_t)bp->E * Np, Nall = (int64_t)Nf * bp->offset;
+  const int64_t nbA = ((int64_t)bp->E + EPB - 1) / EPB, nbB = ((int64_t)G + 31) / 32;
+  double* z = (double*)calloc((size_t)Nall, sizeof(double));
+  double* p = (double*)calloc((size_t)Nall, sizeof(double));
+  double* Ap = (double*)calloc((size_t)Nall, sizeof(double));
+  double* share = (double*)calloc((size_t)(bp->E > 0 ? bp->E : 1), sizeof(double));
+  double* pa = (double*)calloc((size_t)(nbA > 0 ? nbA : 1), sizeof(double));
+  double* pb = (double*)calloc((size_t)(nbB > 0 ? nbB : 1), sizeof(double));
+  double scal[2] = {0.0, 0.0};
+  memset(x, 0, sizeof(double) * (size_t)Nall);
+  for (int k = -1; k < K; ++k) {
+    double alpha = 0.0;
+    if (k >= 0) {
+      /* pass A */
+      const double rz = nbo_tree_sum_partials(nbB, pb);
+      double beta = k == 0 ? 0.0 : rz / scal[(k - 1) & 1];
+      if (!isfinite(beta)) beta = 0.0; /* r reached zero: the iterate stays */
+      scal[k & 1] = rz;
+      for (int f = 0; f < Nf; ++f)
+        for (int64_t i = 0; i < n; ++i) {
+          const int64_t id = i + (int64_t)f * bp->offset;
+          p[id] = fma(beta, p[id], z[id]);
+        }
+      nbo_axhelm(1, Nf, bp->bk, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, p, Ap);
+#pragma omp parallel for schedule(static)
+      for (int32_t e = 0; e < bp->E; ++e) {
+        double lanes[4];
+        for (int t = 0; t < Nq2; ++t) {
+          double acc = 0.0;
+          for (int f = 0; f < Nf; ++f)
+            for (int kk = 0; kk < 2; ++kk) {
+              const int64_t id = (int64_t)e * Np + (int64_t)kk * Nq2 + t + (int64_t)f * bp->offset;
+              acc = fma(p[id], Ap[id], acc);
+            }
+          lanes[t] = acc;
+        }
+        for (int h = 2; h >= 1; h >>= 1)
+          for (int t = 0; t < h; ++t) lanes[t] = lanes[t] + lanes[t + h];
+        share[e] = lanes[0];
+      }
+      for (int64_t b = 0; b < nbA; ++b) {
+        double S[4];
+        for (int w = 0; w < 4; ++w) {
+          double sh[16];
+          for (int t = 0; t < 16; ++t) {
+            const int64_t e = b * EPB + w * 16 + t;
+            sh[t] = e < bp->E ? share[e] : 0.0;
+          }
+          for (int h = 8; h >= 1; h >>= 1)
+            for (int t = 0; t < h; ++t) sh[t] = sh[t] + sh[t + h];
+          S[w] = sh[0];
+        }
+        pa[b] = (S[0] + S[2]) + (S[1] + S[3]);
+      }
+      alpha = scal[k & 1] / nbo_tree_sum_partials(nbA, pa);
+      if (!isfinite(alpha)) alpha = 0.0;
+    }
+    /* pass B */
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < nbB; ++b) {
+      double lane[256];
+      for (int t = 0; t < 256; ++t) lane[t] = 0.0;
+      for (int j = 0; j < 32; ++j) {
+        const int64_t g = b * 32 + j;
+        double v = 0.0;
+        if (g < G) {
+          const int32_t s0 = o->offsets[g], s1 = o->offsets[g + 1];
+          const double di = invD[o->gatherIds[s0]];
+          for (int f = 0; f < Nf; ++f) {
+            const int64_t fo = (int64_t)f * bp->offset;
+            double rr = r[fo + o->gatherIds[s0]];
+            if (k >= 0) {
+              double ag = 0.0;
+              for (int32_t q = s0; q < s1; ++q) ag = ag + Ap[fo + o->gatherIds[q]];
+              rr = fma(-alpha, ag, rr);
+              for (int32_t q = s0; q < s1; ++q) {
+                const int64_t id = fo + o->gatherIds[q];
+                x[id] = fma(alpha, p[id], x[id]);
+                r[id] = rr;
+              }
+            }
+            const double zz = di * rr;
+            for (int32_t q = s0; q < s1; ++q) z[fo + o->gatherIds[q]] = zz;
+            v = fma(rr, zz, v);
+          }
+        }
+        lane[8 * j] = v;
+      }
+      pb[b] = nbo_tree256(lane);
+    }
+  }
+  free(z); free(p); free(Ap); free(share); free(pa); free(pb);
+  return 0;
+}
+
 /* V-cycle at level l: x = M_l r (x overwritten) */
 void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
 {
@@ -491,9 +609,11 @@ void nbo_mg_vcycle(const nbo_mg* mg, int l, const double* r, double* x)
   if (l == mg->nlev - 1) {  /* coarse: fixed-count Jacobi PCG from x = 0 (r is copied) */
     double* rr = (double*)malloc(sizeof(double) * Nall);
     memcpy(rr, r, sizeof(double) * Nall);
-    memset(x, 0, sizeof(double) * Nall);
-    nbo_pcg_stats st;
-    pcg_core(bp, 1, 0, 1, NULL, invD, rr, x, 0.0, 0.0, mg->coarse_iters, 0, NULL, NULL, &st);
+    if (bp->N != 1 || coarse_cg(bp, invD, rr, x, mg->coarse_iters) != 0) {
+      memset(x, 0, sizeof(double) * Nall);
+      nbo_pcg_stats st;
+      pcg_core(bp, 1, 0, 1, NULL, invD, rr, x, 0.0, 0.0, mg->coarse_iters, 0, NULL, NULL, &st);
+    }
     free(rr);
     return;
   }
