@@ -411,6 +411,25 @@ def run_graded_64x64x32(nbx, torch, args):
     bm = nbx.BP(box, bpmode=True, precon=nbx.PRECON_MG)
     bm.mg_setup([7, 3, 1], degree=3, coarse_iters=30, prec=nbx.NBX_BF16)
     run("mixed_mg_bf16cycle", lambda: bm.pcg(r, x, 1e-10, 500))
+    if not args.no_learned:
+        # the learned step, trained on the multigrid pairs of a 16x16x32 patch with the
+        # same element shapes (x, y extent 1/4, the same z grading), then run on this mesh
+        from nekbench_amd.learned import train_mg_network
+        patch = nbx.BP(nbx.Box.make(7, 16, 16, 32, perturb=0.1, zaspect=10.0, seed=2,
+                                    extent=(0.0, 0.25, 0.0, 0.25, 0.0, 1.0)), bpmode=True, precon=nbx.PRECON_MG)
+        patch.mg_setup([7, 3, 1], degree=3, coarse_iters=30, prec=nbx.NBX_BF16)
+        mnet, minfo = train_mg_network(nbx, torch, patch, seeds=(1000, args.mg_pair_seeds), kcount=8,
+                                       steps=args.train_steps_mg, alpha=args.mg_alpha, log=log)
+        patch.free()
+        mnet.set_inference(True)
+        mnet.attach(bm)
+        bm.mg_set_learned(True)
+        run("learned_mixed_mg_bf16cycle", lambda: bm.pcg(r, x, 1e-10, 500))
+        out["learned_mixed_mg_bf16cycle"]["training"] = dict(minfo, mesh="16x16x32 patch, extent 1/4 x 1/4 x 1")
+        out["learned_mixed_mg_bf16cycle"]["gain_vs_same_mg"] = round(
+            out["mixed_mg_bf16cycle"]["time_s"] / out["learned_mixed_mg_bf16cycle"]["time_s"], 3)
+        bm.mg_set_learned(False)
+        mnet.free()
     bm.free()
     small = nbx.BP(nbx.Box.make(7, 16, 16, 8, perturb=0.1, zaspect=10.0, seed=2), bpmode=True,
                    precon=nbx.PRECON_LEARNED)
