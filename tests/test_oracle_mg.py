@@ -155,3 +155,60 @@ def test_learned_step_with_damped_jacobi_network_is_a_jacobi_step(nbo, prec, tol
     exp = z + (1 - alpha) * step
     assert np.abs(zl - exp).max() <= tol * (1 - alpha) * np.abs(step).max() + 1e-13 * np.abs(exp).max()
     assert np.abs(zl - z).max() > 1e-3 * np.abs(z).max()  # the step is not negligible
+
+
+@pytest.mark.parametrize("Nf,bpmode,lambda1", [(1, True, 0.0), (3, False, 0.1)])
+def test_coarse_cg_is_textbook_jacobi_pcg(nbo, Nf, bpmode, lambda1):
+    """The order-1 coarse solve (coarse_cg in oracle/nbo_pcg.c, the restatement
+    of the library's two-pass CG) is Jacobi-preconditioned CG from x = 0 on the
+    assembled system: after K iterations it equals textbook PCG in numpy on the
+    dense assembled matrix (fields coupled only through the inner products), to
+    rounding; every local copy of a node holds the same value, masked nodes
+    stay zero; with many iterations it reaches the direct solve."""
+    box = nbo.Box.make(1, 4, 3, 3, perturb=0.1, seed=3)
+    for K, tol in ((6, 1e-10), (200, 1e-9)):
+        mg = nbo.OracleMG(box, [1], degree=2, coarse_iters=K, bpmode=bpmode, lambda1=lambda1, Nf=Nf)
+        ob = mg.fine
+        n = ob.offset
+        gid = ob.mesh["ids"]
+        keep = np.setdiff1d(np.arange(n), ob.maskIds)
+        uniq, first, inv = np.unique(gid[keep], return_index=True, return_inverse=True)
+        reps = keep[first]
+        cols = []
+        for g in uniq:
+            e = np.zeros(Nf * n)
+            e[:n][gid == g] = 1.0
+            cols.append(ob.apply(e)[reps])
+        A = np.array(cols).T
+        assert np.allclose(A, A.T, rtol=1e-12, atol=1e-14 * np.abs(A).max())
+        Minv = 1.0 / np.diag(A)
+        assert np.allclose(Minv, mg.invD[0][reps], rtol=1e-13)
+        r = ob.rhs.copy()
+        b = np.stack([r[f * n + reps] for f in range(Nf)])
+        # textbook PCG on the block-diagonal system, dots summed over fields
+        x = np.zeros_like(b)
+        rr = b.copy()
+        z = Minv * rr
+        pvec = z.copy()
+        rz = np.sum(rr * z)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for _ in range(K):  # once the residual is exactly zero, 0/0 steps are taken as 0
+                Ap = pvec @ A.T
+                alpha = rz / np.sum(pvec * Ap)
+                alpha = alpha if np.isfinite(alpha) else 0.0
+                x += alpha * pvec
+                rr -= alpha * Ap
+                z = Minv * rr
+                rz_new = np.sum(rr * z)
+                beta = rz_new / rz
+                pvec = z + (beta if np.isfinite(beta) else 0.0) * pvec
+                rz = rz_new
+        out = mg.vcycle(r)
+        for f in range(Nf):
+            of = out[f * n:(f + 1) * n]
+            assert np.abs(of[reps] - x[f]).max() <= tol * np.abs(x[f]).max()
+            assert np.array_equal(of[keep], of[reps][inv])  # every copy identical
+            assert not np.any(of[ob.maskIds])
+        if K == 200:
+            xd = np.linalg.solve(A, b.T).T
+            assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
