@@ -393,11 +393,13 @@ void nbo_ogs_free(nbo_ogs* o)
  * :148-176 mul, :300-330 min, :455-485 max): per group and field, add/mul
  * start from 0/1 and combine every member in ascending order; min/max start
  * from the first member and compare the rest with (b < acc) ? b : acc.  The
- * result goes back to every member; singletons are skipped. */
+ * result goes back to every member; singletons are skipped.  Groups are
+ * disjoint, so they run in parallel with the same result. */
 #define GS_LOOP(T, FIRST, INIT, COMB)                                             \
   do {                                                                            \
     T* v = (T*)q;                                                                 \
     for (int f = 0; f < k; ++f)                                                   \
+      _Pragma("omp parallel for schedule(static)")                                \
       for (int32_t g = 0; g < o->NlocalGather; ++g) {                            \
         const int32_t s = o->offsets[g], e = o->offsets[g + 1];                   \
         if (s + 1 == e) continue;                                                 \
@@ -432,6 +434,79 @@ int nbo_gs_many(const nbo_ogs* o, void* q, int k, int32_t stride, int type, int 
 }
 
 /* ------------------------------------------------------------------------ */
+/* One element at a compile-time Nq (one copy per order, below; kept outside
+ * the OpenMP region so the order stays a constant inside it), with D also
+ * held transposed, so that every loop over i reads contiguous data and the
+ * compiler vectorises across the nodes of a row; each node's fma chains are
+ * the ones written here, in this order, whatever the vector width. */
+static inline __attribute__((always_inline)) void axhelm_elem(
+    const int Nq, const int64_t e, const int Nf, const int bk, const int32_t offset, const double* restrict ggeo,
+    const double* restrict D, const double* restrict DT, const double* restrict lambda, const double* restrict q,
+    double* restrict Aq, double* restrict s_q, double* restrict s_r, double* restrict s_s, double* restrict s_t)
+{
+  const int Nq2 = Nq * Nq, Np = Nq2 * Nq;
+  const double* restrict g = ggeo + e * (int64_t)NGGEO * Np;
+  for (int f = 0; f < Nf; ++f) {
+    const double* restrict qe = q + e * Np + (int64_t)f * offset;
+    double* restrict Aqe = Aq + e * Np + (int64_t)f * offset;
+    for (int n = 0; n < Np; ++n) s_q[n] = qe[n];
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const int n = i + j * Nq + k * Nq2;
+          double qr = 0.0, qs = 0.0, qt = 0.0;
+#pragma GCC unroll 16
+          for (int m = 0; m < Nq; ++m) {
+            qr = fma(DT[m * Nq + i], s_q[m + j * Nq + k * Nq2], qr);
+            qs = fma(D[j * Nq + m], s_q[i + m * Nq + k * Nq2], qs);
+            qt = fma(D[k * Nq + m], s_q[i + j * Nq + m * Nq2], qt);
+          }
+          const double G00 = g[G00ID * Np + n], G01 = g[G01ID * Np + n], G02 = g[G02ID * Np + n];
+          const double G11 = g[G11ID * Np + n], G12 = g[G12ID * Np + n], G22 = g[G22ID * Np + n];
+          double Gr = fma(G02, qt, fma(G01, qs, G00 * qr));
+          double Gs = fma(G12, qt, fma(G11, qs, G01 * qr));
+          double Gt = fma(G22, qt, fma(G12, qs, G02 * qr));
+          if (!bk) {
+            const double l0 = lambda[e * Np + n];
+            Gr = l0 * Gr; Gs = l0 * Gs; Gt = l0 * Gt;
+          }
+          s_r[n] = Gr; s_s[n] = Gs; s_t[n] = Gt;
+        }
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const int n = i + j * Nq + k * Nq2;
+          double ar = 0.0, as = 0.0, at = 0.0;
+#pragma GCC unroll 16
+          for (int m = 0; m < Nq; ++m) {
+            ar = fma(D[m * Nq + i], s_r[m + j * Nq + k * Nq2], ar);
+            as = fma(D[m * Nq + j], s_s[i + m * Nq + k * Nq2], as);
+            at = fma(D[m * Nq + k], s_t[i + j * Nq + m * Nq2], at);
+          }
+          double res = (ar + as) + at;
+          if (!bk) {
+            const double gl = g[GWJID * Np + n] * lambda[e * Np + n + offset];
+            res = fma(gl, s_q[n], res);
+          }
+          Aqe[n] = res;
+        }
+  }
+}
+
+typedef void (*axhelm_elem_fn)(int64_t, int, int, int32_t, const double*, const double*, const double*,
+                               const double*, const double*, double*, double*, double*, double*, double*);
+#define AXHELM_NQ(NQ)                                                                                  \
+  static void axhelm_elem##NQ(int64_t e, int Nf, int bk, int32_t offset, const double* ggeo,           \
+                              const double* D, const double* DT, const double* lambda, const double* q, \
+                              double* Aq, double* s_q, double* s_r, double* s_s, double* s_t)            \
+  {                                                                                                    \
+    if (bk) axhelm_elem(NQ, e, Nf, 1, offset, ggeo, D, DT, lambda, q, Aq, s_q, s_r, s_s, s_t);          \
+    else axhelm_elem(NQ, e, Nf, 0, offset, ggeo, D, DT, lambda, q, Aq, s_q, s_r, s_s, s_t);             \
+  }
+AXHELM_NQ(2) AXHELM_NQ(3) AXHELM_NQ(4) AXHELM_NQ(5) AXHELM_NQ(6) AXHELM_NQ(7) AXHELM_NQ(8)
+AXHELM_NQ(9) AXHELM_NQ(10) AXHELM_NQ(11) AXHELM_NQ(12) AXHELM_NQ(13) AXHELM_NQ(14) AXHELM_NQ(15)
+AXHELM_NQ(16)
+
 /* axhelm_v0 / axhelm_n3_v0 / axhelm_bk_v0 / axhelm_bk_n3_v0 and their Partial
  * forms (axhelm/kernel/CPU/axhelm.c:27-413; elementList as in
  * axhelm/kernel/VOLTA/axhelm.okl:158-165).  Per node:
@@ -442,62 +517,25 @@ void nbo_axhelm(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const 
                 const double* ggeo, const double* D, const double* lambda,
                 const double* q, double* Aq)
 {
-  const int Nq = N + 1, Nq2 = Nq * Nq, Np = Nq2 * Nq;
+  static const axhelm_elem_fn fns[15] = {axhelm_elem2,  axhelm_elem3,  axhelm_elem4,  axhelm_elem5,
+                                         axhelm_elem6,  axhelm_elem7,  axhelm_elem8,  axhelm_elem9,
+                                         axhelm_elem10, axhelm_elem11, axhelm_elem12, axhelm_elem13,
+                                         axhelm_elem14, axhelm_elem15, axhelm_elem16};
+  if (N < 1 || N > 15) return;
+  const axhelm_elem_fn fn = fns[N - 1];
+  const int Nq = N + 1, Np = Nq * Nq * Nq;
+  double DT[16 * 16];
+  for (int i = 0; i < Nq; ++i)
+    for (int m = 0; m < Nq; ++m) DT[m * Nq + i] = D[i * Nq + m];
 #pragma omp parallel
   {
-    double* s_q = (double*)malloc(sizeof(double) * Np);
-    double* s_r = (double*)malloc(sizeof(double) * Np);
-    double* s_s = (double*)malloc(sizeof(double) * Np);
-    double* s_t = (double*)malloc(sizeof(double) * Np);
+    double* s_q = (double*)malloc(sizeof(double) * 4 * Np);
 #pragma omp for schedule(static)
     for (int32_t ee = 0; ee < Nelements; ++ee) {
       const int64_t e = elementList ? elementList[ee] : ee;
-      const double* g = ggeo + e * (int64_t)NGGEO * Np;
-      for (int f = 0; f < Nf; ++f) {
-        const double* qe = q + e * Np + (int64_t)f * offset;
-        double* Aqe = Aq + e * Np + (int64_t)f * offset;
-        for (int n = 0; n < Np; ++n) s_q[n] = qe[n];
-        for (int k = 0; k < Nq; ++k)
-          for (int j = 0; j < Nq; ++j)
-            for (int i = 0; i < Nq; ++i) {
-              const int n = i + j * Nq + k * Nq2;
-              double qr = 0.0, qs = 0.0, qt = 0.0;
-              for (int m = 0; m < Nq; ++m) {
-                qr = fma(D[i * Nq + m], s_q[m + j * Nq + k * Nq2], qr);
-                qs = fma(D[j * Nq + m], s_q[i + m * Nq + k * Nq2], qs);
-                qt = fma(D[k * Nq + m], s_q[i + j * Nq + m * Nq2], qt);
-              }
-              const double G00 = g[G00ID * Np + n], G01 = g[G01ID * Np + n], G02 = g[G02ID * Np + n];
-              const double G11 = g[G11ID * Np + n], G12 = g[G12ID * Np + n], G22 = g[G22ID * Np + n];
-              double Gr = fma(G02, qt, fma(G01, qs, G00 * qr));
-              double Gs = fma(G12, qt, fma(G11, qs, G01 * qr));
-              double Gt = fma(G22, qt, fma(G12, qs, G02 * qr));
-              if (!bk) {
-                const double l0 = lambda[e * Np + n];
-                Gr = l0 * Gr; Gs = l0 * Gs; Gt = l0 * Gt;
-              }
-              s_r[n] = Gr; s_s[n] = Gs; s_t[n] = Gt;
-            }
-        for (int k = 0; k < Nq; ++k)
-          for (int j = 0; j < Nq; ++j)
-            for (int i = 0; i < Nq; ++i) {
-              const int n = i + j * Nq + k * Nq2;
-              double ar = 0.0, as = 0.0, at = 0.0;
-              for (int m = 0; m < Nq; ++m) {
-                ar = fma(D[m * Nq + i], s_r[m + j * Nq + k * Nq2], ar);
-                as = fma(D[m * Nq + j], s_s[i + m * Nq + k * Nq2], as);
-                at = fma(D[m * Nq + k], s_t[i + j * Nq + m * Nq2], at);
-              }
-              double res = (ar + as) + at;
-              if (!bk) {
-                const double gl = g[GWJID * Np + n] * lambda[e * Np + n + offset];
-                res = fma(gl, s_q[n], res);
-              }
-              Aqe[n] = res;
-            }
-      }
+      fn(e, Nf, bk, offset, ggeo, D, DT, lambda, q, Aq, s_q, s_q + Np, s_q + 2 * Np, s_q + 3 * Np);
     }
-    free(s_q); free(s_r); free(s_s); free(s_t);
+    free(s_q);
   }
 }
 

@@ -138,8 +138,10 @@ static void lo_axhelm(const nbo_bp* bp, const lo_ops* lo, const double* p, doubl
   const int64_t Nall = (int64_t)bp->Nf * bp->offset;
   float* p32 = (float*)malloc(sizeof(float) * Nall);
   float* a32 = (float*)malloc(sizeof(float) * Nall);
+  #pragma omp parallel for schedule(static)
   for (int64_t n = 0; n < Nall; ++n) p32[n] = (float)p[n];
   nbo_axhelm_f32(bp->N, bp->Nf, bp->bk, bp->E, bp->offset, lo->g, lo->D, lo->lam, p32, a32);
+  #pragma omp parallel for schedule(static)
   for (int64_t n = 0; n < Nall; ++n) Ap[n] = (double)a32[n];
   free(p32); free(a32);
 }
@@ -149,6 +151,7 @@ static void local_jacobi(const nbo_bp* bp, double* invDiag)
   const int64_t N = (int64_t)bp->E * (bp->N + 1) * (bp->N + 1) * (bp->N + 1);
   nbo_jacobi_local(bp->N, bp->E, bp->offset, NULL, bp->ggeo, bp->D, bp->lambda, invDiag);
   nbo_gs_many(bp->ogs, invDiag, 1, bp->offset, NBO_DOUBLE, NBO_ADD);
+  #pragma omp parallel for schedule(static)
   for (int64_t n = 0; n < N; ++n) invDiag[n] = 1.0 / invDiag[n];
 }
 
@@ -185,6 +188,7 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
 
   if (initial) {  /* Ax = A x; r = r - Ax (solve.cpp:68-69) */
     nbo_bp_apply(bp, x, Ax);
+    #pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < Nall; ++n) r[n] = -1.0 * Ax[n] + 1.0 * r[n];
   }
   for (it = 1;; ++it) {
@@ -197,6 +201,7 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
       zsrc = z;
     } else if (ln) {  /* z = gs(w invDiag (r + s f(r))), Jacobi after a stall */
       if (fallback) {
+        #pragma omp parallel for schedule(static)
         for (int64_t n = 0; n < N; ++n) z[n] = bp->w[n] * (invDiag[n] * r[n]);
       } else {
         nbo_net_apply(bp->E, ln->W, ln->feat, r, invDiag, bp->w, z);
@@ -243,6 +248,7 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
     if (flexible) beta = -alpha * d[2] / rdotz2;                                    /* :92-95 */
     else beta = (it == 1) ? 0.0 : rdotz1 / rdotz2;                                 /* :97 */
     /* p = z + beta p (:100) */
+    #pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < Nall; ++n) p[n] = fma(beta, p[n], zsrc[n]);
     /* Ap = A p; pAp = (p.Ap)_w; alpha = rdotz/pAp (:102-104) */
     double pAp;
@@ -261,6 +267,7 @@ static int pcg_core_ext(const nbo_bp* bp, int jacobi, int flexible, int fused, c
     if (hist_alpha) hist_alpha[it] = alpha;
     if (!isfinite(alpha)) { breakdown = 1; iterations = it - 1; break; }
     /* x += alpha p; r -= alpha Ap (:108) */
+    #pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < Nall; ++n) {
       x[n] = fma(alpha, p[n], x[n]);
       r[n] = fma(-alpha, Ap[n], r[n]);
@@ -340,6 +347,7 @@ static int pcg_mixed_impl(const nbo_bp* bp, int jacobi, int flexible, int prec, 
   double rr0 = 0.0, rr = 0.0;
   for (;;) {
     nbo_bp_apply(bp, x, Ax);
+    #pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < Nall; ++n) r[n] = -1.0 * Ax[n] + 1.0 * b[n];
     rr = nbo_bp_wdot(bp, r, r);
     if (cycles == 0) rr0 = rr;
@@ -352,6 +360,7 @@ static int pcg_mixed_impl(const nbo_bp* bp, int jacobi, int flexible, int prec, 
                  NULL, NULL, &s);
     total += s.iterations;
     breakdown = s.breakdown;
+    #pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < Nall; ++n) x[n] = x[n] + d[n];
     ++cycles;
     if (s.iterations == 0) break;
