@@ -76,7 +76,8 @@ int nbo_gs_many(const nbo_ogs* o, void* q, int k, int32_t stride, int type, int 
 
 /* ---- axhelm (axhelm/kernel/CPU/axhelm.c:27-413; axhelm/axhelmReference.cpp) */
 /* Nf in {1,3}; bk != 0 ignores lambda.  elementList == NULL: all elements
- * (axhelm_*), else the Partial variants over elementList[0..Nelements). */
+ * (axhelm_*), else the Partial variants over elementList[0..Nelements).
+ * 1 <= N <= 15; q and Aq must not overlap (the reference's @restrict). */
 void nbo_axhelm(int N, int Nf, int bk, int32_t Nelements, int32_t offset, const int32_t* elementList,
                 const double* ggeo, const double* D, const double* lambda,
                 const double* q, double* Aq);
