@@ -164,10 +164,13 @@ struct AxGroups {
 // The FP64 field-group kernels read D from LDS as well (4 Nq fewer doubles
 // in registers) and keep three waves per SIMD up to Nq = 10 (168 VGPRs; left
 // alone they take 256 and run two); the one-field BK kernel at N = 7 keeps
-// four (128 VGPRs).
+// four (128 VGPRs).  The FP32-arithmetic kernels keep D in registers: their
+// vectors take half the registers, and reading D from LDS made LDS the bound
+// (one-field BK at N = 7 on 32^3: bf16 geometry 80 -> 65 us, FP32 89 -> 84 us
+// with D in registers, the same bits).
 template <typename T, int NQ, int NF, bool BK>
 struct AxUseDL {
-  static constexpr bool value = AxDL<NQ, NF, BK>::value || (AxGroups<NQ, NF, BK>::fields && sizeof(T) == 8);
+  static constexpr bool value = sizeof(T) == 8 && (AxDL<NQ, NF, BK>::value || AxGroups<NQ, NF, BK>::fields);
 };
 template <typename T, int NQ, int NF, bool BK>
 struct AxMinWaves {
