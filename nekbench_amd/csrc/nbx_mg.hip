@@ -129,8 +129,11 @@ __global__ __launch_bounds__(256) void k_cheb(const int32_t n, const int32_t off
 // Restriction of one element: res -= Ad (stored), u = w res, then
 // rc[a,b,c] = sum_k J[k][c] sum_j J[j][b] sum_i J[i][a] u[i,j,k]
 // (node index i + j Nq + k Nq^2; fma chains in ascending fine index).
-template <typename V, typename O>
-__global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nqc, const int32_t offf,
+// NQF, NQC: the orders at compile time (the loops unroll, the index
+// arithmetic folds); 0 takes them from the arguments.  Either way each
+// output is the same fma chain.  Launched with 256 lanes.
+template <typename V, typename O, int NQF = 0, int NQC = 0>
+__global__ __launch_bounds__(256) void k_mg_restrict(const int nqf, const int nqc, const int32_t offf,
                                                      const int32_t offc, const V* __restrict__ J,
                                                      const double* __restrict__ w, V* __restrict__ res,
                                                      const V* __restrict__ Ad, O* __restrict__ rc,
@@ -140,11 +143,13 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
   __shared__ V s_u[12 * 12 * 12];
   __shared__ V s_t[12 * 12 * 12];
   __shared__ V s_J[12 * 12];
+  constexpr int kLanes = 256;
+  const int Nqf = NQF ? NQF : nqf, Nqc = NQC ? NQC : nqc;
   const int64_t e = blockIdx.x;
   const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
-  for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
+  for (int q = threadIdx.x; q < Nqf * Nqc; q += kLanes) s_J[q] = J[q];
   const int64_t fo = (int64_t)blockIdx.y * offf, fc = (int64_t)blockIdx.y * offc;  // blockIdx.y = field
-  for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Npf; q += kLanes) {
     const int64_t id = e * Npf + q;
     const V rq = res[fo + id] - Ad[fo + id];
     res[fo + id] = rq;
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
   }
   __syncthreads();
   // x: t1[a + Nqc (j + Nqf k)]
-  for (int q = threadIdx.x; q < Nqc * Nqf * Nqf; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Nqc * Nqf * Nqf; q += kLanes) {
     const int a = q % Nqc, jk = q / Nqc;
     V acc = V(0);
     for (int i = 0; i < Nqf; ++i) acc = fmaV<V>(s_J[i * Nqc + a], s_u[i + Nqf * jk], acc);
@@ -160,7 +165,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
   }
   __syncthreads();
   // y: t2[a + Nqc (b + Nqc k)] into s_u
-  for (int q = threadIdx.x; q < Nqc * Nqc * Nqf; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Nqc * Nqc * Nqf; q += kLanes) {
     const int a = q % Nqc, b = (q / Nqc) % Nqc, k = q / (Nqc * Nqc);
     V acc = V(0);
     for (int j = 0; j < Nqf; ++j) acc = fmaV<V>(s_J[j * Nqc + b], s_t[a + Nqc * (j + Nqf * k)], acc);
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
   }
   __syncthreads();
   // z: rc[a + Nqc (b + Nqc c)]
-  for (int q = threadIdx.x; q < Npc; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Npc; q += kLanes) {
     const int ab = q % (Nqc * Nqc), c = q / (Nqc * Nqc);
     V acc = V(0);
     for (int k = 0; k < Nqf; ++k) acc = fmaV<V>(s_J[k * Nqc + c], s_u[ab + Nqc * Nqc * k], acc);
@@ -178,8 +183,8 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const int Nqf, const int Nq
 
 // Prolongation of one element: v[i,j,k] = sum_c J[k][c] sum_b J[j][b] sum_a J[i][a] xc[a,b,c];
 // d = v, x += v.
-template <typename V, typename I, typename X>
-__global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc, const int32_t offf,
+template <typename V, typename I, typename X, int NQF = 0, int NQC = 0>
+__global__ __launch_bounds__(256) void k_mg_prolong(const int nqf, const int nqc, const int32_t offf,
                                                     const int32_t offc, const V* __restrict__ J,
                                                     const I* __restrict__ xc, V* __restrict__ d,
                                                     X* __restrict__ x, const int* __restrict__ skip)
@@ -188,14 +193,16 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc
   __shared__ V s_u[12 * 12 * 12];
   __shared__ V s_t[12 * 12 * 12];
   __shared__ V s_J[12 * 12];
+  constexpr int kLanes = 256;
+  const int Nqf = NQF ? NQF : nqf, Nqc = NQC ? NQC : nqc;
   const int64_t e = blockIdx.x;
   const int Npf = Nqf * Nqf * Nqf, Npc = Nqc * Nqc * Nqc;
-  for (int q = threadIdx.x; q < Nqf * Nqc; q += blockDim.x) s_J[q] = J[q];
+  for (int q = threadIdx.x; q < Nqf * Nqc; q += kLanes) s_J[q] = J[q];
   const int64_t fo = (int64_t)blockIdx.y * offf, fc = (int64_t)blockIdx.y * offc;  // blockIdx.y = field
-  for (int q = threadIdx.x; q < Npc; q += blockDim.x) s_u[q] = (V)xc[fc + e * Npc + q];
+  for (int q = threadIdx.x; q < Npc; q += kLanes) s_u[q] = (V)xc[fc + e * Npc + q];
   __syncthreads();
   // x: t1[i + Nqf (b + Nqc c)]
-  for (int q = threadIdx.x; q < Nqf * Nqc * Nqc; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Nqf * Nqc * Nqc; q += kLanes) {
     const int i = q % Nqf, bc = q / Nqf;
     V acc = V(0);
     for (int a = 0; a < Nqc; ++a) acc = fmaV<V>(s_J[i * Nqc + a], s_u[a + Nqc * bc], acc);
@@ -203,14 +210,14 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const int Nqf, const int Nqc
   }
   __syncthreads();
   // y: t2[i + Nqf (j + Nqf c)] into s_u
-  for (int q = threadIdx.x; q < Nqf * Nqf * Nqc; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Nqf * Nqf * Nqc; q += kLanes) {
     const int i = q % Nqf, j = (q / Nqf) % Nqf, c = q / (Nqf * Nqf);
     V acc = V(0);
     for (int b = 0; b < Nqc; ++b) acc = fmaV<V>(s_J[j * Nqc + b], s_t[i + Nqf * (b + Nqc * c)], acc);
     s_u[q] = acc;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < Npf; q += blockDim.x) {
+  for (int q = threadIdx.x; q < Npf; q += kLanes) {
     const int ij = q % (Nqf * Nqf), k = q / (Nqf * Nqf);
     V acc = V(0);
     for (int c = 0; c < Nqc; ++c) acc = fmaV<V>(s_J[k * Nqc + c], s_u[ij + Nqf * Nqf * c], acc);
@@ -740,16 +747,33 @@ int vcycle_level(nbx_mg* mg, int l, const R* r, X* x, const int* skip)
   if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
   // restriction of the smoothed residual
   CV* rcv = static_cast<CV*>(C.r);
-  hipLaunchKernelGGL((k_mg_restrict<V, CV>), ge, b, 0, bp->stream, L.Nq, C.Nq, off, C.bp->fieldOffset, level_J<V>(L),
-                     nbx_ogs_inv_degree(bp->ogs), res, Ad, rcv, skip);
+  {
+    const double* wv = nbx_ogs_inv_degree(bp->ogs);
+#define NBX_RESTRICT(F, C_)                                                                                  \
+  hipLaunchKernelGGL((k_mg_restrict<V, CV, F, C_>), ge, b, 0, bp->stream, L.Nq, C.Nq, off, C.bp->fieldOffset, \
+                     level_J<V>(L), wv, res, Ad, rcv, skip)
+    if (L.Nq == 8 && C.Nq == 4) NBX_RESTRICT(8, 4);
+    else if (L.Nq == 4 && C.Nq == 2) NBX_RESTRICT(4, 2);
+    else if (L.Nq == 10 && C.Nq == 5) NBX_RESTRICT(10, 5);
+    else if (L.Nq == 5 && C.Nq == 2) NBX_RESTRICT(5, 2);
+    else NBX_RESTRICT(0, 0);
+#undef NBX_RESTRICT
+  }
   NBX_HIP_LAUNCH();
   nbx_bp* bc = C.bp;
   if ((rc = gs_add_mask_launch(bc->ogs, rcv, bc->Nf, bc->fieldOffset, bc->Nmasked, bc->maskIds, skip, bc->stream)))
     return rc;
   if ((rc = vcycle<V>(mg, l + 1, C.r, C.x, skip))) return rc;
   // coarse correction
-  hipLaunchKernelGGL((k_mg_prolong<V, CV, X>), ge, b, 0, bp->stream, L.Nq, C.Nq, off, C.bp->fieldOffset, level_J<V>(L),
-                     static_cast<const CV*>(C.x), d, x, skip);
+#define NBX_PROLONG(F, C_)                                                                                 \
+  hipLaunchKernelGGL((k_mg_prolong<V, CV, X, F, C_>), ge, b, 0, bp->stream, L.Nq, C.Nq, off, C.bp->fieldOffset, \
+                     level_J<V>(L), static_cast<const CV*>(C.x), d, x, skip)
+  if (L.Nq == 8 && C.Nq == 4) NBX_PROLONG(8, 4);
+  else if (L.Nq == 4 && C.Nq == 2) NBX_PROLONG(4, 2);
+  else if (L.Nq == 10 && C.Nq == 5) NBX_PROLONG(10, 5);
+  else if (L.Nq == 5 && C.Nq == 2) NBX_PROLONG(5, 2);
+  else NBX_PROLONG(0, 0);
+#undef NBX_PROLONG
   NBX_HIP_LAUNCH();
   if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
   // post-smoothing
