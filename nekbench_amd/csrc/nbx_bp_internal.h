@@ -109,6 +109,8 @@ namespace nbx {
 // element-local learned preconditioner output w invDiag (r + s f(r)) of the
 // attached network into out (Jacobi when *fallback)
 int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallback, const int* skip);
+// matrix-core inference only: the input residual is (double)(res - Ad)
+int net_apply_learned_resid(nbx_bp* bp, const float* res, const float* Ad, double* out, const int* skip);
 // forget a handle that is being freed (the network keeps a list of the
 // handles it is attached to and clears their pointer when it is freed first)
 void net_detach(nbx_net* net, nbx_bp* bp);

@@ -238,6 +238,16 @@ int net_apply_learned(nbx_bp* bp, const double* r, double* out, const int* fallb
                           skip, bp->stream);
 }
 
+// the learned multigrid step on the matrix cores with its input residual
+// (double)(res - Ad) formed inside the network kernel
+int net_apply_learned_resid(nbx_bp* bp, const float* res, const float* Ad, double* out, const int* skip)
+{
+  if (!bp->net || !bp->feat) return fail(NBX_ERR_INVALID, "learned preconditioner: no network attached");
+  if (!bp->net->infer_mfma) return fail(NBX_ERR_UNSUPPORTED, "learned step: residual input on the matrix cores only");
+  return net_apply_mma_resid_launch(bp->E, bp->net->frag, bp->net->wbf, bp->feat16, res, Ad, bp->invDiagA,
+                                    nbx_ogs_inv_degree(bp->ogs), ogs_degree(bp->ogs), out, skip, bp->stream);
+}
+
 }  // namespace nbx
 
 using namespace nbx;

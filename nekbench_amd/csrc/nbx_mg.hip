@@ -790,9 +790,18 @@ int vcycle_level(nbx_mg* mg, int l, const R* r, X* x, const int* skip)
     // learned step: rf = res - A d; x += mask(gs(w invDiag (rf + s f(rf))))
     if ((rc = level_apply(mg, L, d, Ad, skip))) return rc;
     const dim3 gn((unsigned)((n + 255) / 256));
-    hipLaunchKernelGGL((k_mg_resid<V>), gn, b, 0, bp->stream, (int64_t)n, res, Ad, mg->lrf, skip);
-    NBX_HIP_LAUNCH();
-    if ((rc = net_apply_learned(bp, mg->lrf, mg->lzl, nullptr, skip))) return rc;
+    bool fused = false;  // matrix-core inference reads res and A d and forms rf itself
+    if constexpr (std::is_same<V, float>::value) {
+      if (net_inference_mode(bp->net) == 1) {
+        if ((rc = net_apply_learned_resid(bp, res, Ad, mg->lzl, skip))) return rc;
+        fused = true;
+      }
+    }
+    if (!fused) {
+      hipLaunchKernelGGL((k_mg_resid<V>), gn, b, 0, bp->stream, (int64_t)n, res, Ad, mg->lrf, skip);
+      NBX_HIP_LAUNCH();
+      if ((rc = net_apply_learned(bp, mg->lrf, mg->lzl, nullptr, skip))) return rc;
+    }
     if ((rc = gs_add_mask_launch(bp->ogs, mg->lzl, 1, off, bp->Nmasked, bp->maskIds, skip, bp->stream))) return rc;
     hipLaunchKernelGGL(k_mg_add, gn, b, 0, bp->stream, (int64_t)n, mg->lzl, x, skip);
     NBX_HIP_LAUNCH();

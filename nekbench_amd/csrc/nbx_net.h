@@ -24,6 +24,11 @@ int net_features16_launch(int64_t n, const float* feat, uint16_t* feat16, hipStr
 int net_apply_mma_launch(int32_t E, const void* frag, const float* W, const uint16_t* feat16, const double* r,
                          const double* invDiag, const double* w, const uint8_t* deg, double* zl,
                          const int* fallback, const int* skip, hipStream_t s);
+// the same with the input residual (double)(res - Ad) of two FP32 vectors
+// (the learned multigrid step), no fallback flag
+int net_apply_mma_resid_launch(int32_t E, const void* frag, const float* W, const uint16_t* feat16,
+                               const float* res, const float* Ad, const double* invDiag, const double* w,
+                               const uint8_t* deg, double* zl, const int* skip, hipStream_t s);
 size_t net_train_lds_bytes();
 int net_train_launch(int32_t nsamples, int32_t chunk, const int2* samples, int64_t pairStride, const float* R,
                      const float* Ev, const double* invNorm, const float* W, const float* feat, const double* invDiag,

@@ -160,16 +160,22 @@ struct InvDegreeTable {
 };
 __constant__ InvDegreeTable kInvDegree = InvDegreeTable();
 
+// RES: the input residual is (double)(res - Ad) of two FP32 vectors (the
+// learned multigrid step's fine residual, formed here instead of in a pass
+// of its own; the same value), otherwise the FP64 vector r
+template <bool RES>
 __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const LaneFrag* __restrict__ frag,
                                                        const float* __restrict__ W,
                                                        const uint16_t* __restrict__ feat16,
                                                        const double* __restrict__ r,
+                                                       const float* __restrict__ res, const float* __restrict__ Ad,
                                                        const double* __restrict__ invDiag,
                                                        const double* __restrict__ w, const uint8_t* __restrict__ deg,
                                                        double* __restrict__ zl, const int* __restrict__ fallback,
                                                        const int* __restrict__ skip)
 {
   if (skip && *skip) return;
+  auto rload = [&](int64_t n) { return RES ? (double)(res[n] - Ad[n]) : r[n]; };
   // one LDS block, the operand tile first, so that every read and store below
   // is a lane address plus an offset that fits the 16-bit LDS offset field
   struct Lds {
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
     for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
       const int64_t id = e * kP + t;
       const double wn = deg ? kInvDegree.v[deg[id]] : (w ? w[id] : 1.0);
-      zl[id] = wn * (invDiag[id] * r[id]);
+      zl[id] = wn * (invDiag[id] * rload(id));
     }
     return;
   }
@@ -228,7 +234,7 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
 #pragma unroll
     for (int q = 0; q < 7; ++q) s_in[kPadN + q * kFRow + post] = fv[q];
   };
-  double rn = r[e * kP + t], di = invDiag[e * kP + t], wraw = w && !deg ? w[e * kP + t] : 1.0;
+  double rn = rload(e * kP + t), di = invDiag[e * kP + t], wraw = w && !deg ? w[e * kP + t] : 1.0;
   uint32_t dg = deg ? deg[e * kP + t] : 0u;
   uint16_t fe[7];
 #pragma unroll
@@ -247,10 +253,9 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
     const bool more = en < E;
     // (past the last element the loads repeat this one: no branch, no use)
     const int64_t ef = more ? en : e;
-    const double* const rp = r + ef * kP;  // uniform bases, lane offsets fixed
-    const double* const dp = invDiag + ef * kP;
+    const double* const dp = invDiag + ef * kP;  // uniform bases, lane offsets fixed
     const uint16_t* const fp = feat16 + ef * 7 * kP;
-    const double rn_n = rp[t], di_n = dp[t];
+    const double rn_n = rload(ef * kP + t), di_n = dp[t];
     const uint32_t dg_n = deg ? deg[ef * kP + t] : 0u;
     const double wraw_n = w && !deg ? w[ef * kP + t] : 1.0;
     uint16_t fe_n[7];
@@ -343,28 +348,45 @@ int net_features16_launch(int64_t n, const float* feat, uint16_t* feat16, hipStr
   return NBX_SUCCESS;
 }
 
-int net_apply_mma_launch(int32_t E, const void* frag, const float* W, const uint16_t* feat16, const double* r,
-                         const double* invDiag, const double* w, const uint8_t* deg, double* zl,
-                         const int* fallback, const int* skip, hipStream_t s)
+template <bool RES>
+static int net_apply_mma_launch_t(int32_t E, const void* frag, const float* W, const uint16_t* feat16,
+                                  const double* r, const float* res, const float* Ad, const double* invDiag,
+                                  const double* w, const uint8_t* deg, double* zl, const int* fallback,
+                                  const int* skip, hipStream_t s)
 {
   if (E <= 0) return NBX_SUCCESS;
   // as many workgroups as the device holds at once (each loops over its
-  // elements), found once per device
+  // elements), found once per device and kernel form
   static int resident[64] = {};
   int dev = 0;
   NBX_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return fail(NBX_ERR_UNSUPPORTED, "net_apply: device %d", dev);
   if (!resident[dev]) {
     int per_cu = 0, cus = 0;
-    NBX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_net_apply_mma, kP, 0));
+    NBX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_net_apply_mma<RES>, kP, 0));
     NBX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     resident[dev] = std::max(1, per_cu) * std::max(1, cus);
   }
   const int grid = std::min<int64_t>(E, resident[dev]);
-  hipLaunchKernelGGL(k_net_apply_mma, dim3(grid), dim3(kP), 0, s, E, static_cast<const LaneFrag*>(frag), W, feat16,
-                     r, invDiag, w, deg, zl, fallback, skip);
+  hipLaunchKernelGGL(k_net_apply_mma<RES>, dim3(grid), dim3(kP), 0, s, E, static_cast<const LaneFrag*>(frag), W,
+                     feat16, r, res, Ad, invDiag, w, deg, zl, fallback, skip);
   NBX_HIP_LAUNCH();
   return NBX_SUCCESS;
+}
+
+int net_apply_mma_launch(int32_t E, const void* frag, const float* W, const uint16_t* feat16, const double* r,
+                         const double* invDiag, const double* w, const uint8_t* deg, double* zl,
+                         const int* fallback, const int* skip, hipStream_t s)
+{
+  return net_apply_mma_launch_t<false>(E, frag, W, feat16, r, nullptr, nullptr, invDiag, w, deg, zl, fallback, skip,
+                                       s);
+}
+
+int net_apply_mma_resid_launch(int32_t E, const void* frag, const float* W, const uint16_t* feat16,
+                               const float* res, const float* Ad, const double* invDiag, const double* w,
+                               const uint8_t* deg, double* zl, const int* skip, hipStream_t s)
+{
+  return net_apply_mma_launch_t<true>(E, frag, W, feat16, nullptr, res, Ad, invDiag, w, deg, zl, nullptr, skip, s);
 }
 
 }  // namespace nbx
