@@ -643,6 +643,10 @@ static int coarse_cg(nbx_mg* mg, nbx_mg_level& L, const int* skip)
   const unsigned nbA = (unsigned)((bp->E + kCgEpb - 1) / kCgEpb),
                  nbB = (unsigned)((mg->cg_items + kCgNpb - 1) / kCgNpb);
   hipLaunchKernelGGL(k_mg_fill0, mg_grid(bp->Nall), dim3(256), 0, bp->stream, bp->Nall, x, skip);
+  if (mg->cg_items == 0 || nbA == 0) {  // every coarse node masked: the solution is x = 0
+    NBX_HIP_LAUNCH();
+    return NBX_SUCCESS;
+  }
   const dim3 gA(8 * ((nbA + 7) / 8)), gB(8 * ((nbB + 7) / 8));  // eight XCD chunks of blocks
 #define NBX_CGB(NFV, UPD, KK)                                                                                   \
   hipLaunchKernelGGL((k_cg_pass_b<NFV, UPD>), gB, dim3(256), 0, bp->stream, mg->cg_items, mg->cg_members,          \

@@ -16,6 +16,9 @@ CASES = [  # (N, dims, orders, degree, coarse_iters, bpmode, lambda1)
     (7, (3, 3, 2), [7, 3, 1], 2, 10, True, 0.0),
     (5, (2, 3, 2), [5, 2, 1], 3, 6, False, 0.7),
     (3, (3, 2, 2), [3, 1], 1, 12, True, 0.0),
+    # the BP6 orders (transfers compiled for 10->5, 5->2); one element layer: every order-1 vertex lies
+    # on the Dirichlet boundary, so the coarse solve has no unknowns
+    (9, (2, 2, 1), [9, 4, 1], 2, 8, False, 0.3),
 ]
 
 
@@ -33,7 +36,7 @@ PRECS = {"f64": 0, "f32": 1, "bf16": 2}
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm"])
 def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
@@ -50,7 +53,7 @@ def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm"])
 def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
