@@ -26,6 +26,7 @@ CASES = [
     (7, (2, 2, 2), 3, False, 0.3, 1, True, 0, 0),      # vector Helmholtz N=7: one thread group per field
     (9, (2, 2, 1), 3, False, 0.2, 0, False, 0, 0),     # vector Helmholtz N=9: field groups, D from LDS
     (7, (2, 2, 2), 3, True, 0.0, 0, False, 0, 0),      # BP6 N=7: field loop
+    (7, (1, 1, 1), 1, False, 0.0, 1, False, 0, 0),     # one element: no shared node, the mask alone
 ]
 
 

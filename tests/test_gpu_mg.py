@@ -19,6 +19,7 @@ CASES = [  # (N, dims, orders, degree, coarse_iters, bpmode, lambda1)
     # the BP6 orders (transfers compiled for 10->5, 5->2); one element layer: every order-1 vertex lies
     # on the Dirichlet boundary, so the coarse solve has no unknowns
     (9, (2, 2, 1), [9, 4, 1], 2, 8, False, 0.3),
+    (7, (1, 1, 1), [7, 3, 1], 2, 10, True, 0.0),  # one element: no shared node on any level
 ]
 
 
@@ -36,7 +37,7 @@ PRECS = {"f64": 0, "f32": 1, "bf16": 2}
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one"])
 def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
@@ -53,7 +54,7 @@ def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one"])
 def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
