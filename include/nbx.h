@@ -164,7 +164,7 @@ int nbx_box_mesh_device(const nbx_box* box, int mask_boundary, double* ggeo, nbx
  * (result_dev) or host (result_host) pointer; results are deterministic: a
  * fixed tree independent of timing (tiles of 256 lanes x 8 values, lane
  * fma chain, per-wave halving tree, (S0+S2)+(S1+S3), partials combined the
- * same way by a one-workgroup second pass). */
+ * same way by a one-workgroup second pass).  N = 0 gives 0 (the empty sum). */
 int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
                                const double* x, const double* y, double* result_host, void* stream);
 int nbx_scaled_add(nbx_dlong N, double alpha, const double* x, double beta, double* y, void* stream);

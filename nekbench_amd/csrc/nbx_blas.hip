@@ -150,11 +150,17 @@ using namespace nbx;
 static std::mutex g_wdot_mutex;
 static ReduceWork g_wdot_ws;
 
+static __global__ void k_store_zero(double* __restrict__ out) { *out = 0.0; }
+
 extern "C" int nbx_weighted_inner_product(nbx_dlong N, nbx_dlong offset, int Nfields, const double* w,
                                           const double* x, const double* y, double* result_host, void* stream)
 {
-  if (N <= 0 || Nfields < 1 || !w || !x || !y || !result_host)
-    return fail(NBX_ERR_INVALID, "weighted_inner_product: bad arguments");
+  if (N < 0 || Nfields < 1 || !result_host) return fail(NBX_ERR_INVALID, "weighted_inner_product: bad arguments");
+  if (N == 0) {  // the empty sum (the reference's loops do nothing)
+    *result_host = 0.0;
+    return NBX_SUCCESS;
+  }
+  if (!w || !x || !y) return fail(NBX_ERR_INVALID, "weighted_inner_product: null array");
   if (Nfields > 1 && offset < N) return fail(NBX_ERR_INVALID, "weighted_inner_product: offset < N");
   std::lock_guard<std::mutex> lock(g_wdot_mutex);
   const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
@@ -218,8 +224,14 @@ extern "C" int nbx_weighted_inner_product_noalloc(nbx_dlong N, nbx_dlong offset,
                                                   const double* x, const double* y, double* result_dev,
                                                   void* stream)
 {
-  if (N <= 0 || Nfields < 1 || !w || !x || !y || !result_dev)
+  if (N < 0 || Nfields < 1 || !result_dev)
     return fail(NBX_ERR_INVALID, "weighted_inner_product_noalloc: bad arguments");
+  if (N == 0) {  // the empty sum, written on the stream like any result
+    hipLaunchKernelGGL(k_store_zero, dim3(1), dim3(1), 0, as_stream(stream), result_dev);
+    NBX_HIP_LAUNCH();
+    return NBX_SUCCESS;
+  }
+  if (!w || !x || !y) return fail(NBX_ERR_INVALID, "weighted_inner_product_noalloc: null array");
   if (Nfields > 1 && offset < N) return fail(NBX_ERR_INVALID, "weighted_inner_product_noalloc: offset < N");
   const unsigned int nb = (unsigned int)((N + kRedTile - 1) / kRedTile);
   if (nb > kNoAllocMaxTiles)

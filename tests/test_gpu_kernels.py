@@ -427,3 +427,36 @@ def test_native_symbols_match_nbx_axhelm(nbx, dev, N):
         nbx.axhelm(N, nf, bk, ne, off, g, Dd, lam, q, exp, elementList=el if partial else None)
         torch.cuda.synchronize()
         assert torch.equal(got, exp), name
+
+
+def test_empty_inputs_are_no_ops(nbx, dev):
+    """Zero elements, nodes or ids: the entry points return success, launch
+    nothing and leave their outputs alone; the empty weighted sum is 0 (the
+    reference's loops do nothing), from both dot entry points."""
+    import torch
+    one = torch.ones(64, dtype=torch.float64, device=dev)
+    out = one.clone()
+    nbx.axhelm(7, 1, 1, 0, 0, one, one, None, one, out)
+    nbx.axhelm(7, 3, 0, 0, 0, one, one, one, one, out)
+    f32 = torch.ones(64, dtype=torch.float32, device=dev)
+    o32 = f32.clone()
+    for prec in (nbx.NBX_F32, nbx.NBX_BF16_MFMA, nbx.NBX_F16):
+        nbx.axhelm(7, 1, 1, 0, 0, f32, f32, None, f32, o32, prec=prec)
+    torch.cuda.synchronize()
+    assert torch.equal(out, one) and torch.equal(o32, f32)
+    assert nbx.weighted_inner_product(0, 0, 1, one, one, one) == 0.0
+    res = torch.full((1,), 7.0, dtype=torch.float64, device=dev)
+    nbx.weighted_inner_product_noalloc(0, 0, 1, one, one, one, res)
+    torch.cuda.synchronize()
+    assert res.item() == 0.0
+    nbx.scaled_add(0, 2.0, one, 3.0, out)
+    for ids in (np.zeros(0, dtype=np.int64), np.zeros(8, dtype=np.int64)):  # no nodes; every node excluded
+        g = nbx.Ogs(ids)
+        s = g.sizes()
+        assert s["N"] == ids.size and s["NlocalGather"] == 0 and s["Nshared"] == 0
+        v = one.clone()
+        g.gather_scatter_many(v, 1, max(ids.size, 1))
+        g.free()
+        torch.cuda.synchronize()
+        assert torch.equal(v, one)
+    assert torch.equal(out, one)
