@@ -20,12 +20,13 @@ CASES = [  # (N, dims, orders, degree, coarse_iters, bpmode, lambda1)
     # on the Dirichlet boundary, so the coarse solve has no unknowns
     (9, (2, 2, 1), [9, 4, 1], 2, 8, False, 0.3),
     (7, (1, 1, 1), [7, 3, 1], 2, 10, True, 0.0),  # one element: no shared node on any level
+    (5, (2, 3, 2), [5, 2, 1], 2, 8, False, 0.5, 1),  # periodic Helmholtz: no mask, every coarse vertex shared
 ]
 
 
 def make(nbx, nbo, case, prec=0):
-    N, dims, orders, deg, ci, bpmode, l1 = case
-    kw = dict(perturb=0.1, seed=50 + N)
+    N, dims, orders, deg, ci, bpmode, l1 = case[:7]
+    kw = dict(perturb=0.1, seed=50 + N, periodic=case[7] if len(case) > 7 else 0)
     bp = nbx.BP(nbx.Box.make(N, *dims, **kw), bpmode=bpmode, lambda1=l1, precon=nbx.PRECON_MG)
     bp.mg_setup(orders, deg, ci, prec=prec)
     mg = nbo.OracleMG(nbo.Box.make(N, *dims, **kw), orders, degree=deg, coarse_iters=ci, bpmode=bpmode,
@@ -37,7 +38,7 @@ PRECS = {"f64": 0, "f32": 1, "bf16": 2}
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one", "n5per"])
 def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
@@ -54,7 +55,7 @@ def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one", "n5per"])
 def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
