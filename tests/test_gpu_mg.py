@@ -21,6 +21,7 @@ CASES = [  # (N, dims, orders, degree, coarse_iters, bpmode, lambda1)
     (9, (2, 2, 1), [9, 4, 1], 2, 8, False, 0.3),
     (7, (1, 1, 1), [7, 3, 1], 2, 10, True, 0.0),  # one element: no shared node on any level
     (5, (2, 3, 2), [5, 2, 1], 2, 8, False, 0.5, 1),  # periodic Helmholtz: no mask, every coarse vertex shared
+    (5, (2, 2, 2), [5, 2], 2, 12, True, 0.0),  # last level of order 2: the generic five-kernel coarse PCG
 ]
 
 
@@ -38,7 +39,7 @@ PRECS = {"f64": 0, "f32": 1, "bf16": 2}
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one", "n5per"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one", "n5per", "n5to2"])
 def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
@@ -55,7 +56,7 @@ def test_mg_setup_and_vcycle_match_oracle(nbx, nbo, dev, case, prec):
 
 
 @pytest.mark.parametrize("prec", list(PRECS))
-@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one", "n5per"])
+@pytest.mark.parametrize("case", CASES, ids=["n7", "n5helm", "n3", "n9helm", "n7one", "n5per", "n5to2"])
 def test_mg_pcg_iterates_match_oracle(nbx, nbo, dev, case, prec):
     import torch
     bp, mg = make(nbx, nbo, case, PRECS[prec])
