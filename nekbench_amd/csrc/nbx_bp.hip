@@ -80,6 +80,9 @@ __device__ __forceinline__ void pcg_after_dots(PcgScalars* S, double rdotz, doub
     S->beta = (it == 1) ? 0.0 : S->rdotz1 / S->rdotz2;  // solve.cpp:97
 }
 
+// nodes per lane whose loads the update pass issues together
+constexpr int kUpdateBatch = 8;
+
 // 1. [x += alpha p, r -= alpha Ap of the previous iteration]; z = M r
 //    (Jacobi: invDiag r; copy: z is r itself); tile partials of the dots
 template <bool JAC, bool FLEX, bool DEG>
@@ -98,6 +101,47 @@ __global__ __launch_bounds__(256) void k_pcg_update_precon_dots(
 #pragma unroll
   for (int c = 0; c < NV; ++c) v[c] = 0.0;
   const int64_t b0 = (int64_t)blockIdx.x * kRedTile + threadIdx.x;
+  if (update && Nf == 1 && (int64_t)(blockIdx.x + 1) * kRedTile <= N) {
+    // a full tile of one field after the first iteration (every block but the
+    // last): each lane issues the loads of all its nodes before the first
+    // store, instead of one node's loads per round trip to memory; the
+    // arithmetic and the order of the sums are those of the general loop
+    constexpr int kB = kUpdateBatch;
+    for (int h = 0; h < kRedPerLane; h += kB) {
+      double rr[kB], ap[kB], pp[kB], xx[kB], dg[kB], wi[kB];
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const int64_t id = b0 + (h + u) * kRedLanes;
+        rr[u] = ldnt(&r[id]);
+        ap[u] = ldnt(&Ap[id]);
+        pp[u] = ldnt(&p[id]);
+        xx[u] = ldnt(&x[id]);
+        dg[u] = JAC ? ldnt(&invDiag[id]) : 1.0;
+        wi[u] = DEG ? (double)deg[id] : ldnt(&w[id]);  // the weight's source: degree or weight
+      }
+      // the divisions below wait for their degree loads: keep them after every load is issued
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < kB; ++u) {
+        const int64_t id = b0 + (h + u) * kRedLanes;
+        if (DEG) wi[u] = 1.0 / wi[u];  // weight<true>: the same IEEE division
+        stnt(__builtin_fma(alpha, pp[u], xx[u]), &x[id]);
+        const double rf = __builtin_fma(-alpha, ap[u], rr[u]);
+        stnt(rf, &r[id]);
+        double zf = rf;
+        if (JAC) {
+          zf = dg[u] * rf;
+          stnt(zf, &z[id]);
+        }
+        v[0] = __builtin_fma(wi[u] * rf, zf, v[0]);
+        v[1] = __builtin_fma(wi[u] * rf, rf, v[1]);
+        if (FLEX) v[NV - 1] = __builtin_fma(wi[u] * zf, ap[u], v[NV - 1]);
+      }
+    }
+    block_sum256<NV>(v, s4);
+    store_partials<NV>(v, partials);
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kRedPerLane; ++u) {
     const int64_t id = b0 + u * kRedLanes;
