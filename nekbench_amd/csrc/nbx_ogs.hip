@@ -152,6 +152,42 @@ struct GsClasses {
   const int32_t* off;    // CSR offsets
 };
 
+// one group of DEG members (ids[item*DEG ..]): the sum in ascending member
+// order from zero, written back to every member.  DEG is a compile-time
+// constant so that a lane issues the loads of all members before the first
+// use (a run-time count compiled to one load per round trip to memory).
+template <typename T, int DEG>
+__device__ __forceinline__ void gs_group_sum(const int32_t* __restrict__ ids, const int32_t item, const int k,
+                                             const int32_t stride, T* __restrict__ q)
+{
+  int32_t m[DEG];
+  if (DEG == 2) {
+    const int2 v = *reinterpret_cast<const int2*>(ids + (int64_t)item * 2);
+    m[0] = v.x;
+    m[1] = v.y;
+  } else {
+#pragma unroll
+    for (int h = 0; h < DEG / 4; ++h) {
+      const int4 v = *reinterpret_cast<const int4*>(ids + (int64_t)item * DEG + 4 * h);
+      m[4 * h + 0] = v.x;
+      m[4 * h + 1] = v.y;
+      m[4 * h + 2] = v.z;
+      m[4 * h + 3] = v.w;
+    }
+  }
+  for (int f = 0; f < k; ++f) {
+    T* qf = q + (int64_t)f * stride;
+    T v[DEG];
+#pragma unroll
+    for (int n = 0; n < DEG; ++n) v[n] = qf[m[n]];
+    T acc = T(0);
+#pragma unroll
+    for (int n = 0; n < DEG; ++n) acc = acc + v[n];
+#pragma unroll
+    for (int n = 0; n < DEG; ++n) qf[m[n]] = acc;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const int k, const int32_t stride,
                                                      T* __restrict__ q, const int* __restrict__ skip)
@@ -181,27 +217,9 @@ __global__ __launch_bounds__(256) void k_gs_add_mask(const GsClasses C, const in
     }
     return;
   }
-  int32_t m[8];
-  const int deg = 2 << c;
-  if (c == 0) {
-    const int2 v = *reinterpret_cast<const int2*>(C.ids[0] + (int64_t)item * 2);
-    m[0] = v.x;
-    m[1] = v.y;
-  } else {
-    for (int h = 0; h < deg / 4; ++h) {
-      const int4 v = *reinterpret_cast<const int4*>(C.ids[c] + (int64_t)item * deg + 4 * h);
-      m[4 * h + 0] = v.x;
-      m[4 * h + 1] = v.y;
-      m[4 * h + 2] = v.z;
-      m[4 * h + 3] = v.w;
-    }
-  }
-  for (int f = 0; f < k; ++f) {
-    T* qf = q + (int64_t)f * stride;
-    T acc = T(0);
-    for (int n = 0; n < deg; ++n) acc = acc + qf[m[n]];
-    for (int n = 0; n < deg; ++n) qf[m[n]] = acc;
-  }
+  if (c == 0) gs_group_sum<T, 2>(C.ids[0], item, k, stride, q);
+  else if (c == 1) gs_group_sum<T, 4>(C.ids[1], item, k, stride, q);
+  else gs_group_sum<T, 8>(C.ids[2], item, k, stride, q);
 }
 
 __global__ __launch_bounds__(256) void k_mask(const int32_t Nmasked, const int32_t* __restrict__ maskIds,
