@@ -81,7 +81,9 @@ struct AxLane {
 // registers; used where that buys a fourth wave per SIMD)
 // NTG: nontemporal loads of the geometric factors and lambda (false when
 // other thread groups of the workgroup read the same values)
-template <typename T, int NQ, bool BK, typename G = T, bool DL = false, bool NTG = true>
+// EARLY: issue each slab's geometric-factor loads before the slab's
+// barriers, so they are in flight while the slab goes through LDS
+template <typename T, int NQ, bool BK, typename G = T, bool DL = false, bool NTG = true, bool EARLY = false>
 __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&rq)[NQ], T (&out)[NQ],
                                               const T* __restrict__ s_D, T* s_q, T* s_Gr, T* s_Gs,
                                               const G* __restrict__ g, const G* __restrict__ l0,
@@ -94,6 +96,17 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
   for (int k = 0; k < NQ; ++k) rAt[k] = T(0);
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
+    const int n = k * NQ2;
+    T Gk[6];
+    auto load_g = [&]() {
+      Gk[0] = ldgc<T, G, NTG>(&g[kG00 * NP + n]);
+      Gk[1] = ldgc<T, G, NTG>(&g[kG01 * NP + n]);
+      Gk[2] = ldgc<T, G, NTG>(&g[kG02 * NP + n]);
+      Gk[3] = ldgc<T, G, NTG>(&g[kG11 * NP + n]);
+      Gk[4] = ldgc<T, G, NTG>(&g[kG12 * NP + n]);
+      Gk[5] = ldgc<T, G, NTG>(&g[kG22 * NP + n]);
+    };
+    if (EARLY) load_g();
     __syncthreads();
     s_q[L.t] = rq[k];
     __syncthreads();
@@ -104,10 +117,8 @@ __device__ __forceinline__ void axhelm_column(const AxLane<T, NQ>& L, const T (&
       qs = fmaT(DL ? s_D[L.j * NQ + m] : L.Dj[m], s_q[m * NQ + L.i], qs);
       qt = fmaT(s_D[k * NQ + m], rq[m], qt);
     }
-    const int n = k * NQ2;
-    const T G00 = ldgc<T, G, NTG>(&g[kG00 * NP + n]), G01 = ldgc<T, G, NTG>(&g[kG01 * NP + n]);
-    const T G02 = ldgc<T, G, NTG>(&g[kG02 * NP + n]), G11 = ldgc<T, G, NTG>(&g[kG11 * NP + n]);
-    const T G12 = ldgc<T, G, NTG>(&g[kG12 * NP + n]), G22 = ldgc<T, G, NTG>(&g[kG22 * NP + n]);
+    if (!EARLY) load_g();
+    const T G00 = Gk[0], G01 = Gk[1], G02 = Gk[2], G11 = Gk[3], G12 = Gk[4], G22 = Gk[5];
     T Gr = fmaT(G02, qt, fmaT(G01, qs, G00 * qr));
     T Gs = fmaT(G12, qt, fmaT(G11, qs, G01 * qr));
     T Gt = fmaT(G22, qt, fmaT(G12, qs, G02 * qr));
@@ -172,6 +183,16 @@ template <typename T, int NQ, int NF, bool BK>
 struct AxUseDL {
   static constexpr bool value = sizeof(T) == 8 && (AxDL<NQ, NF, BK>::value || AxGroups<NQ, NF, BK>::fields);
 };
+// Early geometric-factor loads (axhelm_column's EARLY), measured on 32^3,
+// N = 7, one field: the FP64 BK PCG-form operator 230 -> 219 us (and no
+// spill where it had 12 bytes), the FP32 operator 83.7 -> 82.3 us; bf16
+// geometry got slower (63.2 -> 64.4 us), the stand-alone FP64 kernel spilled
+// 124 bytes and the FP64 non-BK PCG form grew from 168 to 256 VGPRs (two
+// waves per SIMD instead of three), so those keep the loads after the barriers.
+template <typename T, int NQ, int NF, bool BK, typename G, bool PCG>
+struct AxEarly {
+  static constexpr bool value = NQ == 8 && NF == 1 && sizeof(G) != 2 && (sizeof(T) == 4 || (PCG && BK));
+};
 template <typename T, int NQ, int NF, bool BK>
 struct AxMinWaves {
   static constexpr int value =
@@ -213,7 +234,7 @@ __global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK, G>::value), (AxMinW
     T rq[NQ], out[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) rq[k] = ldnt(&q[base + k * NQ2]);
-    axhelm_column<T, NQ, BK, G, AxUseDL<T, NQ, NF, BK>::value, !FG>(L, rq, out, s_D, s_q[slot], s_Gr[slot],
+    axhelm_column<T, NQ, BK, G, AxUseDL<T, NQ, NF, BK>::value, !FG, AxEarly<T, NQ, NF, BK, G, false>::value>(L, rq, out, s_D, s_q[slot], s_Gr[slot],
                                                                      s_Gs[slot], g, l0, l1);
     if (active) {
 #pragma unroll
@@ -293,7 +314,7 @@ __global__ __launch_bounds__((NQ * NQ * AxGroups<NQ, NF, BK, G>::value), (AxMinW
 #pragma unroll
       for (int k = 0; k < NQ; ++k) p[base + k * NQ2] = pk[k];
     }
-    axhelm_column<T, NQ, BK, G, AxUseDL<T, NQ, NF, BK>::value, !FG>(L, rq, out, s_D, s_q[slot], s_Gr[slot],
+    axhelm_column<T, NQ, BK, G, AxUseDL<T, NQ, NF, BK>::value, !FG, AxEarly<T, NQ, NF, BK, G, true>::value>(L, rq, out, s_D, s_q[slot], s_Gr[slot],
                                                                      s_Gs[slot], g, l0, l1);
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
