@@ -48,10 +48,20 @@ __global__ __launch_bounds__(128, 8) void k_axhelm_bk8_split(const int32_t Nelem
   const int o = H * w * NQ2 + t;
 #pragma unroll
   for (int kk = 0; kk < H; ++kk) s_qt[o + kk * NQ2] = ldnt(&qe[o + kk * NQ2]);
-  __syncthreads();
   const double* g = ggeo + e * (int64_t)(kNggeo * NP);
   const double *g00 = g + kG00 * NP, *g01 = g + kG01 * NP, *g02 = g + kG02 * NP;
   const double *g11 = g + kG11 * NP, *g12 = g + kG12 * NP, *g22 = g + kG22 * NP;
+  // the first slab's geometric factors are in flight across the barrier and
+  // its derivatives (few registers are live here; later slabs load theirs
+  // after the derivatives, inside the 64-register budget)
+  double G0[6];
+  G0[0] = ldnt(&g00[o]);
+  G0[1] = ldnt(&g01[o]);
+  G0[2] = ldnt(&g02[o]);
+  G0[3] = ldnt(&g11[o]);
+  G0[4] = ldnt(&g12[o]);
+  G0[5] = ldnt(&g22[o]);
+  __syncthreads();
   double Gt[H], ars[H];
 #pragma unroll
   for (int kk = 0; kk < H; ++kk) {
@@ -65,8 +75,9 @@ __global__ __launch_bounds__(128, 8) void k_axhelm_bk8_split(const int32_t Nelem
       if (m % 2 == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the reads the scheduler batches
     }
     const int n = o + kk * NQ2;
-    const double G00 = ldnt(&g00[n]), G01 = ldnt(&g01[n]), G02 = ldnt(&g02[n]);
-    const double G11 = ldnt(&g11[n]), G12 = ldnt(&g12[n]), G22 = ldnt(&g22[n]);
+    const double G00 = kk == 0 ? G0[0] : ldnt(&g00[n]), G01 = kk == 0 ? G0[1] : ldnt(&g01[n]);
+    const double G02 = kk == 0 ? G0[2] : ldnt(&g02[n]), G11 = kk == 0 ? G0[3] : ldnt(&g11[n]);
+    const double G12 = kk == 0 ? G0[4] : ldnt(&g12[n]), G22 = kk == 0 ? G0[5] : ldnt(&g22[n]);
     double* sr = s_Gr[w][kk & 1];
     double* ss = s_Gs[w][kk & 1];
     sr[t] = __builtin_fma(G02, qt, __builtin_fma(G01, qs, G00 * qr));
