@@ -15,6 +15,10 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
+# The update pass has a one-field path for full 2048-node tiles besides its
+# general loop; the one-field cases cover local node counts with only full
+# tiles (case1: 6144), none (case0: 1152, case9: 512) and both (case2: 2592,
+# case5: 5832), with the copy, Jacobi and flexible variants.
 CASES = [
     # (N, dims, Nf, bpmode, lambda1, precon, flexible, periodic, rhs_kind)
     (3, (3, 3, 2), 1, False, 0.0, 1, False, 0, 0),     # Jacobi Poisson, Dirichlet
