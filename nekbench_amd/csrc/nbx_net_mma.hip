@@ -327,8 +327,7 @@ __global__ __launch_bounds__(512) void k_net_apply_mma(const int32_t E, const La
     di = di_n;
     wraw = wraw_n;
     dg = dg_n;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) fe[q] = fe_n[q];
+    __builtin_memcpy(fe, fe_n, sizeof fe);
   }
 }
 
